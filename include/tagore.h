@@ -1,0 +1,127 @@
+/* tagore.h -- C ABI of the MI355X (gfx950) Tagore library, libtagore.so.
+ *
+ * The reference exposes its path only through a pybind11 module (pybind.cu:393-399):
+ * GNN_descent, Pruning, Pruning_with_CFS and largeIndex, which allocate device buffers,
+ * launch the CUDA kernels and hand raw device pointers back to Python.  This header is the
+ * boundary underneath an equivalent module: plain pointers and sizes, an opaque handle that
+ * owns every device buffer, integer status codes (0 = success) instead of exit(), and a
+ * thread-local message for the last failure.  The pybind11 module tagore_amd/Tagore*.so is
+ * a thin layer over these calls; ctypes/cffi callers bind them directly.
+ *
+ * Threading: a handle is used by one host thread at a time; calls are synchronous (they
+ * return after the device work completes) and run on the handle's own HIP stream.
+ */
+#ifndef TAGORE_H
+#define TAGORE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tg_state tg_state;
+
+/* status codes */
+#define TG_OK 0
+#define TG_ERR_ARG 1      /* invalid argument or combination */
+#define TG_ERR_IO 2       /* file missing, short or unwritable */
+#define TG_ERR_HIP 3      /* HIP runtime failure */
+#define TG_ERR_STATE 4    /* call out of order (e.g. prune before a kNN graph exists) */
+#define TG_ERR_CAPACITY 5 /* an internal bounded buffer overflowed */
+
+/* kNN distance precision (the screen; graph decisions are always canonical FP32) */
+#define TG_PREC_EXACT 0
+#define TG_PREC_FP32 1
+#define TG_PREC_BF16 2
+
+/* pruning index kinds (Pruning: NSG/Vamana -> PATH, NSSG -> TWOHOP, CAGRA, DPG) */
+#define TG_KIND_PATH 0
+#define TG_KIND_TWOHOP 1
+#define TG_KIND_CAGRA 2
+#define TG_KIND_DPG 3
+/* rules */
+#define TG_RULE_DIST 0
+#define TG_RULE_ANGLE 1
+#define TG_RULE_RANK 2
+
+typedef struct {
+  uint32_t K;         /* neighbours per node */
+  uint32_t iters;     /* iterations; iters/2 local-join + iters/2 expansion (pybind.cu:101) */
+  uint64_t seed;      /* random initial lists */
+  int32_t precision;  /* TG_PREC_* */
+} tg_knn_params;
+
+typedef struct {
+  int32_t kind;      /* TG_KIND_* */
+  int32_t fwd_rule;  /* rule of the Filter after Collect (path: DIST, two-hop: ANGLE) */
+  int32_t rev_rule;  /* rule of the reverse phase (DIST/ANGLE) or TG_RULE_RANK */
+  uint32_t FD;       /* FINAL_DEGREE */
+  uint32_t TOPM;     /* beam width m of the path Collect */
+  float thre;        /* threshold: alpha for dist, cos(angle) for angle */
+} tg_prune_params;
+
+/* Last error message of the calling thread (never NULL). */
+const char* tg_last_error(void);
+
+/* Handle with room for N vectors of dimension D (D % 4 == 0) on HIP device `device`.
+ * Replaces the cudaMalloc block of GNN_descent (pybind.cu:64-93). */
+int tg_create(uint32_t N, uint32_t D, int device, tg_state** out);
+int tg_free(tg_state* s);
+
+/* Vectors: upload host FP32 rows [N*D], or generate a seeded synthetic set on the device
+ * (kind 0: integer-valued clipped Gaussian mixture in [0,255] with sigma; 1: unit-norm
+ * Gaussian mixture; 2: |Gaussian mixture| scaled to [0,1]). */
+int tg_set_data(tg_state* s, const float* host_rows);
+int tg_gen_data(tg_state* s, int32_t kind, uint32_t ncomp, uint64_t seed, float sigma);
+int tg_get_data(tg_state* s, float* host_rows, uint32_t first, uint32_t count);
+
+/* .fvecs reader (common.cu:6-22): *rows is malloc'ed, free with tg_host_free. */
+int tg_read_fvecs(const char* path, float** rows, uint32_t* N, uint32_t* D);
+void tg_host_free(void* p);
+
+/* kNN graph.  tg_knn_build runs GNN-Descent (GNN_descent, pybind.cu:19-142); tg_set_knn
+ * installs a given graph (rows sorted by (distance, id)); tg_get_knn copies it back. */
+int tg_knn_build(tg_state* s, const tg_knn_params* p);
+int tg_set_knn(tg_state* s, uint32_t K, const uint32_t* ids, const float* dists);
+int tg_get_knn(tg_state* s, uint32_t* ids, float* dists);
+
+/* CFS pruning of the current kNN graph (Pruning / Pruning_with_CFS, pybind.cu:145-295).
+ * The kNN graph is left intact, so several prunings may follow one build. */
+int tg_prune(tg_state* s, const tg_prune_params* p);
+/* Pruned adjacency: deg[N], ids[N*FD] (row stride FD), entry point (path kinds). */
+int tg_get_pruned(tg_state* s, uint32_t* deg, uint32_t* ids, uint32_t* ep);
+/* Forward-phase adjacency (after Collect+Filter, before the reverse phase). */
+int tg_get_forward(tg_state* s, uint32_t* deg, uint32_t* ids, float* dists);
+/* Index file of the last prune: "NSG", "Vamana", "NSSG", "DPG", "CAGRA"
+ * (StoreNSG..StoreCAGRA, common.cu:151-248). */
+int tg_store_index(tg_state* s, const char* path, const char* index_type, uint64_t nssg_seed);
+
+/* FP64 recall@k of the kNN graph over nq seeded sample nodes (tie at the k-th distance
+ * counts as a hit).  Queries: the first nq of a seeded permutation prefix. */
+int tg_recall(tg_state* s, uint32_t nq, uint64_t seed, uint32_t k, double* recall);
+
+/* Device buffers in the reference's capsule order (pybind.cu:132-141):
+ * 0 graph u32[N*K], 1 reverse graph, 2 data f32[N*D], 3 distances f32[N*K],
+ * 4 reverse distances, 5 reverse counts, 6 centroid f32[D], 7 sample lists u32[N*128]. */
+int tg_buffer(tg_state* s, int32_t idx, void** dev_ptr, size_t* bytes);
+
+/* Timing of the last tg_knn_build / tg_prune (device window, HIP events, ms) and the
+ * summed duration of the dominant kernel (ms) and its launch count. */
+typedef struct {
+  double knn_ms, prune_ms;
+  double knn_kernel_ms[8];   /* init, sample, rsample, localjoin, merge, reverse, phase2, final */
+  double prune_kernel_ms[4]; /* ep, forward, reverse, other */
+  uint64_t knn_evals;        /* distance evaluations of the last build */
+  uint64_t prune_counters[8];/* collect, filter, reverse, adjacency rows, ep evaluations */
+} tg_stats;
+int tg_get_stats(tg_state* s, tg_stats* out);
+/* 1: time each kernel with events (adds a little overhead), 0: only the windows. */
+int tg_set_profiling(tg_state* s, int on);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TAGORE_H */

@@ -1,0 +1,265 @@
+"""tagore_amd -- MI355X-native GNN-Descent kNN construction and CFS pruning.
+
+The product is the HIP library libtagore.so (C ABI in include/tagore.h) plus the drop-in
+pybind11 module `Tagore` built next to it.  This package loads both:
+
+    from tagore_amd import Tagore          # the reference's Python surface
+    from tagore_amd import State           # ctypes wrapper of the C ABI (tests, bench)
+
+Nothing here falls back to CPU code: if the HIP library has not been built, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import importlib.util
+import os
+import sysconfig
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtagore.so")
+INCLUDE = os.path.join(os.path.dirname(_HERE), "include", "tagore.h")
+
+TG_OK, TG_ERR_ARG, TG_ERR_IO, TG_ERR_HIP, TG_ERR_STATE, TG_ERR_CAPACITY = range(6)
+PREC = {"exact": 0, "fp32": 1, "bf16": 2}
+KIND = {"path": 0, "2-hop": 1, "cagra": 2, "dpg": 3}
+RULE = {"dist": 0, "angle": 1, "rank": 2}
+
+# index types of Pruning(): (kind, fwd_rule, rev_rule, fixed threshold or None)
+INDEX_TYPES = {
+    "NSG": (0, 0, 0, 1.0),
+    "Vamana": (0, 0, 0, None),
+    "NSSG": (1, 1, 1, None),
+    "CAGRA": (2, 0, 2, None),
+    "DPG": (3, 0, 2, None),
+}
+
+
+class TagoreError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[tagore status {code}] {msg}")
+        self.code = code
+
+
+class KnnParams(ctypes.Structure):
+    _fields_ = [("K", ctypes.c_uint32), ("iters", ctypes.c_uint32), ("seed", ctypes.c_uint64),
+                ("precision", ctypes.c_int32)]
+
+
+class PruneParams(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("fwd_rule", ctypes.c_int32),
+                ("rev_rule", ctypes.c_int32), ("FD", ctypes.c_uint32),
+                ("TOPM", ctypes.c_uint32), ("thre", ctypes.c_float)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("knn_ms", ctypes.c_double), ("prune_ms", ctypes.c_double),
+                ("knn_kernel_ms", ctypes.c_double * 8), ("prune_kernel_ms", ctypes.c_double * 4),
+                ("knn_evals", ctypes.c_uint64), ("prune_counters", ctypes.c_uint64 * 8)]
+
+
+KNN_SECTIONS = ["init", "sample", "rsample", "localjoin", "merge", "reverse", "phase2", "final"]
+PRUNE_SECTIONS = ["ep", "forward", "reverse", "group"]
+
+_lib = None
+
+
+def exported_symbols():
+    """Function names declared in include/tagore.h."""
+    import re
+
+    txt = open(INCLUDE).read()
+    return sorted(set(re.findall(r"\b(tg_[a-z_0-9]+)\s*\(", txt)))
+
+
+def lib():
+    """The loaded libtagore.so with argument types set.  Raises if it was never built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m tagore_amd.build` "
+                          "(there is no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    P, vp, c = ctypes.POINTER, ctypes.c_void_p, ctypes
+    L.tg_last_error.restype = c.c_char_p
+    L.tg_create.argtypes = [c.c_uint32, c.c_uint32, c.c_int, P(vp)]
+    L.tg_free.argtypes = [vp]
+    L.tg_set_data.argtypes = [vp, vp]
+    L.tg_gen_data.argtypes = [vp, c.c_int32, c.c_uint32, c.c_uint64, c.c_float]
+    L.tg_get_data.argtypes = [vp, vp, c.c_uint32, c.c_uint32]
+    L.tg_read_fvecs.argtypes = [c.c_char_p, P(vp), P(c.c_uint32), P(c.c_uint32)]
+    L.tg_host_free.argtypes = [vp]
+    L.tg_host_free.restype = None
+    L.tg_knn_build.argtypes = [vp, P(KnnParams)]
+    L.tg_set_knn.argtypes = [vp, c.c_uint32, vp, vp]
+    L.tg_get_knn.argtypes = [vp, vp, vp]
+    L.tg_prune.argtypes = [vp, P(PruneParams)]
+    L.tg_get_pruned.argtypes = [vp, vp, vp, P(c.c_uint32)]
+    L.tg_get_forward.argtypes = [vp, vp, vp, vp]
+    L.tg_store_index.argtypes = [vp, c.c_char_p, c.c_char_p, c.c_uint64]
+    L.tg_recall.argtypes = [vp, c.c_uint32, c.c_uint64, c.c_uint32, P(c.c_double)]
+    L.tg_buffer.argtypes = [vp, c.c_int32, P(vp), P(c.c_size_t)]
+    L.tg_get_stats.argtypes = [vp, P(Stats)]
+    L.tg_set_profiling.argtypes = [vp, c.c_int]
+    for name in exported_symbols():
+        f = getattr(L, name)
+        if name not in ("tg_last_error", "tg_host_free"):
+            f.restype = c.c_int
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != TG_OK:
+        raise TagoreError(rc, lib().tg_last_error().decode(errors="replace"))
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class State:
+    """Owner of one device handle (tg_state) -- N vectors of dimension D on one GPU."""
+
+    def __init__(self, N: int, D: int, device: int = 0):
+        self._h = ctypes.c_void_p()
+        _check(lib().tg_create(N, D, device, ctypes.byref(self._h)))
+        self.N, self.D = N, D
+        self.K = None
+        self.FD = None
+
+    def close(self):
+        if self._h:
+            lib().tg_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # data
+    def set_data(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        assert X.shape == (self.N, self.D)
+        _check(lib().tg_set_data(self._h, _ptr(X)))
+
+    def gen_data(self, kind: int, ncomp: int, seed: int, sigma: float = 20.0):
+        _check(lib().tg_gen_data(self._h, kind, ncomp, seed, sigma))
+
+    def get_data(self, first: int = 0, count: int | None = None):
+        count = self.N - first if count is None else count
+        out = np.empty((count, self.D), np.float32)
+        _check(lib().tg_get_data(self._h, _ptr(out), first, count))
+        return out
+
+    # kNN
+    def knn_build(self, K: int, iters: int, seed: int = 0x7A600000, precision: str = "exact"):
+        p = KnnParams(K, iters, seed, PREC[precision])
+        _check(lib().tg_knn_build(self._h, ctypes.byref(p)))
+        self.K = K
+
+    def set_knn(self, ids, dists):
+        ids = np.ascontiguousarray(ids, np.uint32)
+        dists = np.ascontiguousarray(dists, np.float32)
+        _check(lib().tg_set_knn(self._h, ids.shape[1], _ptr(ids), _ptr(dists)))
+        self.K = ids.shape[1]
+
+    def get_knn(self):
+        ids = np.empty((self.N, self.K), np.uint32)
+        d = np.empty((self.N, self.K), np.float32)
+        _check(lib().tg_get_knn(self._h, _ptr(ids), _ptr(d)))
+        return ids, d
+
+    # pruning
+    def prune(self, kind, fwd_rule, rev_rule, FD: int, TOPM: int = 64, thre: float = 1.0):
+        kind = KIND[kind] if isinstance(kind, str) else kind
+        fwd_rule = RULE[fwd_rule] if isinstance(fwd_rule, str) else fwd_rule
+        rev_rule = RULE[rev_rule] if isinstance(rev_rule, str) else rev_rule
+        p = PruneParams(kind, fwd_rule, rev_rule, FD, TOPM, thre)
+        _check(lib().tg_prune(self._h, ctypes.byref(p)))
+        self.FD = FD
+
+    def get_pruned(self):
+        deg = np.empty(self.N, np.uint32)
+        ids = np.empty((self.N, self.FD), np.uint32)
+        ep = ctypes.c_uint32()
+        _check(lib().tg_get_pruned(self._h, _ptr(deg), _ptr(ids), ctypes.byref(ep)))
+        return deg, ids, int(ep.value)
+
+    def get_forward(self):
+        deg = np.empty(self.N, np.uint32)
+        ids = np.empty((self.N, self.FD), np.uint32)
+        d = np.empty((self.N, self.FD), np.float32)
+        _check(lib().tg_get_forward(self._h, _ptr(deg), _ptr(ids), _ptr(d)))
+        return deg, ids, d
+
+    def store_index(self, path: str, index_type: str, nssg_seed: int = 0x5EED):
+        _check(lib().tg_store_index(self._h, path.encode(), index_type.encode(), nssg_seed))
+
+    def recall(self, nq: int, k: int = 10, seed: int = 0x5A11):
+        r = ctypes.c_double()
+        _check(lib().tg_recall(self._h, nq, seed, k, ctypes.byref(r)))
+        return r.value
+
+    def stats(self):
+        s = Stats()
+        _check(lib().tg_get_stats(self._h, ctypes.byref(s)))
+        return dict(knn_ms=s.knn_ms, prune_ms=s.prune_ms,
+                    knn_kernel_ms=dict(zip(KNN_SECTIONS, list(s.knn_kernel_ms))),
+                    prune_kernel_ms=dict(zip(PRUNE_SECTIONS, list(s.prune_kernel_ms))),
+                    knn_evals=int(s.knn_evals),
+                    prune_counters=dict(zip(["collect", "filter", "reverse", "adj_rows", "ep"],
+                                            list(s.prune_counters)[:5])))
+
+    def set_profiling(self, on: bool):
+        _check(lib().tg_set_profiling(self._h, 1 if on else 0))
+
+    def buffer(self, idx: int):
+        p, n = ctypes.c_void_p(), ctypes.c_size_t()
+        _check(lib().tg_buffer(self._h, idx, ctypes.byref(p), ctypes.byref(n)))
+        return p.value, n.value
+
+
+def nssg_entry_points(N: int, seed: int):
+    """The 10 NSSG entry ids tg_store_index writes (SplitMix64 Fisher-Yates prefix)."""
+    perm = list(range(N))
+    x = seed & 0xFFFFFFFFFFFFFFFF
+    M = 0xFFFFFFFFFFFFFFFF
+    for i in range(min(10, N)):
+        x = (x + 0x9E3779B97F4A7C15) & M
+        z = x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        z ^= z >> 31
+        j = i + z % (N - i)
+        perm[i], perm[j] = perm[j], perm[i]
+    return (perm[:10] + [0] * 10)[:10]
+
+
+def _load_module():
+    path = os.path.join(_HERE, "Tagore" + sysconfig.get_config_var("EXT_SUFFIX"))
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it with `python -m tagore_amd.build`")
+    lib()  # the module links libtagore.so; load it first so a missing build fails here
+    spec = importlib.util.spec_from_file_location("Tagore", path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def __getattr__(name):
+    if name == "Tagore":
+        mod = _load_module()
+        globals()["Tagore"] = mod
+        return mod
+    raise AttributeError(name)

@@ -1,0 +1,835 @@
+// api.hip -- host orchestration and the C ABI declared in include/tagore.h.
+//
+// Owns every device buffer of a handle, drives the GNN-Descent schedule (pybind.cu:19-142)
+// and the CFS dispatch (pybind.cu:145-295) on one HIP stream, times both with HIP events,
+// and writes the index files (common.cu:151-248).  Errors are returned as status codes with
+// a thread-local message; nothing here calls exit().
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <initializer_list>
+#include <string>
+#include <vector>
+
+#include "../../include/tagore.h"
+#include "kernels.hpp"
+
+using namespace tg;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPX(expr)                                                                   \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return fail(TG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+  } while (0)
+
+template <class T>
+int dalloc(T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(p), n * sizeof(T));
+  if (e != hipSuccess)
+    return fail(TG_ERR_HIP, "hipMalloc of " + std::to_string(n * sizeof(T)) +
+                                " bytes: " + hipGetErrorString(e));
+  return TG_OK;
+}
+template <class T>
+void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+struct PB {
+  PushBuf d{};
+  uint32_t N = 0;
+  bool alloc(uint32_t n, uint32_t cap, size_t ov_cap) {
+    N = n;
+    d.cap = cap;
+    d.ov_cap = uint32_t(std::min<size_t>(ov_cap, 0xFFFFFFF0u));
+    if (dalloc(&d.cnt, n) || dalloc(&d.slots, size_t(n) * cap) || dalloc(&d.ov_dest, d.ov_cap) ||
+        dalloc(&d.ov_key, d.ov_cap) || dalloc(&d.ov_used, 1) || dalloc(&d.ov_off, size_t(n) + 1) ||
+        dalloc(&d.ov_sorted, d.ov_cap) || dalloc(&d.err, 1))
+      return false;
+    return true;
+  }
+  void release() {
+    dfree(d.cnt);
+    dfree(d.slots);
+    dfree(d.ov_dest);
+    dfree(d.ov_key);
+    dfree(d.ov_used);
+    dfree(d.ov_off);
+    dfree(d.ov_sorted);
+    dfree(d.err);
+  }
+};
+
+}  // namespace
+
+struct tg_state {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint32_t N = 0, D = 0;
+  float* X = nullptr;
+  float* centres = nullptr;  // synthetic-data centres
+  uint32_t ncomp_alloc = 0;
+  uint32_t* misc = nullptr;  // [64] small scratch
+  // kNN
+  uint32_t K = 0;
+  bool has_knn = false;
+  uint32_t* G = nullptr;
+  float* GD = nullptr;
+  uint8_t* V = nullptr;
+  uint32_t K_scratch = 0;
+  uint32_t *G1 = nullptr, *H = nullptr, *na = nullptr, *nb = nullptr, *nnew = nullptr,
+           *nold = nullptr;
+  float* GD1 = nullptr;
+  uint64_t *R0 = nullptr, *R1 = nullptr;
+  uint32_t *Rn0 = nullptr, *Rn1 = nullptr;
+  PB pn, po, pc, pr;
+  uint32_t* fill = nullptr;
+  uint32_t* scan_tmp = nullptr;
+  unsigned long long* counters = nullptr;  // [16]
+  // prune
+  uint32_t FD_alloc = 0;
+  uint32_t *fwd_deg = nullptr, *fwd_ids = nullptr, *out_deg = nullptr, *out_ids = nullptr;
+  float* fwd_d = nullptr;
+  float* centroid = nullptr;
+  double* cpart = nullptr;
+  uint32_t* ep_dev = nullptr;
+  PB rev;
+  bool has_prune = false;
+  tg_prune_params last_prune{};
+  uint32_t ep = 0;
+  // stats
+  bool profiling = false;
+  tg_stats stats{};
+  std::vector<hipEvent_t> ev_pool;
+};
+
+namespace {
+
+int ensure_events(tg_state* s, size_t n) {
+  while (s->ev_pool.size() < n) {
+    hipEvent_t e;
+    HIPX(hipEventCreate(&e));
+    s->ev_pool.push_back(e);
+  }
+  return TG_OK;
+}
+
+int check_pb_err(tg_state* s, PB& pb, const char* what) {
+  uint32_t e = 0;
+  HIPX(hipMemcpyAsync(&e, pb.d.err, 4, hipMemcpyDeviceToHost, s->stream));
+  HIPX(hipStreamSynchronize(s->stream));
+  if (e)
+    return fail(TG_ERR_CAPACITY, std::string("push-buffer overflow area exhausted in ") + what);
+  return TG_OK;
+}
+
+int alloc_pb(tg_state* s, PB& pb, uint32_t cap, size_t ov_cap) {
+  if (pb.d.cnt && pb.d.cap == cap && pb.d.ov_cap >= std::min<size_t>(ov_cap, 0xFFFFFFF0u) &&
+      pb.N == s->N)
+    return TG_OK;
+  pb.release();
+  if (!pb.alloc(s->N, cap, ov_cap)) return TG_ERR_HIP;
+  fill_u32(pb.d.err, 0, 1, s->stream);
+  return TG_OK;
+}
+
+void clear_errors(tg_state* s, std::initializer_list<PB*> pbs) {
+  for (PB* pb : pbs)
+    if (pb->d.err) fill_u32(pb->d.err, 0, 1, s->stream);
+}
+
+int ensure_common_scratch(tg_state* s) {
+  if (!s->fill) {
+    if (dalloc(&s->fill, s->N) || dalloc(&s->scan_tmp, scan_tmp_words(s->N) + 16) ||
+        dalloc(&s->counters, 16))
+      return TG_ERR_HIP;
+  }
+  return TG_OK;
+}
+
+int ensure_knn_scratch(tg_state* s, uint32_t K) {
+  const size_t N = s->N;
+  if (s->K_scratch != K) {
+    dfree(s->G1);
+    dfree(s->GD1);
+    dfree(s->R0);
+    dfree(s->R1);
+    dfree(s->Rn0);
+    dfree(s->Rn1);
+    if (dalloc(&s->G1, N * K) || dalloc(&s->GD1, N * K) || dalloc(&s->R0, N * K) ||
+        dalloc(&s->R1, N * K) || dalloc(&s->Rn0, N) || dalloc(&s->Rn1, N))
+      return TG_ERR_HIP;
+    s->K_scratch = K;
+  }
+  if (!s->H) {
+    if (dalloc(&s->H, N * 128) || dalloc(&s->na, N) || dalloc(&s->nb, N) ||
+        dalloc(&s->nnew, N) || dalloc(&s->nold, N))
+      return TG_ERR_HIP;
+  }
+  if (ensure_common_scratch(s)) return TG_ERR_HIP;
+  if (alloc_pb(s, s->pn, 64, N * 32) || alloc_pb(s, s->po, 64, N * 32) ||
+      alloc_pb(s, s->pc, 256, N * 256) || alloc_pb(s, s->pr, 128, N * size_t(K)))
+    return TG_ERR_HIP;
+  return TG_OK;
+}
+
+int ensure_graph(tg_state* s, uint32_t K) {
+  if (s->K != K || !s->G) {
+    dfree(s->G);
+    dfree(s->GD);
+    dfree(s->V);
+    if (dalloc(&s->G, size_t(s->N) * K) || dalloc(&s->GD, size_t(s->N) * K) ||
+        dalloc(&s->V, size_t(s->N) * K))
+      return TG_ERR_HIP;
+    s->K = K;
+  }
+  return TG_OK;
+}
+
+KnnArgs knn_args(tg_state* s, uint64_t seed) {
+  KnnArgs a{};
+  a.X = s->X;
+  a.N = s->N;
+  a.D = s->D;
+  a.K = s->K;
+  a.G = s->G;
+  a.GD = s->GD;
+  a.V = s->V;
+  a.G1 = s->G1;
+  a.GD1 = s->GD1;
+  a.H = s->H;
+  a.na = s->na;
+  a.nb = s->nb;
+  a.nnew = s->nnew;
+  a.nold = s->nold;
+  a.pn = s->pn.d;
+  a.po = s->po.d;
+  a.pc = s->pc.d;
+  a.pr = s->pr.d;
+  a.R0 = s->R0;
+  a.Rn0 = s->Rn0;
+  a.R1 = s->R1;
+  a.Rn1 = s->Rn1;
+  a.seed = seed;
+  a.counters = s->counters;
+  return a;
+}
+
+// event-timed section helper
+struct Sections {
+  tg_state* s;
+  std::vector<std::pair<int, size_t>> rec;  // (slot, event index of start)
+  size_t next = 0;
+  int start(int slot) {
+    if (!s->profiling) return TG_OK;
+    if (ensure_events(s, next + 2)) return TG_ERR_HIP;
+    HIPX(hipEventRecord(s->ev_pool[next], s->stream));
+    rec.push_back({slot, next});
+    next += 2;
+    return TG_OK;
+  }
+  int stop() {
+    if (!s->profiling) return TG_OK;
+    HIPX(hipEventRecord(s->ev_pool[rec.back().second + 1], s->stream));
+    return TG_OK;
+  }
+  int collect(double* out) {
+    for (auto& r : rec) {
+      float ms = 0.f;
+      HIPX(hipEventElapsedTime(&ms, s->ev_pool[r.second], s->ev_pool[r.second + 1]));
+      out[r.first] += ms;
+    }
+    return TG_OK;
+  }
+};
+
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(TG_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  return TG_OK;
+}
+
+uint64_t splitmix64(uint64_t& x) {
+  uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+}  // namespace
+
+// ===================================================================== C ABI
+extern "C" {
+
+const char* tg_last_error(void) { return g_err.c_str(); }
+
+int tg_create(uint32_t N, uint32_t D, int device, tg_state** out) {
+  if (!out) return fail(TG_ERR_ARG, "out is null");
+  *out = nullptr;
+  if (N < 2) return fail(TG_ERR_ARG, "N must be >= 2");
+  if (D == 0 || D % 4 != 0 || D > 1024)
+    return fail(TG_ERR_ARG, "D must be a multiple of 4 in [4, 1024]");
+  int ndev = 0;
+  HIPX(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(TG_ERR_ARG, "no such HIP device");
+  HIPX(hipSetDevice(device));
+  tg_state* s = new tg_state();
+  s->device = device;
+  s->N = N;
+  s->D = D;
+  hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete s;
+    return fail(TG_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  if (dalloc(&s->X, size_t(N) * D) || dalloc(&s->misc, 64) || dalloc(&s->centroid, D) ||
+      dalloc(&s->ep_dev, 1)) {
+    std::string m = g_err;
+    tg_free(s);
+    return fail(TG_ERR_HIP, m);
+  }
+  *out = s;
+  return TG_OK;
+}
+
+int tg_free(tg_state* s) {
+  if (!s) return TG_OK;
+  (void)hipSetDevice(s->device);
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  dfree(s->X);
+  dfree(s->centres);
+  dfree(s->misc);
+  dfree(s->G);
+  dfree(s->GD);
+  dfree(s->V);
+  dfree(s->G1);
+  dfree(s->GD1);
+  dfree(s->H);
+  dfree(s->na);
+  dfree(s->nb);
+  dfree(s->nnew);
+  dfree(s->nold);
+  dfree(s->R0);
+  dfree(s->R1);
+  dfree(s->Rn0);
+  dfree(s->Rn1);
+  s->pn.release();
+  s->po.release();
+  s->pc.release();
+  s->pr.release();
+  s->rev.release();
+  dfree(s->fill);
+  dfree(s->scan_tmp);
+  dfree(s->counters);
+  dfree(s->fwd_deg);
+  dfree(s->fwd_ids);
+  dfree(s->fwd_d);
+  dfree(s->out_deg);
+  dfree(s->out_ids);
+  dfree(s->centroid);
+  dfree(s->cpart);
+  dfree(s->ep_dev);
+  for (auto e : s->ev_pool) (void)hipEventDestroy(e);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return TG_OK;
+}
+
+int tg_set_data(tg_state* s, const float* rows) {
+  if (!s || !rows) return fail(TG_ERR_ARG, "null argument");
+  HIPX(hipSetDevice(s->device));
+  HIPX(hipMemcpyAsync(s->X, rows, size_t(s->N) * s->D * 4, hipMemcpyHostToDevice, s->stream));
+  HIPX(hipStreamSynchronize(s->stream));
+  s->has_knn = false;
+  s->has_prune = false;
+  return TG_OK;
+}
+
+int tg_gen_data(tg_state* s, int32_t kind, uint32_t ncomp, uint64_t seed, float sigma) {
+  if (!s) return fail(TG_ERR_ARG, "null handle");
+  if (kind < 0 || kind > 2 || ncomp == 0) return fail(TG_ERR_ARG, "bad data kind / ncomp");
+  HIPX(hipSetDevice(s->device));
+  if (s->ncomp_alloc < ncomp) {
+    dfree(s->centres);
+    if (dalloc(&s->centres, size_t(ncomp) * s->D)) return TG_ERR_HIP;
+    s->ncomp_alloc = ncomp;
+  }
+  launch_gen_data(kind, s->N, s->D, ncomp, seed, sigma, s->centres, s->X, s->misc, s->stream);
+  if (int rc = check_launch("gen_data")) return rc;
+  HIPX(hipStreamSynchronize(s->stream));
+  s->has_knn = false;
+  s->has_prune = false;
+  return TG_OK;
+}
+
+int tg_get_data(tg_state* s, float* rows, uint32_t first, uint32_t count) {
+  if (!s || !rows) return fail(TG_ERR_ARG, "null argument");
+  if (uint64_t(first) + count > s->N) return fail(TG_ERR_ARG, "range outside the data");
+  HIPX(hipSetDevice(s->device));
+  HIPX(hipMemcpy(rows, s->X + size_t(first) * s->D, size_t(count) * s->D * 4,
+                 hipMemcpyDeviceToHost));
+  return TG_OK;
+}
+
+int tg_read_fvecs(const char* path, float** rows, uint32_t* N, uint32_t* D) {
+  if (!path || !rows || !N || !D) return fail(TG_ERR_ARG, "null argument");
+  *rows = nullptr;
+  FILE* f = fopen(path, "rb");
+  if (!f) return fail(TG_ERR_IO, std::string("cannot open ") + path);
+  uint32_t dim = 0;
+  if (fread(&dim, 4, 1, f) != 1 || dim == 0) {
+    fclose(f);
+    return fail(TG_ERR_IO, std::string("short or empty fvecs file ") + path);
+  }
+  fseek(f, 0, SEEK_END);
+  const long long size = ftell(f);
+  const long long rec = 4ll * (dim + 1);
+  if (size % rec != 0) {
+    fclose(f);
+    return fail(TG_ERR_IO, std::string("fvecs size is not a multiple of the record: ") + path);
+  }
+  const uint64_t n = uint64_t(size / rec);
+  float* out = static_cast<float*>(malloc(n * dim * 4 + 4));
+  if (!out) {
+    fclose(f);
+    return fail(TG_ERR_IO, "out of host memory");
+  }
+  fseek(f, 0, SEEK_SET);
+  for (uint64_t i = 0; i < n; ++i) {
+    uint32_t d2 = 0;
+    if (fread(&d2, 4, 1, f) != 1 || d2 != dim || fread(out + i * dim, 4, dim, f) != dim) {
+      fclose(f);
+      free(out);
+      return fail(TG_ERR_IO, std::string("inconsistent fvecs record in ") + path);
+    }
+  }
+  fclose(f);
+  *rows = out;
+  *N = uint32_t(n);
+  *D = dim;
+  return TG_OK;
+}
+
+void tg_host_free(void* p) { free(p); }
+
+int tg_set_profiling(tg_state* s, int on) {
+  if (!s) return fail(TG_ERR_ARG, "null handle");
+  s->profiling = on != 0;
+  return TG_OK;
+}
+
+int tg_get_stats(tg_state* s, tg_stats* out) {
+  if (!s || !out) return fail(TG_ERR_ARG, "null argument");
+  *out = s->stats;
+  return TG_OK;
+}
+
+// ----------------------------------------------------------------------- kNN build
+int tg_knn_build(tg_state* s, const tg_knn_params* p) {
+  if (!s || !p) return fail(TG_ERR_ARG, "null argument");
+  const uint32_t K = p->K;
+  if (K < 2 || K > 128) return fail(TG_ERR_ARG, "K must be in [2, 128]");
+  if (K >= s->N) return fail(TG_ERR_ARG, "K must be smaller than N");
+  if (p->precision != TG_PREC_EXACT)
+    return fail(TG_ERR_ARG, "only exact precision is available in this build");
+  HIPX(hipSetDevice(s->device));
+  if (ensure_graph(s, K) || ensure_knn_scratch(s, K)) return TG_ERR_HIP;
+  hipStream_t st = s->stream;
+  if (ensure_events(s, 2)) return TG_ERR_HIP;
+  // window events (separate from the per-section pool)
+  hipEvent_t w0, w1;
+  HIPX(hipEventCreate(&w0));
+  HIPX(hipEventCreate(&w1));
+  Sections sec{s};
+  sec.next = 0;
+  fill_u32(reinterpret_cast<uint32_t*>(s->counters), 0, 32, st);
+  clear_errors(s, {&s->pn, &s->po, &s->pc, &s->pr});
+  HIPX(hipEventRecord(w0, st));
+  KnnArgs a = knn_args(s, p->seed);
+  const uint32_t N = s->N;
+  if (sec.start(0)) return TG_ERR_HIP;
+  launch_knn_init(a, st);
+  if (sec.stop()) return TG_ERR_HIP;
+  const uint32_t it1 = p->iters / 2, it2 = p->iters / 2;
+  for (uint32_t it = 0; it < it1; ++it) {
+    if (sec.start(1)) return TG_ERR_HIP;
+    pb_reset(a.pn, N, s->fill, st);
+    pb_reset(a.po, N, s->fill, st);
+    launch_knn_sample(a, st);
+    if (sec.stop()) return TG_ERR_HIP;
+    if (sec.start(2)) return TG_ERR_HIP;
+    pb_group_overflow(a.pn, N, s->fill, s->scan_tmp, st);
+    fill_u32(s->fill, 0, N, st);
+    pb_group_overflow(a.po, N, s->fill, s->scan_tmp, st);
+    launch_knn_rsample(a, st);
+    if (sec.stop()) return TG_ERR_HIP;
+    if (sec.start(3)) return TG_ERR_HIP;
+    pb_reset(a.pc, N, s->fill, st);
+    launch_knn_localjoin(a, p->precision, st);
+    if (sec.stop()) return TG_ERR_HIP;
+    if (sec.start(4)) return TG_ERR_HIP;
+    pb_group_overflow(a.pc, N, s->fill, s->scan_tmp, st);
+    launch_knn_merge(a, st);
+    if (sec.stop()) return TG_ERR_HIP;
+    if (int rc = check_launch("GNN-Descent phase 1")) return rc;
+  }
+  if (sec.start(5)) return TG_ERR_HIP;
+  pb_reset(a.pr, N, s->fill, st);
+  launch_knn_rev_build(a, st);
+  pb_group_overflow(a.pr, N, s->fill, s->scan_tmp, st);
+  launch_knn_rev_reduce(a, st);
+  if (sec.stop()) return TG_ERR_HIP;
+  for (uint32_t it = 0; it < it2; ++it) {
+    if (sec.start(6)) return TG_ERR_HIP;
+    pb_reset(a.pr, N, s->fill, st);
+    launch_knn_phase2(a, st);
+    pb_group_overflow(a.pr, N, s->fill, s->scan_tmp, st);
+    launch_knn_rupdate(a, st);
+    if (sec.stop()) return TG_ERR_HIP;
+    std::swap(s->G, s->G1);
+    std::swap(s->GD, s->GD1);
+    std::swap(s->R0, s->R1);
+    std::swap(s->Rn0, s->Rn1);
+    a = knn_args(s, p->seed);
+    if (int rc = check_launch("GNN-Descent phase 2")) return rc;
+  }
+  if (sec.start(7)) return TG_ERR_HIP;
+  launch_knn_final(a, st);
+  if (sec.stop()) return TG_ERR_HIP;
+  HIPX(hipEventRecord(w1, st));
+  HIPX(hipStreamSynchronize(st));
+  if (int rc = check_launch("GNN-Descent")) return rc;
+  float ms = 0.f;
+  HIPX(hipEventElapsedTime(&ms, w0, w1));
+  (void)hipEventDestroy(w0);
+  (void)hipEventDestroy(w1);
+  s->stats.knn_ms = ms;
+  for (double& v : s->stats.knn_kernel_ms) v = 0.0;
+  if (sec.collect(s->stats.knn_kernel_ms)) return TG_ERR_HIP;
+  unsigned long long cnt[8];
+  HIPX(hipMemcpy(cnt, s->counters, sizeof(cnt), hipMemcpyDeviceToHost));
+  s->stats.knn_evals = cnt[0];
+  if (check_pb_err(s, s->pn, "reverse sampling") || check_pb_err(s, s->po, "reverse sampling") ||
+      check_pb_err(s, s->pc, "local join") || check_pb_err(s, s->pr, "reverse lists"))
+    return TG_ERR_CAPACITY;
+  s->has_knn = true;
+  s->has_prune = false;
+  return TG_OK;
+}
+
+int tg_set_knn(tg_state* s, uint32_t K, const uint32_t* ids, const float* dists) {
+  if (!s || !ids || !dists) return fail(TG_ERR_ARG, "null argument");
+  if (K < 2 || K > 128 || K >= s->N) return fail(TG_ERR_ARG, "K must be in [2, min(128, N-1)]");
+  for (size_t i = 0; i < size_t(s->N) * K; ++i)
+    if (ids[i] >= s->N) return fail(TG_ERR_ARG, "kNN id out of range");
+  HIPX(hipSetDevice(s->device));
+  if (ensure_graph(s, K)) return TG_ERR_HIP;
+  HIPX(hipMemcpy(s->G, ids, size_t(s->N) * K * 4, hipMemcpyHostToDevice));
+  HIPX(hipMemcpy(s->GD, dists, size_t(s->N) * K * 4, hipMemcpyHostToDevice));
+  s->has_knn = true;
+  s->has_prune = false;
+  return TG_OK;
+}
+
+int tg_get_knn(tg_state* s, uint32_t* ids, float* dists) {
+  if (!s) return fail(TG_ERR_ARG, "null handle");
+  if (!s->has_knn) return fail(TG_ERR_STATE, "no kNN graph");
+  HIPX(hipSetDevice(s->device));
+  if (ids) HIPX(hipMemcpy(ids, s->G, size_t(s->N) * s->K * 4, hipMemcpyDeviceToHost));
+  if (dists) HIPX(hipMemcpy(dists, s->GD, size_t(s->N) * s->K * 4, hipMemcpyDeviceToHost));
+  return TG_OK;
+}
+
+// -------------------------------------------------------------------------- prune
+int tg_prune(tg_state* s, const tg_prune_params* p) {
+  if (!s || !p) return fail(TG_ERR_ARG, "null argument");
+  if (!s->has_knn) return fail(TG_ERR_STATE, "prune needs a kNN graph (build or set one)");
+  const uint32_t K = s->K, FD = p->FD;
+  const bool rank = p->kind == TG_KIND_CAGRA || p->kind == TG_KIND_DPG;
+  if (p->kind < 0 || p->kind > 3) return fail(TG_ERR_ARG, "unknown index kind");
+  if (rank != (p->rev_rule == TG_RULE_RANK))
+    return fail(TG_ERR_ARG,
+                "rule/collect combination mixes the [deg,ids] and (id,rank) layouts "
+                "(1-hop needs metric 'rank'; path and 2-hop need 'dist' or 'angle')");
+  if (!rank && (p->fwd_rule < 0 || p->fwd_rule > 1 || p->rev_rule < 0 || p->rev_rule > 1))
+    return fail(TG_ERR_ARG, "unknown rule");
+  if (FD < 1 || FD > 128) return fail(TG_ERR_ARG, "FINAL_DEGREE must be in [1, 128]");
+  if (rank && FD > K) return fail(TG_ERR_ARG, "FINAL_DEGREE must not exceed K for 1-hop");
+  if (p->kind == TG_KIND_PATH && (p->TOPM < 1 || p->TOPM > 128))
+    return fail(TG_ERR_ARG, "TOPM must be in [1, 128]");
+  if (p->kind == TG_KIND_TWOHOP && size_t(K) * K < 480)
+    return fail(TG_ERR_ARG, "2-hop collect needs K*K >= 480");
+  HIPX(hipSetDevice(s->device));
+  const uint32_t N = s->N;
+  if (s->FD_alloc < FD) {
+    dfree(s->fwd_deg);
+    dfree(s->fwd_ids);
+    dfree(s->fwd_d);
+    dfree(s->out_deg);
+    dfree(s->out_ids);
+    if (dalloc(&s->fwd_deg, N) || dalloc(&s->fwd_ids, size_t(N) * FD) ||
+        dalloc(&s->fwd_d, size_t(N) * FD) || dalloc(&s->out_deg, N) ||
+        dalloc(&s->out_ids, size_t(N) * FD))
+      return TG_ERR_HIP;
+    s->FD_alloc = FD;
+  }
+  if (!s->cpart && dalloc(&s->cpart, size_t((N + 1023) / 1024) * s->D)) return TG_ERR_HIP;
+  if (ensure_common_scratch(s)) return TG_ERR_HIP;
+  if (alloc_pb(s, s->rev, 64, size_t(N) * FD)) return TG_ERR_HIP;
+  hipStream_t st = s->stream;
+  PruneArgs a{};
+  a.X = s->X;
+  a.N = N;
+  a.D = s->D;
+  a.K = K;
+  a.G = s->G;
+  a.GD = s->GD;
+  a.FD = FD;
+  a.TOPM = p->TOPM;
+  a.th = p->thre;
+  a.fwd_rule = p->fwd_rule;
+  a.rev_rule = p->rev_rule;
+  a.ep = s->ep_dev;
+  a.fwd_deg = s->fwd_deg;
+  a.fwd_ids = s->fwd_ids;
+  a.fwd_d = s->fwd_d;
+  a.out_deg = s->out_deg;
+  a.out_ids = s->out_ids;
+  a.rev = s->rev.d;
+  a.counters = s->counters;
+  hipEvent_t w0, w1;
+  HIPX(hipEventCreate(&w0));
+  HIPX(hipEventCreate(&w1));
+  Sections sec{s};
+  fill_u32(reinterpret_cast<uint32_t*>(s->counters), 0, 32, st);
+  fill_u32(s->ep_dev, 0, 1, st);
+  clear_errors(s, {&s->rev});
+  HIPX(hipEventRecord(w0, st));
+  pb_reset(a.rev, N, s->fill, st);
+  if (p->kind == TG_KIND_PATH) {
+    if (sec.start(0)) return TG_ERR_HIP;
+    launch_centroid(s->X, N, s->D, s->cpart, s->centroid, st);
+    launch_prune(a, PH_EP, s->centroid, s->ep_dev, st);
+    if (sec.stop()) return TG_ERR_HIP;
+  }
+  if (sec.start(1)) return TG_ERR_HIP;
+  switch (p->kind) {
+    case TG_KIND_PATH: launch_prune(a, PH_FWD_PATH, nullptr, nullptr, st); break;
+    case TG_KIND_TWOHOP: launch_prune(a, PH_FWD_2HOP, nullptr, nullptr, st); break;
+    case TG_KIND_CAGRA: launch_prune(a, PH_CAGRA, nullptr, nullptr, st); break;
+    default: launch_prune(a, PH_DPG, nullptr, nullptr, st); break;
+  }
+  if (sec.stop()) return TG_ERR_HIP;
+  if (sec.start(3)) return TG_ERR_HIP;
+  pb_group_overflow(a.rev, N, s->fill, s->scan_tmp, st);
+  if (sec.stop()) return TG_ERR_HIP;
+  if (sec.start(2)) return TG_ERR_HIP;
+  launch_prune(a, rank ? PH_RANK_REV : PH_REV, nullptr, nullptr, st);
+  if (sec.stop()) return TG_ERR_HIP;
+  HIPX(hipEventRecord(w1, st));
+  HIPX(hipStreamSynchronize(st));
+  if (int rc = check_launch("prune")) return rc;
+  float ms = 0.f;
+  HIPX(hipEventElapsedTime(&ms, w0, w1));
+  (void)hipEventDestroy(w0);
+  (void)hipEventDestroy(w1);
+  s->stats.prune_ms = ms;
+  for (double& v : s->stats.prune_kernel_ms) v = 0.0;
+  if (sec.collect(s->stats.prune_kernel_ms)) return TG_ERR_HIP;
+  unsigned long long cnt[8];
+  HIPX(hipMemcpy(cnt, s->counters, sizeof(cnt), hipMemcpyDeviceToHost));
+  for (int i = 0; i < 8; ++i) s->stats.prune_counters[i] = cnt[i];
+  HIPX(hipMemcpy(&s->ep, s->ep_dev, 4, hipMemcpyDeviceToHost));
+  if (check_pb_err(s, s->rev, "reverse edges")) return TG_ERR_CAPACITY;
+  s->has_prune = true;
+  s->last_prune = *p;
+  return TG_OK;
+}
+
+int tg_get_pruned(tg_state* s, uint32_t* deg, uint32_t* ids, uint32_t* ep) {
+  if (!s) return fail(TG_ERR_ARG, "null handle");
+  if (!s->has_prune) return fail(TG_ERR_STATE, "no pruned graph");
+  HIPX(hipSetDevice(s->device));
+  const uint32_t FD = s->last_prune.FD;
+  if (deg) HIPX(hipMemcpy(deg, s->out_deg, size_t(s->N) * 4, hipMemcpyDeviceToHost));
+  if (ids) HIPX(hipMemcpy(ids, s->out_ids, size_t(s->N) * FD * 4, hipMemcpyDeviceToHost));
+  if (ep) *ep = s->ep;
+  return TG_OK;
+}
+
+int tg_get_forward(tg_state* s, uint32_t* deg, uint32_t* ids, float* dists) {
+  if (!s) return fail(TG_ERR_ARG, "null handle");
+  if (!s->has_prune) return fail(TG_ERR_STATE, "no pruned graph");
+  HIPX(hipSetDevice(s->device));
+  const uint32_t FD = s->last_prune.FD;
+  if (deg) HIPX(hipMemcpy(deg, s->fwd_deg, size_t(s->N) * 4, hipMemcpyDeviceToHost));
+  if (ids) HIPX(hipMemcpy(ids, s->fwd_ids, size_t(s->N) * FD * 4, hipMemcpyDeviceToHost));
+  if (dists) HIPX(hipMemcpy(dists, s->fwd_d, size_t(s->N) * FD * 4, hipMemcpyDeviceToHost));
+  return TG_OK;
+}
+
+// ------------------------------------------------------------------------ writers
+int tg_store_index(tg_state* s, const char* path, const char* index_type, uint64_t nssg_seed) {
+  if (!s || !path || !index_type) return fail(TG_ERR_ARG, "null argument");
+  if (!s->has_prune) return fail(TG_ERR_STATE, "no pruned graph to store");
+  const std::string t(index_type);
+  int kind = -1;
+  if (t == "NSG") kind = 0;
+  else if (t == "Vamana") kind = 1;
+  else if (t == "NSSG") kind = 2;
+  else if (t == "DPG") kind = 3;
+  else if (t == "CAGRA") kind = 4;
+  else return fail(TG_ERR_ARG, "unknown index type " + t);
+  const uint32_t N = s->N, FD = s->last_prune.FD;
+  std::vector<uint32_t> deg(N), ids(size_t(N) * FD);
+  if (int rc = tg_get_pruned(s, deg.data(), ids.data(), nullptr)) return rc;
+  FILE* f = fopen(path, "wb");
+  if (!f) return fail(TG_ERR_IO, std::string("cannot open ") + path + " for writing");
+  bool ok = true;
+  auto put = [&](const void* p, size_t n) { ok = ok && fwrite(p, 1, n, f) == n; };
+  const uint32_t width = FD, ep = s->ep;
+  if (kind == 0 || kind == 2) {
+    put(&width, 4);
+    if (kind == 0) {
+      put(&ep, 4);
+    } else {
+      // first 10 of a SplitMix64-driven Fisher-Yates shuffle of 0..N-1
+      std::vector<uint32_t> perm(N);
+      for (uint32_t i = 0; i < N; ++i) perm[i] = i;
+      uint64_t x = nssg_seed;
+      const uint32_t m = std::min<uint32_t>(10, N);
+      for (uint32_t i = 0; i < m; ++i) {
+        const uint64_t j = i + splitmix64(x) % uint64_t(N - i);
+        std::swap(perm[i], perm[j]);
+      }
+      uint32_t eps[10] = {0};
+      for (uint32_t i = 0; i < m; ++i) eps[i] = perm[i];
+      const uint32_t n_ep = 10;
+      put(&n_ep, 4);
+      put(eps, 40);
+    }
+    for (uint32_t u = 0; u < N; ++u) {
+      put(&deg[u], 4);
+      put(&ids[size_t(u) * FD], 4ull * deg[u]);
+    }
+  } else if (kind == 1) {
+    uint64_t size = 24;
+    uint32_t maxd = 0;
+    for (uint32_t u = 0; u < N; ++u) {
+      size += 4ull * (deg[u] + 1);
+      maxd = std::max(maxd, deg[u]);
+    }
+    const uint64_t frozen = 0;
+    put(&size, 8);
+    put(&maxd, 4);
+    put(&ep, 4);
+    put(&frozen, 8);
+    for (uint32_t u = 0; u < N; ++u) {
+      put(&deg[u], 4);
+      put(&ids[size_t(u) * FD], 4ull * deg[u]);
+    }
+  } else if (kind == 3) {
+    const char magic[8] = {'K', 'N', 'N', 'G', 'R', 'A', 'P', 'H'};
+    const uint32_t vmaj = 2, vmin = 0;
+    put(magic, 8);
+    put(&vmaj, 4);
+    put(&vmin, 4);
+    put(&N, 4);
+    for (uint32_t u = 0; u < N; ++u) {
+      put(&deg[u], 4);
+      put(&deg[u], 4);
+      put(&ids[size_t(u) * FD], 4ull * deg[u]);
+    }
+  } else {
+    for (uint32_t u = 0; u < N; ++u) {
+      put(&width, 4);
+      put(&ids[size_t(u) * FD], 4ull * FD);
+    }
+  }
+  ok = (fclose(f) == 0) && ok;
+  if (!ok) return fail(TG_ERR_IO, std::string("short write to ") + path);
+  return TG_OK;
+}
+
+// ------------------------------------------------------------------------ recall
+int tg_recall(tg_state* s, uint32_t nq, uint64_t seed, uint32_t k, double* recall) {
+  if (!s || !recall) return fail(TG_ERR_ARG, "null argument");
+  if (!s->has_knn) return fail(TG_ERR_STATE, "no kNN graph");
+  if (k < 1 || k > 16 || k > s->K) return fail(TG_ERR_ARG, "k must be in [1, min(16, K)]");
+  if (nq < 1 || nq > s->N) return fail(TG_ERR_ARG, "nq must be in [1, N]");
+  HIPX(hipSetDevice(s->device));
+  // queries: first nq entries of a SplitMix64 Fisher-Yates shuffle of 0..N-1
+  std::vector<uint32_t> perm(s->N);
+  for (uint32_t i = 0; i < s->N; ++i) perm[i] = i;
+  uint64_t x = seed;
+  for (uint32_t i = 0; i < nq; ++i) {
+    const uint64_t j = i + splitmix64(x) % uint64_t(s->N - i);
+    std::swap(perm[i], perm[j]);
+  }
+  const uint32_t cap = 256;
+  uint32_t *q = nullptr, *cand_n = nullptr, *hits = nullptr;
+  double *kth = nullptr, *cand_d = nullptr;
+  int rc = TG_OK;
+  if (dalloc(&q, nq) || dalloc(&cand_n, nq) || dalloc(&hits, nq) || dalloc(&kth, nq) ||
+      dalloc(&cand_d, size_t(nq) * cap + size_t(nq) * k)) {
+    rc = TG_ERR_HIP;
+  } else {
+    hipError_t e = hipMemcpy(q, perm.data(), size_t(nq) * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      launch_recall(s->X, s->N, s->D, s->G, s->K, q, nq, k, kth, cand_n, cand_d, cap, hits,
+                    s->stream);
+      e = hipStreamSynchronize(s->stream);
+    }
+    if (e == hipSuccess) e = hipGetLastError();
+    std::vector<uint32_t> h(nq);
+    if (e == hipSuccess) e = hipMemcpy(h.data(), hits, size_t(nq) * 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+      rc = fail(TG_ERR_HIP, std::string("recall: ") + hipGetErrorString(e));
+    } else {
+      double tot = 0.0;
+      for (uint32_t i = 0; i < nq; ++i) tot += double(std::min(h[i], k)) / k;
+      *recall = tot / nq;
+    }
+  }
+  dfree(q);
+  dfree(cand_n);
+  dfree(hits);
+  dfree(kth);
+  dfree(cand_d);
+  return rc;
+}
+
+int tg_buffer(tg_state* s, int32_t idx, void** dev_ptr, size_t* bytes) {
+  if (!s || !dev_ptr || !bytes) return fail(TG_ERR_ARG, "null argument");
+  const size_t N = s->N, K = s->K;
+  switch (idx) {
+    case 0: *dev_ptr = s->G; *bytes = N * K * 4; break;
+    case 1: *dev_ptr = s->R0; *bytes = s->R0 ? N * K * 8 : 0; break;
+    case 2: *dev_ptr = s->X; *bytes = N * s->D * 4; break;
+    case 3: *dev_ptr = s->GD; *bytes = N * K * 4; break;
+    case 4: *dev_ptr = s->R1; *bytes = s->R1 ? N * K * 8 : 0; break;
+    case 5: *dev_ptr = s->Rn0; *bytes = s->Rn0 ? N * 4 : 0; break;
+    case 6: *dev_ptr = s->centroid; *bytes = size_t(s->D) * 4; break;
+    case 7: *dev_ptr = s->H; *bytes = s->H ? N * 128 * 4 : 0; break;
+    default: return fail(TG_ERR_ARG, "buffer index must be in [0, 7]");
+  }
+  return TG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,280 @@
+// common.hpp -- device primitives shared by the Tagore-on-MI355X kernels.
+//
+// * (distance, id) keys packed into u64 so that unsigned order == (d, id) order for the
+//   non-negative distances used everywhere (+inf sorts after every finite distance).
+// * Canonical FP32 squared L2 evaluated by a 16-lane group with float4 loads: lane t owns
+//   the partial sums p[4t..4t+3] over i == l (mod 64); the tree p[l] += p[l+s] runs as
+//   shuffle-down steps 8,4,2,1 across the group and the last two levels inside lane 0.
+//   This is the same arithmetic, in the same order, as the CPU oracle (bit-identical).
+// * Wave-level (64-lane) bitonic sorts and scans in LDS.
+// * PushBuf: per-destination append lists with a bounded slot area and a global overflow
+//   area; every reduction sorts, so results do not depend on arrival order.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define TG_WAVE 64
+#define TG_INV 0xFFFFFFFFu
+#define TG_KEY_INF 0xFFFFFFFFFFFFFFFFull
+
+namespace tg {
+
+__device__ __forceinline__ uint64_t mk_key(float d, uint32_t id) {
+  return (uint64_t(__float_as_uint(d)) << 32) | id;
+}
+__device__ __forceinline__ float key_d(uint64_t k) { return __uint_as_float(uint32_t(k >> 32)); }
+__device__ __forceinline__ uint32_t key_id(uint64_t k) { return uint32_t(k); }
+__device__ __forceinline__ uint64_t key_with_id(uint64_t k, uint32_t id) {
+  return (k & 0xFFFFFFFF00000000ull) | id;
+}
+// finite distance (not +inf / invalid key)
+__device__ __forceinline__ bool key_finite(uint64_t k) { return uint32_t(k >> 32) < 0x7F800000u; }
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// wave-wide ballot (64-bit mask)
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+// number of set bits of m below this lane
+__device__ __forceinline__ int mask_rank(uint64_t m) {
+  return __popcll(m & ((1ull << lane_id()) - 1ull));
+}
+
+// ---------------------------------------------------------------------------------------
+// Canonical distance.  `MC` = number of 64-element chunks held (ceil(D/64) <= MC).
+// Row a is preloaded by the group (ra[]), row b is read from memory.  Every lane of the
+// group returns the same value.  Requires D % 4 == 0 (host checks).
+// ---------------------------------------------------------------------------------------
+template <int MC>
+struct Row16 {
+  float4 v[MC];
+};
+
+template <int MC>
+__device__ __forceinline__ void load_row16(Row16<MC>& r, const float* __restrict__ x, int D,
+                                           int t) {
+#pragma unroll
+  for (int m = 0; m < MC; ++m) {
+    const int i = m * 64 + 4 * t;
+    if (i < D)
+      r.v[m] = *reinterpret_cast<const float4*>(x + i);
+    else
+      r.v[m] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+}
+
+// tree reduction of the group's four partials (lane t holds p[4t..4t+3])
+__device__ __forceinline__ float tree16(float p0, float p1, float p2, float p3) {
+#pragma unroll
+  for (int s = 8; s >= 1; s >>= 1) {
+    p0 = p0 + __shfl_down(p0, s, 16);
+    p1 = p1 + __shfl_down(p1, s, 16);
+    p2 = p2 + __shfl_down(p2, s, 16);
+    p3 = p3 + __shfl_down(p3, s, 16);
+  }
+  p0 = p0 + p2;
+  p1 = p1 + p3;
+  p0 = p0 + p1;
+  return __shfl(p0, 0, 16);
+}
+
+template <int MC>
+__device__ __forceinline__ float dist16(const Row16<MC>& a, const float* __restrict__ b, int D,
+                                        int t) {
+  float p0 = 0.f, p1 = 0.f, p2 = 0.f, p3 = 0.f;
+#pragma unroll
+  for (int m = 0; m < MC; ++m) {
+    const int i = m * 64 + 4 * t;
+    if (i < D) {
+      const float4 y = *reinterpret_cast<const float4*>(b + i);
+      const float e0 = a.v[m].x - y.x, e1 = a.v[m].y - y.y, e2 = a.v[m].z - y.z,
+                  e3 = a.v[m].w - y.w;
+      p0 = __builtin_fmaf(e0, e0, p0);
+      p1 = __builtin_fmaf(e1, e1, p1);
+      p2 = __builtin_fmaf(e2, e2, p2);
+      p3 = __builtin_fmaf(e3, e3, p3);
+    }
+  }
+  return tree16(p0, p1, p2, p3);
+}
+
+// both rows from memory
+template <int MC>
+__device__ __forceinline__ float dist16_mm(const float* __restrict__ a,
+                                           const float* __restrict__ b, int D, int t) {
+  float p0 = 0.f, p1 = 0.f, p2 = 0.f, p3 = 0.f;
+#pragma unroll
+  for (int m = 0; m < MC; ++m) {
+    const int i = m * 64 + 4 * t;
+    if (i < D) {
+      const float4 x = *reinterpret_cast<const float4*>(a + i);
+      const float4 y = *reinterpret_cast<const float4*>(b + i);
+      const float e0 = x.x - y.x, e1 = x.y - y.y, e2 = x.z - y.z, e3 = x.w - y.w;
+      p0 = __builtin_fmaf(e0, e0, p0);
+      p1 = __builtin_fmaf(e1, e1, p1);
+      p2 = __builtin_fmaf(e2, e2, p2);
+      p3 = __builtin_fmaf(e3, e3, p3);
+    }
+  }
+  return tree16(p0, p1, p2, p3);
+}
+
+// ---------------------------------------------------------------------------------------
+// Filter rules (Tagore_src.cu:1488-1495).  res = d(p,c), dis = d(u,c), cur = d(u,p).
+// Compiled with -ffp-contract=off and correctly rounded division / square root.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ float rule_eval(int rule, float res, float dis, float cur, float th) {
+  if (rule == 0) return th * res - dis;
+  return th - ((dis + cur) - res) / 2.0f / __builtin_sqrtf(dis * cur);
+}
+
+// ---------------------------------------------------------------------------------------
+// Wave-level sorts in LDS.  Called by all 64 lanes of a one-wave workgroup; n is a power of
+// two.  __syncthreads() orders the LDS traffic between network stages.
+// ---------------------------------------------------------------------------------------
+template <class T>
+__device__ void wave_bitonic(T* a, int n) {
+  const int lane = threadIdx.x;
+  for (int k = 2; k <= n; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = lane; i < (n >> 1); i += TG_WAVE) {
+        const int lo = 2 * j * (i / j) + (i % j);
+        const int hi = lo + j;
+        const bool up = (lo & k) == 0;
+        T x = a[lo], y = a[hi];
+        if ((x > y) == up) {
+          a[lo] = y;
+          a[hi] = x;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+__host__ __device__ __forceinline__ int pow2_at_least(int n) {
+  int p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
+
+// inclusive wave scan of an int
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int s = 1; s < TG_WAVE; s <<= 1) {
+    int o = __shfl_up(v, s, TG_WAVE);
+    if (lane >= s) v += o;
+  }
+  return v;
+}
+
+// ---------------------------------------------------------------------------------------
+// Push buffers ("update lists").  Producers append keys for a destination node: the first
+// `cap` go to the node's slot row, the rest to a global overflow array of (dest, key)
+// pairs.  After the producer kernel, overflow pairs are grouped per destination (count ->
+// scan -> scatter, host-driven).  Reducers read slot row + overflow segment and sort, so
+// the outcome is independent of arrival order.  Overflow beyond ov_cap sets *err.
+// ---------------------------------------------------------------------------------------
+struct PushBuf {
+  uint32_t* cnt;       // [N] pushes per destination
+  uint64_t* slots;     // [N * cap]
+  uint32_t* ov_dest;   // [ov_cap]
+  uint64_t* ov_key;    // [ov_cap]
+  uint32_t* ov_used;   // [1]
+  uint32_t* ov_off;    // [N + 1] exclusive offsets of grouped overflow
+  uint64_t* ov_sorted; // [ov_cap] overflow keys grouped by destination
+  uint32_t* err;       // [1]
+  uint32_t cap;
+  uint32_t ov_cap;
+};
+
+__device__ __forceinline__ void pb_push(const PushBuf& pb, uint32_t v, uint64_t key) {
+  const uint32_t s = atomicAdd(&pb.cnt[v], 1u);
+  if (s < pb.cap) {
+    pb.slots[size_t(v) * pb.cap + s] = key;
+  } else {
+    const uint32_t o = atomicAdd(pb.ov_used, 1u);
+    if (o < pb.ov_cap) {
+      pb.ov_dest[o] = v;
+      pb.ov_key[o] = key;
+    } else {
+      atomicOr(pb.err, 1u);
+    }
+  }
+}
+
+// Number of keys pushed to v, and accessor of the i-th (slot area first, then overflow).
+__device__ __forceinline__ uint32_t pb_count(const PushBuf& pb, uint32_t v) { return pb.cnt[v]; }
+__device__ __forceinline__ uint64_t pb_get(const PushBuf& pb, uint32_t v, uint32_t i) {
+  if (i < pb.cap) return pb.slots[size_t(v) * pb.cap + i];
+  const uint32_t o = pb.ov_off[v] + (i - pb.cap);
+  // past ov_cap only when the overflow area itself overflowed (*err set, call fails)
+  return o < pb.ov_cap ? pb.ov_sorted[o] : TG_KEY_INF;
+}
+
+// Keeps the R smallest keys pushed to v that pass `pred` (and, if uniq, merges identical
+// keys), sorted ascending into buf[0..R); returns how many there are (<= R <= 128).
+// buf must hold 256 u64 (LDS).  Optional seed keys (sorted, unique, <= R, all passing pred)
+// start the running selection.
+template <class Pred>
+__device__ int pb_top(const PushBuf& pb, uint32_t v, int R, bool uniq, Pred pred, uint64_t* buf,
+                      const uint64_t* seed = nullptr, int nseed = 0) {
+  const int lane = threadIdx.x;
+  const uint32_t n = pb_count(pb, v);
+  int nbest = nseed;
+  for (int i = lane; i < 128; i += TG_WAVE) buf[i] = i < nseed ? seed[i] : TG_KEY_INF;
+  __syncthreads();
+  for (uint32_t base = 0; base < n; base += 128) {
+    for (int i = lane; i < 128; i += TG_WAVE) {
+      uint64_t k = TG_KEY_INF;
+      if (base + i < n) {
+        k = pb_get(pb, v, base + i);
+        if (!pred(k)) k = TG_KEY_INF;
+      }
+      buf[128 + i] = k;
+    }
+    __syncthreads();
+    wave_bitonic(buf, 256);
+    // compaction: keep first R (unique if requested) into buf[0..R)
+    uint64_t mine[4];
+    bool keep[4];
+    int cntk = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = lane * 4 + q;
+      mine[q] = buf[i];
+      keep[q] = mine[q] != TG_KEY_INF && (!uniq || i == 0 || buf[i - 1] != mine[q]);
+      cntk += keep[q] ? 1 : 0;
+    }
+    const int incl = wave_incl_scan(cntk);
+    const int total = __shfl(incl, TG_WAVE - 1, TG_WAVE);
+    __syncthreads();
+    int pos = incl - cntk;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (keep[q]) {
+        if (pos < R) buf[pos] = mine[q];
+        ++pos;
+      }
+    }
+    nbest = total < R ? total : R;
+    __syncthreads();
+    for (int i = nbest + lane; i < 256; i += TG_WAVE) buf[i] = TG_KEY_INF;
+    __syncthreads();
+  }
+  return nbest;
+}
+
+// lower_bound of key x in the sorted LDS array a[0..n)
+template <class T>
+__device__ __forceinline__ int lower_bound_lds(const T* a, int n, T x) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+}  // namespace tg

@@ -1,0 +1,99 @@
+// kernels.hpp -- argument blocks and launchers shared between the HIP kernels and the
+// host orchestration (api.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common.hpp"
+
+namespace tg {
+
+enum PrunePhase {
+  PH_EP = 0,
+  PH_FWD_PATH = 1,
+  PH_FWD_2HOP = 2,
+  PH_REV = 3,
+  PH_CAGRA = 4,
+  PH_DPG = 5,
+  PH_RANK_REV = 6,
+};
+
+struct PruneArgs {
+  const float* X;
+  uint32_t N, D, K;
+  const uint32_t* G;   // kNN ids [N*K], rows sorted by (d, id)
+  const float* GD;     // kNN distances [N*K]
+  uint32_t FD, TOPM;
+  float th;
+  int fwd_rule, rev_rule;  // 0 dist, 1 angle
+  const uint32_t* ep;      // device scalar
+  uint32_t* fwd_deg;       // [N]
+  uint32_t* fwd_ids;       // [N*FD]
+  float* fwd_d;            // [N*FD]
+  uint32_t* out_deg;       // [N]
+  uint32_t* out_ids;       // [N*FD]
+  PushBuf rev;             // reverse pushes
+  unsigned long long* counters;  // [8] evaluation counters or null
+};
+
+void launch_prune(const PruneArgs& a, int phase, const float* centroid, uint32_t* ep,
+                  hipStream_t s);
+void launch_centroid(const float* X, uint32_t N, uint32_t D, double* part, float* c,
+                     hipStream_t s);
+
+// ---------------------------------------------------------------------- kNN construction
+enum KnnPrecision { KP_EXACT = 0, KP_FP32 = 1, KP_BF16 = 2 };
+
+struct KnnArgs {
+  const float* X;
+  uint32_t N, D, K;
+  uint32_t* G;      // [N*K]
+  float* GD;        // [N*K]
+  uint8_t* V;       // [N*K] visited flags
+  uint32_t* G1;     // phase-2 output ids
+  float* GD1;       // phase-2 output distances
+  uint32_t* H;      // [N*128] sample lists: new' at [0,64), old' at [64,128)
+  uint32_t* na;     // [N] |new'|
+  uint32_t* nb;     // [N] |old'|
+  uint32_t* nnew;   // [N] |new| before reverse samples
+  uint32_t* nold;   // [N]
+  PushBuf pn, po, pc, pr;
+  uint64_t* R0;     // [N*K] reverse lists (keys), read side
+  uint32_t* Rn0;    // [N]
+  uint64_t* R1;     // [N*K] write side
+  uint32_t* Rn1;    // [N]
+  uint64_t seed;
+  unsigned long long* counters;  // [8] or null
+};
+
+void launch_knn_init(const KnnArgs& a, hipStream_t s);
+void launch_knn_sample(const KnnArgs& a, hipStream_t s);
+void launch_knn_rsample(const KnnArgs& a, hipStream_t s);
+void launch_knn_localjoin(const KnnArgs& a, int precision, hipStream_t s);
+void launch_knn_merge(const KnnArgs& a, hipStream_t s);
+void launch_knn_rev_build(const KnnArgs& a, hipStream_t s);
+void launch_knn_rev_reduce(const KnnArgs& a, hipStream_t s);
+void launch_knn_phase2(const KnnArgs& a, hipStream_t s);
+void launch_knn_rupdate(const KnnArgs& a, hipStream_t s);
+void launch_knn_final(const KnnArgs& a, hipStream_t s);
+
+// ------------------------------------------------------------------------------ utilities
+// push-buffer bookkeeping
+void pb_reset(const PushBuf& pb, uint32_t N, uint32_t* fill, hipStream_t s);
+void pb_group_overflow(const PushBuf& pb, uint32_t N, uint32_t* fill, uint32_t* scan_tmp,
+                       hipStream_t s);
+size_t scan_tmp_words(uint32_t N);
+
+void fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+void fill_u8(uint8_t* p, uint8_t v, size_t n, hipStream_t s);
+
+// synthetic data: kind 0 integer GMM in [0,255], 1 unit-norm GMM, 2 |GMM| scaled to [0,1]
+void launch_gen_data(int kind, uint32_t N, uint32_t D, uint32_t ncomp, uint64_t seed,
+                     float sigma, float* centres, float* X, uint32_t* scratch, hipStream_t s);
+
+// FP64 recall judge
+void launch_recall(const float* X, uint32_t N, uint32_t D, const uint32_t* G, uint32_t K,
+                   const uint32_t* q, uint32_t nq, uint32_t k, double* kth, uint32_t* cand_n,
+                   double* cand_d, uint32_t cand_cap, uint32_t* hits, hipStream_t s);
+
+}  // namespace tg

@@ -1,0 +1,476 @@
+// knn.hip -- GNN-Descent kNN-graph construction kernels for gfx950 (exact FP32 mode).
+//
+// Deterministic restatement of Tagore's two-phase GNN-Descent (Tagore_src.cu:150-1206 as
+// driven by pybind.cu:101-125); the CPU oracle (or_gnn_descent) states the same spec, so
+// in exact mode the graph produced here is bit-identical to it:
+//   init                 initialize_graph           Tagore_src.cu:158-166
+//   sample / rsample     nn_descent_opt_sample      Tagore_src.cu:531-594, 603-674
+//   local join           nn_descent_opt_cal         Tagore_src.cu:684-1033
+//   merge                nn_descent_opt_merge       Tagore_src.cu:1043-1161
+//   reverse build        do_reverse_graph           Tagore_src.cu:1187-1206
+//   phase 2              sample_kernel6             Tagore_src.cu:178-403
+//   final reverse merge  merge_reverse_plus         Tagore_src.cu:420-512
+// Candidate and reverse lists go through PushBufs (order-independent reductions) instead
+// of the reference's atomic slot counters.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace tg {
+
+constexpr int S = 32;        // SAMPLE
+constexpr int TILE = 16;     // local-join tile edge
+constexpr int HROW = 128;    // sample-list row: new' [0,64), old' [64,128)
+constexpr int LEADERS = 4;   // phase-2 expansions per node
+constexpr int TAKE = 64;     // forward / reverse entries taken per leader
+
+__device__ __forceinline__ uint64_t sm_next(uint64_t& s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ uint64_t sm_stream(uint64_t seed, uint64_t idx) {
+  uint64_t s = seed ^ (idx * 0xD6E8FEB86659FD93ull);
+  return sm_next(s);
+}
+__device__ __forceinline__ uint32_t sm_bounded(uint64_t x, uint32_t n) {
+  return uint32_t(((x >> 32) * uint64_t(n)) >> 32);
+}
+
+// ------------------------------------------------------------------------------- init
+__global__ void k_init_ids(KnnArgs a) {
+  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= a.N) return;
+  uint32_t* row = a.G + size_t(u) * a.K;
+  uint64_t st = sm_stream(a.seed, u);
+  uint32_t cnt = 0;
+  while (cnt < a.K) {
+    const uint32_t c = sm_bounded(sm_next(st), a.N);
+    if (c == u) continue;
+    bool dup = false;
+    for (uint32_t j = 0; j < cnt; ++j) dup |= (row[j] == c);
+    if (dup) continue;
+    row[cnt++] = c;
+  }
+}
+
+template <int MC>
+__global__ __launch_bounds__(64) void k_init_sort(KnnArgs a) {
+  __shared__ uint64_t kk[128];
+  const uint32_t u = blockIdx.x;
+  if (u >= a.N) return;
+  const int lane = threadIdx.x, g = lane >> 4, t = lane & 15;
+  const int K = a.K, D = a.D, n = pow2_at_least(K);
+  Row16<MC> xu;
+  load_row16<MC>(xu, a.X + size_t(u) * D, D, t);
+  for (int j = lane; j < n; j += TG_WAVE) kk[j] = TG_KEY_INF;
+  __syncthreads();
+  for (int j = g; j < K; j += 4) {
+    const uint32_t y = a.G[size_t(u) * K + j];
+    const float d = dist16<MC>(xu, a.X + size_t(y) * D, D, t);
+    if (t == 0) kk[j] = mk_key(d, y);
+  }
+  __syncthreads();
+  wave_bitonic(kk, n);
+  for (int j = lane; j < K; j += TG_WAVE) {
+    a.G[size_t(u) * K + j] = key_id(kk[j]);
+    a.GD[size_t(u) * K + j] = key_d(kk[j]);
+    a.V[size_t(u) * K + j] = 0;
+  }
+  if (a.counters && lane == 0) atomicAdd(&a.counters[0], (unsigned long long)K);
+}
+
+// ------------------------------------------------------------------------- phase 1
+// new[u] = first S unvisited entries (marked visited), old[u] = first S visited entries:
+// the closed form of the reference's three scan loops (Tagore_src.cu:536-567).
+__global__ __launch_bounds__(64) void k_sample(KnnArgs a) {
+  const uint32_t u = blockIdx.x;
+  if (u >= a.N) return;
+  const int lane = threadIdx.x;
+  const int K = a.K;
+  int cu = 0, cv = 0;
+  for (int base = 0; base < K; base += TG_WAVE) {
+    const int j = base + lane;
+    const bool valid = j < K;
+    const bool vis = valid && a.V[size_t(u) * K + j] != 0;
+    const bool unv = valid && !vis;
+    const uint64_t mU = ballot(unv), mV = ballot(vis);
+    const int rU = cu + mask_rank(mU), rV = cv + mask_rank(mV);
+    if (unv && rU < S) {
+      const uint32_t id = a.G[size_t(u) * K + j];
+      a.H[size_t(u) * HROW + rU] = id;
+      a.V[size_t(u) * K + j] = 1;
+      pb_push(a.pn, id, mk_key(a.GD[size_t(u) * K + j], u));
+    }
+    if (vis && rV < S) {
+      const uint32_t id = a.G[size_t(u) * K + j];
+      a.H[size_t(u) * HROW + 64 + rV] = id;
+      pb_push(a.po, id, mk_key(a.GD[size_t(u) * K + j], u));
+    }
+    cu += __popcll(mU);
+    cv += __popcll(mV);
+  }
+  if (lane == 0) {
+    a.nnew[u] = min(cu, S);
+    a.nold[u] = min(cv, S);
+  }
+}
+
+// reverse samples: the S nearest pushers of each kind, appended when not already present
+__global__ __launch_bounds__(64) void k_rsample(KnnArgs a) {
+  __shared__ uint64_t buf[256];
+  __shared__ uint32_t own[64];
+  const uint32_t v = blockIdx.x;
+  if (v >= a.N) return;
+  const int lane = threadIdx.x;
+  uint32_t* h = a.H + size_t(v) * HROW;
+  for (int kind = 0; kind < 2; ++kind) {
+    const int c0 = kind == 0 ? a.nnew[v] : a.nold[v];
+    uint32_t* list = h + (kind == 0 ? 0 : 64);
+    if (lane < c0) own[lane] = list[lane];
+    __syncthreads();
+    const int r = pb_top(kind == 0 ? a.pn : a.po, v, S, false, [](uint64_t) { return true; }, buf);
+    bool add = false;
+    uint32_t w = 0;
+    if (lane < r) {
+      w = key_id(buf[lane]);
+      add = true;
+      for (int j = 0; j < c0; ++j) add = add && (own[j] != w);
+    }
+    const uint64_t m = ballot(add);
+    if (add) list[c0 + mask_rank(m)] = w;
+    if (lane == 0) {
+      if (kind == 0) a.na[v] = c0 + __popcll(m);
+      else a.nb[v] = c0 + __popcll(m);
+    }
+    __syncthreads();
+  }
+}
+
+// local join, exact canonical distances; one 256-thread workgroup (16 groups) per node
+template <int MC>
+__global__ __launch_bounds__(256) void k_localjoin_exact(KnnArgs a) {
+  __shared__ uint32_t A[64], B[64];
+  __shared__ float wA[64], wB[64];
+  __shared__ float MAB[64 * 64], MAA[64 * 64];
+  const uint32_t u = blockIdx.x;
+  if (u >= a.N) return;
+  const int tid = threadIdx.x, gg = tid >> 4, t = tid & 15;
+  const int K = a.K, D = a.D;
+  const int nA = a.na[u], nB = a.nb[u];
+  if (tid < 64) {
+    if (tid < nA) {
+      A[tid] = a.H[size_t(u) * HROW + tid];
+      wA[tid] = a.GD[size_t(A[tid]) * K + K - 1];
+    }
+    if (tid < nB) {
+      B[tid] = a.H[size_t(u) * HROW + 64 + tid];
+      wB[tid] = a.GD[size_t(B[tid]) * K + K - 1];
+    }
+  }
+  __syncthreads();
+  const float INF = __uint_as_float(0x7F800000u);
+  for (int i = gg; i < nA; i += 16) {
+    Row16<MC> ra;
+    load_row16<MC>(ra, a.X + size_t(A[i]) * D, D, t);
+    for (int j = 0; j < nB; ++j) {
+      float d = INF;
+      if (A[i] != B[j]) d = dist16<MC>(ra, a.X + size_t(B[j]) * D, D, t);
+      if (t == 0) MAB[i * 64 + j] = d;
+    }
+    for (int j = i + 1; j < nA; ++j) {
+      const float d = dist16<MC>(ra, a.X + size_t(A[j]) * D, D, t);
+      if (t == 0) {
+        MAA[i * 64 + j] = d;
+        MAA[j * 64 + i] = d;
+      }
+    }
+    if (t == 0) MAA[i * 64 + i] = INF;
+  }
+  __syncthreads();
+  const int r = tid >> 2, tile = tid & 3, lo = tile * TILE;
+  // rows of A against tiles of B
+  if (r < nA && lo < nB) {
+    uint64_t best = TG_KEY_INF;
+    for (int j = lo; j < min(nB, lo + TILE); ++j) {
+      const float d = MAB[r * 64 + j];
+      if (d < INF) best = min(best, mk_key(d, B[j]));
+    }
+    if (best != TG_KEY_INF && key_d(best) < wA[r]) pb_push(a.pc, A[r], best);
+  }
+  // columns of B against tiles of A
+  if (r < nB && lo < nA) {
+    uint64_t best = TG_KEY_INF;
+    for (int i = lo; i < min(nA, lo + TILE); ++i) {
+      const float d = MAB[i * 64 + r];
+      if (d < INF) best = min(best, mk_key(d, A[i]));
+    }
+    if (best != TG_KEY_INF && key_d(best) < wB[r]) pb_push(a.pc, B[r], best);
+  }
+  // rows of A against tiles of A
+  if (r < nA && lo < nA) {
+    uint64_t best = TG_KEY_INF;
+    for (int j = lo; j < min(nA, lo + TILE); ++j) {
+      const float d = MAA[r * 64 + j];
+      if (d < INF) best = min(best, mk_key(d, A[j]));
+    }
+    if (best != TG_KEY_INF && key_d(best) < wA[r]) pb_push(a.pc, A[r], best);
+  }
+  if (a.counters && tid == 0)
+    atomicAdd(&a.counters[0], (unsigned long long)(nA * nB + nA * (nA - 1) / 2));
+}
+
+// ------------------------------------------------------------------ row merge helper
+// Keeps the K smallest of the row (keys gk[0..K), flags vf) and the candidates cand[0..nc)
+// (sorted, unique, none already listed); writes ids/dists/flags to the output row.
+__device__ void merge_row(int K, const uint64_t* gk, const uint8_t* vf, const uint64_t* cand,
+                          int nc, uint32_t* oid, float* od, uint8_t* ov) {
+  const int lane = threadIdx.x;
+  for (int i = lane; i < K; i += TG_WAVE) {
+    const uint64_t k = gk[i];
+    const int pos = i + lower_bound_lds(cand, nc, k);
+    if (pos < K) {
+      oid[pos] = key_id(k);
+      od[pos] = key_d(k);
+      ov[pos] = vf[i];
+    }
+  }
+  for (int b = lane; b < nc; b += TG_WAVE) {
+    const uint64_t k = cand[b];
+    const int pos = b + lower_bound_lds(gk, K, k);
+    if (pos < K) {
+      oid[pos] = key_id(k);
+      od[pos] = key_d(k);
+      ov[pos] = 0;
+    }
+  }
+}
+
+struct RowLds {
+  uint64_t gk[128];
+  uint32_t gid[128];
+  uint8_t vf[128];
+};
+
+// loads row u into LDS: keys, flags, ids sorted ascending (padding INV)
+__device__ void load_row_lds(const KnnArgs& a, const uint32_t* G, const float* GD, uint32_t u,
+                             RowLds& r) {
+  const int lane = threadIdx.x, K = a.K, n = pow2_at_least(K);
+  for (int j = lane; j < n; j += TG_WAVE) {
+    if (j < K) {
+      const uint32_t id = G[size_t(u) * K + j];
+      r.gk[j] = mk_key(GD[size_t(u) * K + j], id);
+      r.gid[j] = id;
+      r.vf[j] = a.V[size_t(u) * K + j];
+    } else {
+      r.gid[j] = TG_INV;
+    }
+  }
+  __syncthreads();
+  wave_bitonic(r.gid, n);
+}
+
+__device__ __forceinline__ bool listed(const RowLds& r, int K, uint32_t id) {
+  const int p = lower_bound_lds(r.gid, K, id);
+  return p < K && r.gid[p] == id;
+}
+
+// merge of the local-join candidates
+__global__ __launch_bounds__(64) void k_merge(KnnArgs a) {
+  __shared__ RowLds row;
+  __shared__ uint64_t buf[256];
+  const uint32_t v = blockIdx.x;
+  if (v >= a.N) return;
+  const int K = a.K;
+  load_row_lds(a, a.G, a.GD, v, row);
+  const float worst = key_d(row.gk[K - 1]);
+  const int nc = pb_top(a.pc, v, K, true,
+                        [&](uint64_t k) {
+                          return key_id(k) != v && key_d(k) < worst && !listed(row, K, key_id(k));
+                        },
+                        buf);
+  if (nc > 0)
+    merge_row(K, row.gk, row.vf, buf, nc, a.G + size_t(v) * K, a.GD + size_t(v) * K,
+              a.V + size_t(v) * K);
+}
+
+// ---------------------------------------------------------------- reverse build
+__global__ __launch_bounds__(64) void k_rev_build(KnnArgs a) {
+  const uint32_t u = blockIdx.x;
+  if (u >= a.N) return;
+  for (int j = threadIdx.x; j < int(a.K); j += TG_WAVE)
+    pb_push(a.pr, a.G[size_t(u) * a.K + j], mk_key(a.GD[size_t(u) * a.K + j], u));
+}
+
+__global__ __launch_bounds__(64) void k_rev_reduce(KnnArgs a) {
+  __shared__ uint64_t buf[256];
+  const uint32_t v = blockIdx.x;
+  if (v >= a.N) return;
+  const int K = a.K;
+  const int n = pb_top(a.pr, v, K, true, [](uint64_t) { return true; }, buf);
+  for (int i = threadIdx.x; i < K; i += TG_WAVE) {
+    a.R0[size_t(v) * K + i] = i < n ? buf[i] : TG_KEY_INF;
+    a.V[size_t(v) * K + i] = 0;
+  }
+  if (threadIdx.x == 0) a.Rn0[v] = n;
+}
+
+// ------------------------------------------------------------------------- phase 2
+template <int MC>
+__global__ __launch_bounds__(64) void k_phase2(KnnArgs a) {
+  __shared__ RowLds row;
+  __shared__ uint32_t ids[512];
+  __shared__ uint16_t cl[512];
+  __shared__ uint64_t kv[512];
+  __shared__ uint32_t lead[LEADERS];
+  const uint32_t u = blockIdx.x;
+  if (u >= a.N) return;
+  const int lane = threadIdx.x, g = lane >> 4, t = lane & 15;
+  const int K = a.K, D = a.D;
+  load_row_lds(a, a.G, a.GD, u, row);
+  // leaders: first LEADERS unvisited positions in list order
+  int nl = 0;
+  for (int base = 0; base < K && nl < LEADERS; base += TG_WAVE) {
+    const int j = base + lane;
+    const bool unv = j < K && row.vf[j] == 0;
+    const uint64_t m = ballot(unv);
+    const int rk = nl + mask_rank(m);
+    if (unv && rk < LEADERS) {
+      lead[rk] = key_id(row.gk[j]);
+      row.vf[j] = 1;
+    }
+    nl = min(LEADERS, nl + int(__popcll(m)));
+  }
+  __syncthreads();
+  // gather candidate ids
+  int tot = 0;
+  for (int l = 0; l < nl; ++l) {
+    const uint32_t v = lead[l];
+    const int nf = min(TAKE, K);
+    const int nr = min(TAKE, int(a.Rn0[v]));
+    for (int i = lane; i < nf + nr; i += TG_WAVE)
+      ids[tot + i] = i < nf ? a.G[size_t(v) * K + i] : key_id(a.R0[size_t(v) * K + (i - nf)]);
+    tot += nf + nr;
+  }
+  const int n = pow2_at_least(max(tot, 1));
+  for (int i = tot + lane; i < n; i += TG_WAVE) ids[i] = TG_INV;
+  __syncthreads();
+  wave_bitonic(ids, n);
+  // unique, not u, not listed
+  int cnt = 0;
+  for (int base = 0; base < tot; base += TG_WAVE) {
+    const int i = base + lane;
+    bool ok = false;
+    if (i < tot) {
+      const uint32_t id = ids[i];
+      ok = id != TG_INV && id != u && (i == 0 || ids[i - 1] != id) && !listed(row, K, id);
+    }
+    const uint64_t m = ballot(ok);
+    if (ok) cl[cnt + mask_rank(m)] = uint16_t(i);
+    cnt += __popcll(m);
+  }
+  __syncthreads();
+  Row16<MC> xu;
+  load_row16<MC>(xu, a.X + size_t(u) * D, D, t);
+  const float worst = key_d(row.gk[K - 1]);
+  const int nk = pow2_at_least(max(cnt, 1));
+  for (int q = g; q < nk; q += 4) {
+    uint64_t k = TG_KEY_INF;
+    if (q < cnt) {
+      const uint32_t y = ids[cl[q]];
+      const float d = dist16<MC>(xu, a.X + size_t(y) * D, D, t);
+      if (d < worst) k = mk_key(d, y);
+    }
+    if (t == 0) kv[q] = k;
+  }
+  __syncthreads();
+  wave_bitonic(kv, nk);
+  int nc = 0;
+  for (int base = 0; base < nk; base += TG_WAVE) {
+    const int i = base + lane;
+    nc += __popcll(ballot(i < nk && kv[i] != TG_KEY_INF));
+  }
+  nc = min(nc, K);
+  merge_row(K, row.gk, row.vf, kv, nc, a.G1 + size_t(u) * K, a.GD1 + size_t(u) * K,
+            a.V + size_t(u) * K);
+  // reverse pushes for entries that entered the list
+  for (int b = lane; b < nc; b += TG_WAVE) {
+    const uint64_t k = kv[b];
+    const int pos = b + lower_bound_lds(row.gk, K, k);
+    if (pos < K) pb_push(a.pr, key_id(k), mk_key(key_d(k), u));
+  }
+  if (a.counters && lane == 0) atomicAdd(&a.counters[0], (unsigned long long)cnt);
+}
+
+// R1[w] = K smallest of R0[w] and this iteration's pushes
+__global__ __launch_bounds__(64) void k_rupdate(KnnArgs a) {
+  __shared__ uint64_t buf[256];
+  __shared__ uint64_t seed[128];
+  const uint32_t w = blockIdx.x;
+  if (w >= a.N) return;
+  const int K = a.K;
+  const int n0 = a.Rn0[w];
+  for (int i = threadIdx.x; i < n0; i += TG_WAVE) seed[i] = a.R0[size_t(w) * K + i];
+  __syncthreads();
+  const int n = pb_top(a.pr, w, K, true, [](uint64_t) { return true; }, buf, seed, n0);
+  for (int i = threadIdx.x; i < K; i += TG_WAVE)
+    a.R1[size_t(w) * K + i] = i < n ? buf[i] : TG_KEY_INF;
+  if (threadIdx.x == 0) a.Rn1[w] = n;
+}
+
+// final merge of the reverse lists into the forward lists
+__global__ __launch_bounds__(64) void k_final(KnnArgs a) {
+  __shared__ RowLds row;
+  __shared__ uint64_t cand[128];
+  const uint32_t u = blockIdx.x;
+  if (u >= a.N) return;
+  const int lane = threadIdx.x, K = a.K;
+  load_row_lds(a, a.G, a.GD, u, row);
+  const float worst = key_d(row.gk[K - 1]);
+  const int nr = a.Rn0[u];
+  int nc = 0;
+  for (int base = 0; base < nr; base += TG_WAVE) {
+    const int i = base + lane;
+    bool ok = false;
+    uint64_t k = 0;
+    if (i < nr) {
+      k = a.R0[size_t(u) * K + i];
+      ok = key_id(k) != u && key_d(k) < worst && !listed(row, K, key_id(k));
+    }
+    const uint64_t m = ballot(ok);
+    if (ok) cand[nc + mask_rank(m)] = k;
+    nc += __popcll(m);
+  }
+  __syncthreads();
+  if (nc > 0)
+    merge_row(K, row.gk, row.vf, cand, nc, a.G + size_t(u) * K, a.GD + size_t(u) * K,
+              a.V + size_t(u) * K);
+}
+
+// ------------------------------------------------------------------------- launchers
+#define TG_MC_DISPATCH(KERNEL, GRID, BLOCK, S, ARGS)                 \
+  do {                                                               \
+    const int mc_ = (int(a.D) + 63) / 64;                            \
+    if (mc_ <= 2) KERNEL<2><<<GRID, BLOCK, 0, S>>> ARGS;             \
+    else if (mc_ <= 4) KERNEL<4><<<GRID, BLOCK, 0, S>>> ARGS;        \
+    else KERNEL<16><<<GRID, BLOCK, 0, S>>> ARGS;                     \
+  } while (0)
+
+void launch_knn_init(const KnnArgs& a, hipStream_t s) {
+  k_init_ids<<<(a.N + 63) / 64, 64, 0, s>>>(a);
+  TG_MC_DISPATCH(k_init_sort, a.N, 64, s, (a));
+}
+void launch_knn_sample(const KnnArgs& a, hipStream_t s) { k_sample<<<a.N, 64, 0, s>>>(a); }
+void launch_knn_rsample(const KnnArgs& a, hipStream_t s) { k_rsample<<<a.N, 64, 0, s>>>(a); }
+void launch_knn_localjoin(const KnnArgs& a, int precision, hipStream_t s) {
+  (void)precision;
+  TG_MC_DISPATCH(k_localjoin_exact, a.N, 256, s, (a));
+}
+void launch_knn_merge(const KnnArgs& a, hipStream_t s) { k_merge<<<a.N, 64, 0, s>>>(a); }
+void launch_knn_rev_build(const KnnArgs& a, hipStream_t s) { k_rev_build<<<a.N, 64, 0, s>>>(a); }
+void launch_knn_rev_reduce(const KnnArgs& a, hipStream_t s) { k_rev_reduce<<<a.N, 64, 0, s>>>(a); }
+void launch_knn_phase2(const KnnArgs& a, hipStream_t s) { TG_MC_DISPATCH(k_phase2, a.N, 64, s, (a)); }
+void launch_knn_rupdate(const KnnArgs& a, hipStream_t s) { k_rupdate<<<a.N, 64, 0, s>>>(a); }
+void launch_knn_final(const KnnArgs& a, hipStream_t s) { k_final<<<a.N, 64, 0, s>>>(a); }
+
+}  // namespace tg

@@ -1,0 +1,542 @@
+// prune.hip -- CFS (Collect-Filter-Store) pruning kernels for gfx950.
+//
+// One wave64 workgroup per node.  The Collect step (path beam search / two-hop list), the
+// incremental Filter and the Store of reverse edges follow the deterministic rule spec that
+// the CPU oracle restates (oracle/tagore_oracle.cpp), so the adjacency produced here is
+// bit-identical to it:
+//   collect_path  Tagore_src.cu:1209-1349     collect_2hop      Tagore_src.cu:1352-1396
+//   Filter_path   Tagore_src.cu:1399-1485     rules             Tagore_src.cu:1488-1495
+//   filter_reverse Tagore_src.cu:1537-1657    cal_ep_gpu        Tagore_src.cu:1889-2058
+//   filter_cagra  Tagore_src.cu:1668-1710     filter_dpg        Tagore_src.cu:1713-1785
+//   filter_reverse_1hop Tagore_src.cu:1812-1886
+// Distances are the canonical FP32 squared L2 on FP32 rows (16 lanes per distance, four
+// distances in flight per wave).  The kNN input graph is never modified: pruned rows go to
+// separate buffers, so one kNN build can feed several prunings.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace tg {
+
+constexpr int RING = 480;
+constexpr int POOL_MAX = 1024;  // K + 480 <= 608, padded to a power of two
+constexpr int W_MAX = 256;      // TOPM + K <= 256
+constexpr int EXPL_MAX = 32;
+
+// -------------------------------------------------------------------------------------
+// Beam search shared by cal_ep and path Collect (spec in the oracle's beam_search).
+// W: beam (sorted keys, TOPM) followed by the K newest candidates; dset: sorted ids of W.
+// The explored flag of a beam entry is represented by membership of its id in `expl`:
+// the beam never holds two finite entries with one id, and an expanded node that falls
+// out of the beam can never re-enter it (the beam's TOPM-th key only decreases), so the
+// two representations agree.
+// -------------------------------------------------------------------------------------
+template <int MC>
+__device__ void beam_search(const PruneArgs& a, const Row16<MC>& tgt, uint32_t start,
+                            int max_exp, uint64_t* W, uint32_t* dset, uint32_t* expl,
+                            uint16_t* cl, uint64_t* ring, uint32_t& ring_c,
+                            uint32_t& n_eval, uint32_t& n_adj) {
+  const int lane = threadIdx.x, g = lane >> 4, t = lane & 15;
+  const int K = a.K, TOPM = a.TOPM, D = a.D;
+  const float* X = a.X;
+  // start list
+  for (int j = lane; j < W_MAX; j += TG_WAVE) W[j] = TG_KEY_INF;
+  __syncthreads();
+  for (int j = lane; j < K; j += TG_WAVE) dset[j] = a.G[size_t(start) * K + j];
+  __syncthreads();
+  for (int j = g; j < K; j += 4) {
+    const uint32_t y = dset[j];
+    const float d = dist16<MC>(tgt, X + size_t(y) * D, D, t);
+    if (t == 0) {
+      const uint64_t k = mk_key(d, y);
+      W[j] = k;
+      if (ring) ring[j % RING] = k;
+    }
+  }
+  n_eval += K;
+  n_adj += 1;
+  ring_c = K;
+  __syncthreads();
+  wave_bitonic(W, W_MAX);
+  // B = W[0..min(TOPM,K)); W[min(TOPM,K)..TOPM) already padding.  W beyond TOPM keeps S's
+  // tail until overwritten; it is part of the first dedup set, as in the spec.
+  for (int i = lane; i < W_MAX; i += TG_WAVE)
+    dset[i] = (i < TOPM + K) ? key_id(W[i]) : TG_INV;
+  __syncthreads();
+  wave_bitonic(dset, W_MAX);
+  int nexpl = 0;
+  for (int r = 0; r < max_exp; ++r) {
+    // first explorable beam entry
+    int first = -1;
+    for (int base = 0; base < TOPM && first < 0; base += TG_WAVE) {
+      const int i = base + lane;
+      bool ok = false;
+      if (i < TOPM) {
+        const uint64_t k = W[i];
+        ok = key_finite(k);
+        if (ok) {
+          const uint32_t id = key_id(k);
+          for (int e = 0; e < nexpl; ++e) ok = ok && (expl[e] != id);
+        }
+      }
+      const uint64_t m = ballot(ok);
+      if (m) first = base + __ffsll((unsigned long long)m) - 1;
+    }
+    if (first < 0) break;
+    const uint32_t x = key_id(W[first]);
+    __syncthreads();
+    if (lane == 0) expl[nexpl] = x;
+    ++nexpl;
+    n_adj += 1;
+    // new candidates: dedup against dset, compact the non-duplicates in j order
+    int cnt = 0;
+    for (int base = 0; base < K; base += TG_WAVE) {
+      const int j = base + lane;
+      bool fresh = false;
+      uint32_t y = 0;
+      if (j < K) {
+        y = a.G[size_t(x) * K + j];
+        // lower_bound over dset[0..TOPM+K)
+        int lo = 0, hi = TOPM + K;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (dset[mid] < y) lo = mid + 1; else hi = mid;
+        }
+        fresh = !(lo < TOPM + K && dset[lo] == y);
+        W[TOPM + j] = mk_key(__uint_as_float(0x7F800000u), y);
+      }
+      const uint64_t m = ballot(fresh);
+      if (fresh) cl[cnt + mask_rank(m)] = uint16_t(j);
+      cnt += __popcll(m);
+    }
+    __syncthreads();
+    for (int q = g; q < cnt; q += 4) {
+      const int j = cl[q];
+      const uint32_t y = key_id(W[TOPM + j]);
+      const float d = dist16<MC>(tgt, X + size_t(y) * D, D, t);
+      if (t == 0) {
+        const uint64_t k = mk_key(d, y);
+        W[TOPM + j] = k;
+        if (ring) ring[(ring_c + q) % RING] = k;
+      }
+    }
+    ring_c += cnt;
+    n_eval += cnt;
+    __syncthreads();
+    wave_bitonic(W, W_MAX);
+    for (int i = lane; i < W_MAX; i += TG_WAVE)
+      dset[i] = (i < TOPM + K) ? key_id(W[i]) : TG_INV;
+    __syncthreads();
+    wave_bitonic(dset, W_MAX);
+  }
+}
+
+// -------------------------------------------------------------------------------------
+// Sort a pool of L keys (padded to n with TG_KEY_INF) and replace the id of every entry
+// whose id equals its predecessor's (original values) by INV.
+// -------------------------------------------------------------------------------------
+__device__ void sort_dedup(uint64_t* P, int n) {
+  const int lane = threadIdx.x;
+  wave_bitonic(P, n);
+  bool dup[POOL_MAX / TG_WAVE];
+#pragma unroll
+  for (int q = 0; q < POOL_MAX / TG_WAVE; ++q) {
+    const int i = q * TG_WAVE + lane;
+    dup[q] = (i < n && i > 0) ? (key_id(P[i]) == key_id(P[i - 1])) : false;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < POOL_MAX / TG_WAVE; ++q) {
+    const int i = q * TG_WAVE + lane;
+    if (dup[q]) P[i] = key_with_id(P[i], TG_INV);
+  }
+  __syncthreads();
+}
+
+// first index >= from, < L, whose id is not INV (and not `skip`); L if none
+__device__ int next_valid(const uint64_t* P, int from, int L, uint32_t skip) {
+  const int lane = threadIdx.x;
+  for (int base = from; base < L; base += TG_WAVE) {
+    const int i = base + lane;
+    bool ok = false;
+    if (i < L) {
+      const uint32_t id = key_id(P[i]);
+      ok = id != TG_INV && id != skip;
+    }
+    const uint64_t m = ballot(ok);
+    if (m) return base + __ffsll((unsigned long long)m) - 1;
+  }
+  return L;
+}
+
+// -------------------------------------------------------------------------------------
+// Incremental Filter over the sorted pool P[0..L) (spec A.4 in the oracle).  Writes the
+// kept entries to sel[] and returns their number.
+// -------------------------------------------------------------------------------------
+template <int MC>
+__device__ int filter_pool(const PruneArgs& a, uint64_t* P, int L, uint32_t u, int rule,
+                           uint64_t* sel, uint16_t* cl, uint32_t& n_eval) {
+  const int lane = threadIdx.x, g = lane >> 4, t = lane & 15;
+  const int D = a.D, FD = a.FD;
+  int tix = next_valid(P, 0, L, u);
+  if (tix >= L) return 0;
+  int nsel = 1;
+  if (lane == 0) sel[0] = P[tix];
+  float cur = key_d(P[tix]);
+  uint32_t pid = key_id(P[tix]);
+  ++tix;
+  __syncthreads();
+  while (nsel < FD) {
+    Row16<MC> prow;
+    load_row16<MC>(prow, a.X + size_t(pid) * D, D, t);
+    // compact the live candidates after tix; entries repeating p's id are invalidated
+    int cnt = 0;
+    for (int base = tix; base < L; base += TG_WAVE) {
+      const int i = base + lane;
+      bool live = false;
+      if (i < L) {
+        const uint32_t id = key_id(P[i]);
+        if (id == pid) P[i] = key_with_id(P[i], TG_INV);
+        live = id != TG_INV && id != pid;
+      }
+      const uint64_t m = ballot(live);
+      if (live) cl[cnt + mask_rank(m)] = uint16_t(i);
+      cnt += __popcll(m);
+    }
+    __syncthreads();
+    for (int q = g; q < cnt; q += 4) {
+      const int i = cl[q];
+      const uint64_t k = P[i];
+      const float res = dist16<MC>(prow, a.X + size_t(key_id(k)) * D, D, t);
+      if (t == 0 && rule_eval(rule, res, key_d(k), cur, a.th) < 0.0f)
+        P[i] = key_with_id(k, TG_INV);
+    }
+    n_eval += cnt;
+    __syncthreads();
+    tix = next_valid(P, tix, L, TG_INV);
+    if (tix < L) {
+      if (lane == 0) sel[nsel] = P[tix];
+      cur = key_d(P[tix]);
+      pid = key_id(P[tix]);
+    }
+    ++nsel;  // an exhausted pool appends a placeholder, dropped below
+    ++tix;
+    __syncthreads();
+    if (tix >= L) break;
+  }
+  return nsel - 1;  // the last appended entry is never kept (Tagore_src.cu:1462-1477)
+}
+
+__device__ __forceinline__ void count_add(unsigned long long* c, int idx, uint32_t v) {
+  if (c && threadIdx.x == 0 && v) atomicAdd(&c[idx], (unsigned long long)v);
+}
+
+// -------------------------------------------------------------------------------------
+// Forward pass (Collect + Filter + Store reverse pushes).  kind 0: path, 1: two-hop.
+// -------------------------------------------------------------------------------------
+template <int MC, int KIND>
+__global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
+  __shared__ uint64_t P[POOL_MAX];
+  __shared__ uint64_t W[W_MAX];
+  __shared__ uint32_t dset[W_MAX];
+  __shared__ uint32_t expl[EXPL_MAX];
+  __shared__ uint16_t cl[POOL_MAX];
+  __shared__ uint64_t sel[129];
+  const uint32_t u = blockIdx.x;
+  if (u >= a.N) return;
+  const int lane = threadIdx.x, g = lane >> 4, t = lane & 15;
+  const int K = a.K, D = a.D;
+  Row16<MC> xu;
+  load_row16<MC>(xu, a.X + size_t(u) * D, D, t);
+  for (int i = lane; i < POOL_MAX; i += TG_WAVE)
+    P[i] = (i < K + RING) ? mk_key(__uint_as_float(0x7F800000u), TG_INV) : TG_KEY_INF;
+  __syncthreads();
+  for (int j = lane; j < K; j += TG_WAVE)
+    P[j] = mk_key(a.GD[size_t(u) * K + j], a.G[size_t(u) * K + j]);
+  uint32_t n_col = 0, n_adj = 0, n_flt = 0;
+  if (KIND == 0) {
+    uint32_t ring_c = 0;
+    beam_search<MC>(a, xu, *a.ep, 10, W, dset, expl, cl, P + K, ring_c, n_col, n_adj);
+  } else {
+    // two-hop ids y_i = G[G[u][i/K]][i%K], i < 480
+    for (int i = lane; i < RING; i += TG_WAVE) {
+      const uint32_t v = a.G[size_t(u) * K + i / K];
+      P[K + i] = mk_key(0.f, a.G[size_t(v) * K + (i % K)]);
+    }
+    __syncthreads();
+    for (int i = g; i < RING; i += 4) {
+      const uint32_t y = key_id(P[K + i]);
+      const float d = dist16<MC>(xu, a.X + size_t(y) * D, D, t);
+      if (t == 0) P[K + i] = mk_key(d, y);
+    }
+    n_col = RING;
+    n_adj = (RING + K - 1) / K;
+  }
+  __syncthreads();
+  const int L = K + RING;
+  sort_dedup(P, pow2_at_least(L));
+  const int deg = filter_pool<MC>(a, P, L, u, a.fwd_rule, sel, cl, n_flt);
+  __syncthreads();
+  for (int k = lane; k < deg; k += TG_WAVE) {
+    const uint64_t s = sel[k];
+    a.fwd_ids[size_t(u) * a.FD + k] = key_id(s);
+    a.fwd_d[size_t(u) * a.FD + k] = key_d(s);
+    pb_push(a.rev, key_id(s), mk_key(key_d(s), u));
+  }
+  if (lane == 0) a.fwd_deg[u] = deg;
+  count_add(a.counters, 0, n_col);
+  count_add(a.counters, 1, n_flt);
+  count_add(a.counters, 3, n_adj);
+}
+
+// -------------------------------------------------------------------------------------
+// Reverse pass (filter_reverse, spec A.7): forward list plus the K best reverse pushes;
+// re-filtered with the reverse rule when more than FD, otherwise their union.
+// -------------------------------------------------------------------------------------
+template <int MC>
+__global__ __launch_bounds__(64) void k_reverse(PruneArgs a) {
+  __shared__ uint64_t buf[256];
+  __shared__ uint64_t P[256];
+  __shared__ uint16_t cl[256];
+  __shared__ uint64_t sel[129];
+  const uint32_t u = blockIdx.x;
+  if (u >= a.N) return;
+  const int lane = threadIdx.x;
+  const int K = a.K, FD = a.FD;
+  const int nr = pb_top(a.rev, u, K, false, [](uint64_t) { return true; }, buf);
+  const int deg = a.fwd_deg[u];
+  const int L = deg + nr;
+  uint32_t* row = a.out_ids + size_t(u) * FD;
+  uint32_t n_flt = 0;
+  if (L > FD) {
+    const int n = pow2_at_least(L);
+    for (int i = lane; i < n; i += TG_WAVE) {
+      uint64_t k = TG_KEY_INF;
+      if (i < deg) k = mk_key(a.fwd_d[size_t(u) * FD + i], a.fwd_ids[size_t(u) * FD + i]);
+      else if (i < L) k = buf[i - deg];
+      P[i] = k;
+    }
+    __syncthreads();
+    sort_dedup(P, n);
+    const int nd = filter_pool<MC>(a, P, L, u, a.rev_rule, sel, cl, n_flt);
+    __syncthreads();
+    for (int k = lane; k < FD; k += TG_WAVE) row[k] = k < nd ? key_id(sel[k]) : TG_INV;
+    if (lane == 0) a.out_deg[u] = nd;
+  } else {
+    // union: forward ids in order, then reverse ids (R order) not already present
+    const uint32_t* f = a.fwd_ids + size_t(u) * FD;
+    for (int k = lane; k < deg; k += TG_WAVE) row[k] = f[k];
+    bool add = false;
+    uint32_t w = 0;
+    if (lane < nr) {
+      w = key_id(buf[lane]);
+      add = true;
+      for (int j = 0; j < deg; ++j) add = add && (f[j] != w);
+    }
+    // nr <= L <= FD <= 128; handle lanes 64..127 in a second step
+    const uint64_t m0 = ballot(add);
+    if (add) row[deg + mask_rank(m0)] = w;
+    int n = deg + __popcll(m0);
+    if (nr > TG_WAVE) {
+      bool add2 = false;
+      uint32_t w2 = 0;
+      if (TG_WAVE + lane < nr) {
+        w2 = key_id(buf[TG_WAVE + lane]);
+        add2 = true;
+        for (int j = 0; j < deg; ++j) add2 = add2 && (f[j] != w2);
+      }
+      const uint64_t m1 = ballot(add2);
+      if (add2) row[n + mask_rank(m1)] = w2;
+      n += __popcll(m1);
+    }
+    for (int k = n + lane; k < FD; k += TG_WAVE) row[k] = TG_INV;
+    if (lane == 0) a.out_deg[u] = n;
+  }
+  count_add(a.counters, 2, n_flt);
+}
+
+// -------------------------------------------------------------------------------------
+// One-hop rank (CAGRA: id membership; DPG: occlusion by distance), spec A.8 / A.9.
+// -------------------------------------------------------------------------------------
+template <int MC, int DPG>
+__global__ __launch_bounds__(64) void k_rank_forward(PruneArgs a) {
+  __shared__ uint64_t srt[256];   // (id << 32 | pos) sorted, then (detour << 32 | id)
+  __shared__ uint32_t det[128];
+  __shared__ uint32_t nei[128];
+  const uint32_t u = blockIdx.x;
+  if (u >= a.N) return;
+  const int lane = threadIdx.x, g = lane >> 4, t = lane & 15;
+  const int K = a.K, D = a.D, d = a.FD;
+  const int n = pow2_at_least(K);
+  for (int j = lane; j < 128; j += TG_WAVE) {
+    det[j] = 0;
+    nei[j] = j < K ? a.G[size_t(u) * K + j] : TG_INV;
+  }
+  __syncthreads();
+  if (!DPG) {
+    for (int j = lane; j < n; j += TG_WAVE)
+      srt[j] = j < K ? ((uint64_t(nei[j]) << 32) | uint32_t(j)) : TG_KEY_INF;
+    __syncthreads();
+    wave_bitonic(srt, n);
+    for (int r = 0; r < K; ++r) {
+      const uint32_t v = nei[r];
+      for (int l = lane; l < K; l += TG_WAVE) {
+        const uint32_t y = a.G[size_t(v) * K + l];
+        int lo = 0, hi = K;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (uint32_t(srt[mid] >> 32) < y) lo = mid + 1; else hi = mid;
+        }
+        if (lo < K && uint32_t(srt[lo] >> 32) == y) {
+          const int j = int(uint32_t(srt[lo]));
+          if (j > r) atomicAdd(&det[j], 1u);
+        }
+      }
+    }
+    count_add(a.counters, 3, K);
+  } else {
+    // pairs r < j, row by row; the four groups take every fourth j of a row
+    const int npairs = K * (K - 1) / 2;
+    for (int r = 0; r + 1 < K; ++r) {
+      const float* xr = a.X + size_t(nei[r]) * D;
+      for (int j = r + 1 + g; j < K; j += 4) {
+        const float dd = dist16_mm<MC>(xr, a.X + size_t(nei[j]) * D, D, t);
+        if (t == 0 && dd < a.GD[size_t(u) * K + j]) atomicAdd(&det[j], 1u);
+      }
+    }
+    count_add(a.counters, 1, npairs);
+  }
+  __syncthreads();
+  for (int j = lane; j < n; j += TG_WAVE)
+    srt[j] = j < K ? ((uint64_t(det[j]) << 32) | nei[j]) : TG_KEY_INF;
+  __syncthreads();
+  wave_bitonic(srt, n);
+  for (int i = lane; i < d; i += TG_WAVE) {
+    const uint64_t k = srt[i];
+    a.fwd_ids[size_t(u) * d + i] = uint32_t(k);
+    pb_push(a.rev, uint32_t(k), (k & 0xFFFFFFFF00000000ull) | u);
+  }
+  if (lane == 0) a.fwd_deg[u] = d;
+}
+
+// filter_reverse_1hop (spec A.10): half forward, up to half reverse, forward fill.
+__global__ __launch_bounds__(64) void k_rank_reverse(PruneArgs a) {
+  __shared__ uint64_t buf[256];
+  __shared__ uint32_t A[128];
+  const uint32_t u = blockIdx.x;
+  if (u >= a.N) return;
+  const int lane = threadIdx.x;
+  const int d = a.FD, h = d / 2;
+  // R1[u]: the d smallest (detour, src) pushes, already sorted by (detour, id)
+  const int r = pb_top(a.rev, u, d, false, [](uint64_t) { return true; }, buf);
+  const uint32_t* T = a.fwd_ids + size_t(u) * d;
+  uint32_t* row = a.out_ids + size_t(u) * d;
+  // Rv sorted by (id, detour); later copies of an id get detour INV; then by (detour,id).
+  // Pushes carry distinct sources, so the relabelling never fires; it is kept for parity.
+  if (lane == 0) {
+    // insertion-sort by id on buf[128..128+r)
+    for (int i = 0; i < r; ++i) {
+      const uint64_t k = (uint64_t(uint32_t(buf[i])) << 32) | uint32_t(buf[i] >> 32);
+      int j = i;
+      while (j > 0 && buf[128 + j - 1] > k) {
+        buf[128 + j] = buf[128 + j - 1];
+        --j;
+      }
+      buf[128 + j] = k;
+    }
+    for (int i = r - 1; i >= 1; --i)
+      if (uint32_t(buf[128 + i] >> 32) == uint32_t(buf[128 + i - 1] >> 32))
+        buf[128 + i] = (buf[128 + i] & 0xFFFFFFFF00000000ull) | TG_INV;
+    // back to (detour << 32 | id) order
+    for (int i = 0; i < r; ++i) {
+      const uint64_t k = (uint64_t(uint32_t(buf[128 + i])) << 32) | uint32_t(buf[128 + i] >> 32);
+      int j = i;
+      while (j > 0 && buf[j - 1] > k) {
+        buf[j] = buf[j - 1];
+        --j;
+      }
+      buf[j] = k;
+    }
+    int na = 0;
+    const int lim = (r < h) ? (d - r) : h;
+    for (int i = 0; i < r; ++i) {
+      if (r >= h && na >= h) break;
+      const uint32_t w = uint32_t(buf[i]);
+      bool in = false;
+      for (int j = 0; j < lim; ++j) in = in || (T[j] == w);
+      if (!in) A[na++] = w;
+    }
+    for (int i = 0; i < h; ++i) row[i] = T[i];
+    for (int i = 0; i < na; ++i) row[h + i] = A[i];
+    for (int i = 0; h + na + i < d; ++i) row[h + na + i] = T[h + i];
+    a.out_deg[u] = d;
+  }
+}
+
+// -------------------------------------------------------------------------------------
+// Entry point: one workgroup, beam of TOPM from node 0's list toward the centroid.
+// -------------------------------------------------------------------------------------
+template <int MC>
+__global__ __launch_bounds__(64) void k_cal_ep(PruneArgs a, const float* centroid, uint32_t* ep) {
+  __shared__ uint64_t W[W_MAX];
+  __shared__ uint32_t dset[W_MAX];
+  __shared__ uint32_t expl[EXPL_MAX];
+  __shared__ uint16_t cl[W_MAX];
+  Row16<MC> c;
+  load_row16<MC>(c, centroid, a.D, threadIdx.x & 15);
+  uint32_t rc = 0, ne = 0, na = 0;
+  beam_search<MC>(a, c, 0u, 30, W, dset, expl, cl, nullptr, rc, ne, na);
+  if (threadIdx.x == 0) *ep = key_id(W[0]);
+  count_add(a.counters, 4, ne);
+}
+
+// Deterministic centroid: FP64 sums over chunks of 1024 rows, then over chunks in order.
+__global__ void k_centroid_part(const float* X, uint32_t N, uint32_t D, double* part) {
+  const uint32_t j = blockIdx.y * blockDim.x + threadIdx.x;
+  if (j >= D) return;
+  const size_t lo = size_t(blockIdx.x) * 1024, hi = min(size_t(N), lo + 1024);
+  double s = 0.0;
+  for (size_t i = lo; i < hi; ++i) s += double(X[i * D + j]);
+  part[size_t(blockIdx.x) * D + j] = s;
+}
+__global__ void k_centroid_final(const double* part, uint32_t nch, uint32_t N, uint32_t D,
+                                 float* c) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= D) return;
+  double s = 0.0;
+  for (uint32_t ch = 0; ch < nch; ++ch) s += part[size_t(ch) * D + j];
+  c[j] = float(s / double(N));
+}
+
+// ------------------------------------------------------------------------- launchers
+template <int MC>
+static void launch_prune_mc(const PruneArgs& a, int phase, const float* centroid, uint32_t* ep,
+                            hipStream_t s) {
+  const dim3 grid(a.N), block(64);
+  switch (phase) {
+    case PH_EP: k_cal_ep<MC><<<1, 64, 0, s>>>(a, centroid, ep); break;
+    case PH_FWD_PATH: k_forward<MC, 0><<<grid, block, 0, s>>>(a); break;
+    case PH_FWD_2HOP: k_forward<MC, 1><<<grid, block, 0, s>>>(a); break;
+    case PH_REV: k_reverse<MC><<<grid, block, 0, s>>>(a); break;
+    case PH_CAGRA: k_rank_forward<MC, 0><<<grid, block, 0, s>>>(a); break;
+    case PH_DPG: k_rank_forward<MC, 1><<<grid, block, 0, s>>>(a); break;
+    case PH_RANK_REV: k_rank_reverse<<<grid, block, 0, s>>>(a); break;
+    default: break;
+  }
+}
+
+void launch_prune(const PruneArgs& a, int phase, const float* centroid, uint32_t* ep,
+                  hipStream_t s) {
+  const int mc = (a.D + 63) / 64;
+  if (mc <= 2) launch_prune_mc<2>(a, phase, centroid, ep, s);
+  else if (mc <= 4) launch_prune_mc<4>(a, phase, centroid, ep, s);
+  else launch_prune_mc<16>(a, phase, centroid, ep, s);
+}
+
+void launch_centroid(const float* X, uint32_t N, uint32_t D, double* part, float* c,
+                     hipStream_t s) {
+  const uint32_t nch = (N + 1023) / 1024;
+  k_centroid_part<<<dim3(nch, (D + 63) / 64), 64, 0, s>>>(X, N, D, part);
+  k_centroid_final<<<(D + 63) / 64, 64, 0, s>>>(part, nch, N, D, c);
+}
+
+}  // namespace tg

@@ -1,0 +1,210 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the CPU oracle.
+
+Pruning must be bit-identical (ids, order, degrees, entry point, evaluation counts and
+index-file bytes); the exact-mode kNN build must be bit-identical to the oracle's
+GNN-Descent restatement; recall is judged against FP64 brute force.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import tagore_amd
+from tagore_amd import State
+from tests.conftest import gmm_int, gmm_unit
+
+pytestmark = pytest.mark.gpu
+
+INV = 0xFFFFFFFF
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def int_case():
+    X = gmm_int(4000, 128, 32, 101)
+    G, GD = oracle.knn_bruteforce(X, 64)
+    return X, G, GD
+
+
+@pytest.fixture(scope="module")
+def unit_case():
+    X = gmm_unit(3000, 96, 16, 202)
+    G, GD = oracle.knn_bruteforce(X, 48)
+    return X, G, GD
+
+
+def _gpu_prune(X, G, GD, kind, fr, rr, FD, TOPM, th):
+    with State(*X.shape) as s:
+        s.set_data(X)
+        s.set_knn(G, GD)
+        s.prune(kind, fr, rr, FD, TOPM, th)
+        deg, ids, ep = s.get_pruned()
+        fdeg, fids, fd = s.get_forward()
+        st = s.stats()
+    return deg, ids, ep, fdeg, fids, fd, st
+
+
+def _compare(X, G, GD, kind, fr, rr, FD, TOPM, th):
+    _say(f"prune parity kind={kind} fwd={fr} rev={rr} FD={FD} TOPM={TOPM} th={th} "
+         f"N={X.shape[0]} D={X.shape[1]} K={G.shape[1]}")
+    ref = oracle.prune(X, G, GD, FD, TOPM, kind, fr, rr, th, stages=True)
+    deg, ids, ep, fdeg, fids, fd, st = _gpu_prune(X, G, GD, kind, fr, rr, FD, TOPM, th)
+    N = X.shape[0]
+    assert np.array_equal(fdeg, ref["fwd_deg"])
+    for u in range(N):
+        n = fdeg[u]
+        assert np.array_equal(fids[u, :n], ref["fwd_ids"][u, :n]), f"forward row {u}"
+        if kind < 2:
+            assert np.array_equal(fd[u, :n], ref["fwd_d"][u, :n]), f"forward dists {u}"
+    assert np.array_equal(deg, ref["deg"])
+    for u in range(N):
+        assert np.array_equal(ids[u, :deg[u]], ref["ids"][u, :deg[u]]), f"row {u}"
+    if kind == 0:
+        assert ep == ref["ep"]
+    c = ref["counters"]
+    pc = st["prune_counters"]
+    assert pc["collect"] == c["collect_evals"]
+    assert pc["filter"] == c["filter_evals"]
+    assert pc["reverse"] == c["reverse_evals"]
+    assert pc["adj_rows"] == c["adj_rows"]
+    if kind == 0:
+        assert pc["ep"] == c["ep_evals"]
+    return ref
+
+
+@pytest.mark.parametrize("name,kind,fr,rr,th", [
+    ("NSG", 0, 0, 0, 1.0),
+    ("Vamana", 0, 0, 0, 1.2),
+    ("CFS path/angle", 0, 0, 1, 0.5),
+    ("CFS 2-hop/dist", 1, 1, 0, 1.0),
+    ("CAGRA", 2, 0, 2, 0.0),
+    ("DPG", 3, 0, 2, 0.0),
+])
+def test_prune_parity_int(int_case, name, kind, fr, rr, th):
+    X, G, GD = int_case
+    _compare(X, G, GD, kind, fr, rr, 32, 64, th)
+
+
+@pytest.mark.parametrize("name,kind,fr,rr,th", [
+    ("NSSG", 1, 1, 1, 0.5),
+    ("NSG", 0, 0, 0, 1.0),
+    ("Vamana", 0, 0, 0, 1.2),
+])
+def test_prune_parity_unit(unit_case, name, kind, fr, rr, th):
+    X, G, GD = unit_case
+    _compare(X, G, GD, kind, fr, rr, 32, 64, th)
+
+
+def test_prune_parity_small_k_and_odd_sizes():
+    # K < TOPM (beam padding), FD > K/2, a degree cap that is reached often
+    X = gmm_int(1500, 64, 8, 7)
+    G, GD = oracle.knn_bruteforce(X, 32)
+    _compare(X, G, GD, 0, 0, 0, 24, 64, 1.0)
+    _compare(X, G, GD, 0, 0, 0, 6, 16, 1.3)
+    _compare(X, G, GD, 1, 1, 1, 30, 64, 0.3)
+
+
+def test_index_files_are_byte_identical(int_case, tmp_path):
+    X, G, GD = int_case
+    with State(*X.shape) as s:
+        s.set_data(X)
+        s.set_knn(G, GD)
+        for name, (kind, fr, rr, fixed) in tagore_amd.INDEX_TYPES.items():
+            _say(f"index file {name}")
+            th = fixed if fixed is not None else (0.5 if kind == 1 else 1.2)
+            s.prune(kind, fr, rr, 32, 64, th)
+            p_gpu = str(tmp_path / f"gpu.{name}")
+            s.store_index(p_gpu, name, 0x5EED)
+            ref = oracle.prune(X, G, GD, 32, 64, kind, fr, rr, th)
+            p_ref = str(tmp_path / f"ref.{name}")
+            oracle.store(p_ref, name, ref["deg"], ref["ids"], ref["ep"],
+                         oracle.nssg_eps(X.shape[0], 0x5EED))
+            assert open(p_gpu, "rb").read() == open(p_ref, "rb").read(), name
+
+
+@pytest.mark.parametrize("N,D,K,iters,kind", [(3000, 128, 32, 6, "int"), (2500, 96, 48, 8, "unit"),
+                                              (1200, 64, 64, 4, "int")])
+def test_knn_exact_bit_identical(N, D, K, iters, kind):
+    _say(f"kNN exact N={N} D={D} K={K} iters={iters}")
+    X = gmm_int(N, D, 16, 5) if kind == "int" else gmm_unit(N, D, 16, 5)
+    seed = 0x1234
+    Gr, GDr, ev = oracle.gnn_descent(X, K, iters, seed)
+    with State(N, D) as s:
+        s.set_data(X)
+        s.knn_build(K, iters, seed, "exact")
+        G, GD = s.get_knn()
+        st = s.stats()
+    bad = np.nonzero((G != Gr).any(1))[0]
+    assert bad.size == 0, f"{bad.size} rows differ, first {bad[:5]}"
+    assert np.array_equal(GD, GDr)
+    assert st["knn_evals"] == ev
+
+
+def test_recall_judge_matches_oracle():
+    X = gmm_unit(3000, 96, 16, 9)
+    with State(*X.shape) as s:
+        s.set_data(X)
+        s.knn_build(32, 4, 77, "exact")
+        G, _ = s.get_knn()
+        r_gpu = s.recall(500, 10, 0x5A11)
+    perm = np.arange(X.shape[0])
+    # same query selection as tg_recall: SplitMix64 Fisher-Yates prefix
+    M = (1 << 64) - 1
+    x = 0x5A11
+    for i in range(500):
+        x = (x + 0x9E3779B97F4A7C15) & M
+        z = x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        z ^= z >> 31
+        j = i + z % (X.shape[0] - i)
+        perm[i], perm[j] = perm[j], perm[i]
+    q = perm[:500].astype(np.uint32)
+    _, kth = oracle.knn_f64(X, q, 10)
+    r_ref = oracle.recall(X, G, q, 10, kth)
+    assert abs(r_gpu - r_ref) < 1e-12
+    assert 0.5 < r_gpu <= 1.0
+
+
+def test_generated_data_properties():
+    with State(20000, 128) as s:
+        s.gen_data(0, 64, 42, 20.0)
+        X = s.get_data()
+        s.gen_data(0, 64, 42, 20.0)
+        assert np.array_equal(X, s.get_data())  # deterministic
+    assert X.min() >= 0 and X.max() <= 255 and np.array_equal(X, np.rint(X))
+    with State(5000, 96) as s:
+        s.gen_data(1, 32, 3)
+        U = s.get_data()
+    assert np.allclose(np.linalg.norm(U.astype(np.float64), axis=1), 1.0, atol=1e-6)
+    with State(5000, 64) as s:
+        s.gen_data(2, 32, 3)
+        A = s.get_data()
+    assert A.min() >= 0 and A.max() == 1.0
+
+
+def test_dropin_module_end_to_end(tmp_path):
+    from tagore_amd import Tagore
+
+    X = gmm_int(2000, 128, 16, 31)
+    p = tmp_path / "base.fvecs"
+    rec = np.empty((X.shape[0], X.shape[1] + 1), np.float32)
+    rec[:, 0] = np.frombuffer(np.uint32(128).tobytes(), np.float32)[0]
+    rec[:, 1:] = X
+    rec.tofile(p)
+    ptrs = Tagore.GNN_descent(32, 2000, 128, 6, str(p), seed=99)
+    assert len(ptrs) == 8
+    idx = str(tmp_path / "nsg.index")
+    Tagore.Pruning(32, 2000, 128, 24, 64, ptrs, "NSG", index_path=idx)
+    Tagore.Pruning_with_CFS(32, 2000, 128, 24, 64, ptrs, "path", "dist", 1.2)
+    with pytest.raises(ValueError):
+        Tagore.Pruning_with_CFS(32, 2000, 128, 24, 64, ptrs, "1-hop", "dist", 1.0)
+    G, GD, _ = oracle.gnn_descent(X, 32, 6, 99)
+    ref = oracle.prune(X, G, GD, 24, 64, 0, 0, 0, 1.0)
+    p_ref = str(tmp_path / "ref.index")
+    oracle.store(p_ref, "NSG", ref["deg"], ref["ids"], ref["ep"])
+    assert open(idx, "rb").read() == open(p_ref, "rb").read()
