@@ -1,14 +1,21 @@
 #!/usr/bin/env python3
 """Benchmark: kNN-graph construction (GNN-Descent) + NSG pruning on one MI355X.
 
-One step = GNN-Descent (K=96, 10 iterations) over N=1,000,000 x 128 vectors resident in HBM,
-followed by CFS path/dist/1.0 pruning (NSG) with FINAL_DEGREE=32, m=64.  Data: a seeded
-integer-valued clipped 1024-component Gaussian mixture generated on the device.  The JSON
-line reports vectors/s over the timed steps, the roofline position of the dominant kernel
-(timed live with HIP events on the library's stream) and the CPU oracle timed on a bounded
-sample of the same workload.
+Workload (the 1M configuration): N=1,000,000 x 128 integer-valued clipped 1024-component
+Gaussian mixture, generated on the device from a fixed seed; GNN-Descent K=96; CFS
+path/dist/1.0 pruning (NSG) with FINAL_DEGREE=32, m=64.  One step = kNN build + prune on
+data already resident in HBM.
 
-    python bench.py [--gpus 1] [--steps 2] [--warmup 1] [--precision exact]
+Two paths are timed in the same run:
+  * plain FP32: FP32 MFMA screen + canonical FP32 re-score, random init, 10 iterations
+    (the configuration the reference runs: pybind.cu:101 with iter=10);
+  * headline:   BF16 MFMA screen + canonical FP32 re-score (--precision, --iters, --init).
+`value` is the headline's vectors/s over the timed steps; the plain path's figure and the
+ratio are reported beside it.  Both graphs are judged by FP64 recall@10 on a seeded sample.
+The roofline entry is the pruning forward kernel (HBM-bound gather), timed live with HIP
+events on the library's stream; the CPU oracle runs the same pipeline on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps 2] [--warmup 1]
 """
 import argparse
 import json
@@ -26,6 +33,44 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def measure(st, args, precision, iters, label):
+    import tagore_amd
+
+    def step():
+        st.knn_build(args.k, iters, args.seed, precision)
+        st.prune("path", "dist", "dist", args.fd, args.topm, 1.0)
+
+    for w in range(args.warmup):
+        t = time.perf_counter()
+        step()
+        log(f"{label}: warmup {w} done in {time.perf_counter() - t:.3f}s")
+    sec_knn = {k: 0.0 for k in tagore_amd.KNN_SECTIONS}
+    sec_prune = {k: 0.0 for k in tagore_amd.PRUNE_SECTIONS}
+    knn_ms, prune_ms = [], []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+        s = st.stats()
+        knn_ms.append(s["knn_ms"])
+        prune_ms.append(s["prune_ms"])
+        for k, v in s["knn_kernel_ms"].items():
+            sec_knn[k] += v / args.steps
+        for k, v in s["prune_kernel_ms"].items():
+            sec_prune[k] += v / args.steps
+        log(f"{label}: step {i}: knn {s['knn_ms']:.1f} ms, prune {s['prune_ms']:.1f} ms")
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / args.steps
+    last = st.stats()
+    rec = st.recall(min(args.recall_queries, st.N), 10, 0x5A11)
+    log(f"{label}: {ms:.1f} ms/step, recall@10 = {rec:.4f}")
+    return dict(ms_per_step=ms, vectors_per_s=st.N / (ms / 1e3), recall_at_10=rec,
+                precision=precision, iters=iters, init="random",
+                knn_ms=sum(knn_ms) / len(knn_ms), prune_ms=sum(prune_ms) / len(prune_ms),
+                knn_sections=sec_knn, prune_sections=sec_prune,
+                knn_evals=last["knn_evals"], knn_rescored=last["knn_rescored"],
+                prune_counters=last["prune_counters"])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -34,11 +79,14 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--dim", type=int, default=128)
     ap.add_argument("--k", type=int, default=96)
-    ap.add_argument("--iters", type=int, default=10)
+    ap.add_argument("--iters", type=int, default=10, help="headline iterations")
+    ap.add_argument("--plain-iters", type=int, default=10)
     ap.add_argument("--fd", type=int, default=32)
     ap.add_argument("--topm", type=int, default=64)
     ap.add_argument("--ncomp", type=int, default=1024)
-    ap.add_argument("--precision", default="exact")
+    ap.add_argument("--precision", default="bf16", help="headline precision")
+    ap.add_argument("--plain-precision", default="fp32")
+    ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--seed", type=int, default=0x7A600002)
     ap.add_argument("--recall-queries", type=int, default=10000)
     ap.add_argument("--cpu-sample", type=int, default=30000,
@@ -47,63 +95,37 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(clusters would be sharded per GPU; see the design notes).", file=sys.stderr)
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the design notes describe how clusters would shard across GPUs).",
+              file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import tagore_amd
 
     N, D, K = args.n, args.dim, args.k
-    log(f"workload N={N} D={D} K={K} iters={args.iters} NSG FD={args.fd} m={args.topm} "
-        f"precision={args.precision}")
+    log(f"workload N={N} D={D} K={K} NSG FD={args.fd} m={args.topm}")
     st = tagore_amd.State(N, D, 0)
     st.gen_data(0, args.ncomp, args.seed, 20.0)
     st.set_profiling(True)
 
-    def step():
-        st.knn_build(K, args.iters, args.seed, args.precision)
-        st.prune("path", "dist", "dist", args.fd, args.topm, 1.0)
+    plain = None
+    if not args.no_plain:
+        plain = measure(st, args, args.plain_precision, args.plain_iters, "plain")
+    head = measure(st, args, args.precision, args.iters, "headline")
 
-    for w in range(args.warmup):
-        t = time.perf_counter()
-        step()
-        log(f"warmup {w} done in {time.perf_counter() - t:.3f}s")
-
-    knn_ms, prune_ms = [], []
-    sec_knn = {k: 0.0 for k in tagore_amd.KNN_SECTIONS}
-    sec_prune = {k: 0.0 for k in tagore_amd.PRUNE_SECTIONS}
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        step()
-        s = st.stats()
-        knn_ms.append(s["knn_ms"])
-        prune_ms.append(s["prune_ms"])
-        for k, v in s["knn_kernel_ms"].items():
-            sec_knn[k] += v
-        for k, v in s["prune_kernel_ms"].items():
-            sec_prune[k] += v
-        log(f"step {i}: knn {s['knn_ms']:.1f} ms, prune {s['prune_ms']:.1f} ms")
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    value = N / (ms_per_step / 1e3)
-    last = st.stats()
-
-    rec = st.recall(min(args.recall_queries, N), 10, 0x5A11)
-    log(f"recall@10 = {rec:.4f}")
-
-    # ---- roofline of the pruning forward kernel (Collect + Filter), HBM bound
-    pc = last["prune_counters"]
-    fwd_ms = sec_prune["forward"] / args.steps
+    # roofline of the pruning forward kernel (Collect + Filter): algorithmic bytes = every
+    # canonical distance evaluation reads one FP32 row, plus adjacency rows, own lists, output
+    pc = head["prune_counters"]
+    fwd_ms = head["prune_sections"]["forward"]
     evals = pc["collect"] + pc["filter"]
     alg_bytes = evals * D * 4 + pc["adj_rows"] * K * 4 + N * K * 8 + N * args.fd * 8
     achieved = alg_bytes / (fwd_ms / 1e3) / 1e9 if fwd_ms > 0 else None
-    roof = {"bound": "hbm", "kernel": "prune_forward_path", "achieved": achieved,
-            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+    roof = {"bound": "hbm", "kernel": "k_forward (NSG path collect + filter)",
+            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
-            "algorithmic_bytes": alg_bytes, "kernel_ms": fwd_ms}
+            "algorithmic_bytes": alg_bytes, "kernel_ms": fwd_ms,
+            "evaluations": evals}
 
-    # ---- CPU oracle on a bounded sample of the same workload
     cpu = None
     if args.cpu_sample > 0:
         import oracle
@@ -111,28 +133,29 @@ def main():
         n = min(args.cpu_sample, N)
         Xs = st.get_data(0, n)
         t = time.perf_counter()
-        G, GD, _ = oracle.gnn_descent(Xs, K, args.iters, args.seed)
+        G, GD, _ = oracle.gnn_descent(Xs, K, args.plain_iters, args.seed)
         oracle.prune(Xs, G, GD, args.fd, args.topm, 0, 0, 0, 1.0)
         dt = time.perf_counter() - t
         cpu = {"value": n / dt, "unit": "vectors/s", "cores": oracle.num_threads(),
                "kind": "port", "seconds": dt,
-               "sample": f"first {n} vectors of the same generated set, same K/iters/NSG params"}
+               "sample": f"first {n} vectors of the same generated set; GNN-Descent K={K} "
+                         f"{args.plain_iters} iterations + NSG FD={args.fd} m={args.topm}"}
         log(f"cpu baseline: {n} vectors in {dt:.1f}s")
     st.close()
 
     out = {
         "metric": "kNN+prune vectors/s at recall@10>=0.95: learned/BF16 vs plain FP32, 1 GPU",
-        "value": value, "unit": "vectors/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"GNN-Descent K={K} iters={args.iters} + NSG prune FD={args.fd} "
-                               f"m={args.topm} over {N}x{D} integer GMM({args.ncomp})",
+        "value": head["vectors_per_s"], "unit": "vectors/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": args.precision, "data": "synthetic",
+        "config": {"workload": f"GNN-Descent K={K} + NSG prune FD={args.fd} m={args.topm} over "
+                               f"{N}x{D} integer GMM({args.ncomp}), sigma 20",
                    "N": N, "dim": D, "K": K, "iters": args.iters, "final_degree": args.fd,
                    "m": args.topm, "precision": args.precision, "init": "random"},
-        "recall_at_10": rec,
-        "device_ms": {"knn": sum(knn_ms) / len(knn_ms), "prune": sum(prune_ms) / len(prune_ms),
-                      "knn_sections": {k: v / args.steps for k, v in sec_knn.items()},
-                      "prune_sections": {k: v / args.steps for k, v in sec_prune.items()}},
+        "recall_at_10": head["recall_at_10"],
+        "headline": head,
+        "plain_fp32": plain,
+        "speedup_vs_plain": (plain["ms_per_step"] / head["ms_per_step"]) if plain else None,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -114,6 +114,7 @@ typedef struct {
   double knn_kernel_ms[8];   /* init, sample, rsample, localjoin, merge, reverse, phase2, final */
   double prune_kernel_ms[4]; /* ep, forward, reverse, other */
   uint64_t knn_evals;        /* distance evaluations of the last build */
+  uint64_t knn_rescored;     /* canonical FP32 re-scores after the screen (phase 2 + join) */
   uint64_t prune_counters[8];/* collect, filter, reverse, adjacency rows, ep evaluations */
 } tg_stats;
 int tg_get_stats(tg_state* s, tg_stats* out);

@@ -56,7 +56,8 @@ class PruneParams(ctypes.Structure):
 class Stats(ctypes.Structure):
     _fields_ = [("knn_ms", ctypes.c_double), ("prune_ms", ctypes.c_double),
                 ("knn_kernel_ms", ctypes.c_double * 8), ("prune_kernel_ms", ctypes.c_double * 4),
-                ("knn_evals", ctypes.c_uint64), ("prune_counters", ctypes.c_uint64 * 8)]
+                ("knn_evals", ctypes.c_uint64), ("knn_rescored", ctypes.c_uint64),
+                ("prune_counters", ctypes.c_uint64 * 8)]
 
 
 KNN_SECTIONS = ["init", "sample", "rsample", "localjoin", "merge", "reverse", "phase2", "final"]
@@ -217,7 +218,7 @@ class State:
         return dict(knn_ms=s.knn_ms, prune_ms=s.prune_ms,
                     knn_kernel_ms=dict(zip(KNN_SECTIONS, list(s.knn_kernel_ms))),
                     prune_kernel_ms=dict(zip(PRUNE_SECTIONS, list(s.prune_kernel_ms))),
-                    knn_evals=int(s.knn_evals),
+                    knn_evals=int(s.knn_evals), knn_rescored=int(s.knn_rescored),
                     prune_counters=dict(zip(["collect", "filter", "reverse", "adj_rows", "ep"],
                                             list(s.prune_counters)[:5])))
 

@@ -99,6 +99,8 @@ struct tg_state {
   uint64_t *R0 = nullptr, *R1 = nullptr;
   uint32_t *Rn0 = nullptr, *Rn1 = nullptr;
   PB pn, po, pc, pr;
+  uint16_t* Xb = nullptr;  // BF16 copy of X (screen)
+  float *nrm2 = nullptr, *nq = nullptr, *ne = nullptr;
   uint32_t* fill = nullptr;
   uint32_t* scan_tmp = nullptr;
   unsigned long long* counters = nullptr;  // [16]
@@ -228,6 +230,13 @@ KnnArgs knn_args(tg_state* s, uint64_t seed) {
   a.Rn1 = s->Rn1;
   a.seed = seed;
   a.counters = s->counters;
+  a.Xb = s->Xb;
+  a.nrm2 = s->nrm2;
+  a.nq = s->nq;
+  a.ne = s->ne;
+  // relative slack of the screen bound: accumulation (2 D ulps per term, doubled) plus the
+  // canonical distance's own rounding, with margin
+  a.eta = float((8.0 * s->D + 128.0) * 5.9604644775390625e-8);
   return a;
 }
 
@@ -334,6 +343,10 @@ int tg_free(tg_state* s) {
   s->pc.release();
   s->pr.release();
   s->rev.release();
+  dfree(s->Xb);
+  dfree(s->nrm2);
+  dfree(s->nq);
+  dfree(s->ne);
   dfree(s->fill);
   dfree(s->scan_tmp);
   dfree(s->counters);
@@ -446,10 +459,18 @@ int tg_knn_build(tg_state* s, const tg_knn_params* p) {
   const uint32_t K = p->K;
   if (K < 2 || K > 128) return fail(TG_ERR_ARG, "K must be in [2, 128]");
   if (K >= s->N) return fail(TG_ERR_ARG, "K must be smaller than N");
-  if (p->precision != TG_PREC_EXACT)
-    return fail(TG_ERR_ARG, "only exact precision is available in this build");
+  if (p->precision < TG_PREC_EXACT || p->precision > TG_PREC_BF16)
+    return fail(TG_ERR_ARG, "unknown precision");
+  if (p->precision == TG_PREC_BF16 && s->D % 8 != 0)
+    return fail(TG_ERR_ARG, "BF16 screening needs D % 8 == 0");
   HIPX(hipSetDevice(s->device));
   if (ensure_graph(s, K) || ensure_knn_scratch(s, K)) return TG_ERR_HIP;
+  if (p->precision != TG_PREC_EXACT && !s->nrm2) {
+    if (dalloc(&s->nrm2, s->N) || dalloc(&s->nq, s->N) || dalloc(&s->ne, s->N)) return TG_ERR_HIP;
+  }
+  if (p->precision == TG_PREC_BF16 && !s->Xb) {
+    if (dalloc(&s->Xb, size_t(s->N) * s->D)) return TG_ERR_HIP;
+  }
   hipStream_t st = s->stream;
   if (ensure_events(s, 2)) return TG_ERR_HIP;
   // window events (separate from the per-section pool)
@@ -464,6 +485,9 @@ int tg_knn_build(tg_state* s, const tg_knn_params* p) {
   KnnArgs a = knn_args(s, p->seed);
   const uint32_t N = s->N;
   if (sec.start(0)) return TG_ERR_HIP;
+  if (p->precision != TG_PREC_EXACT)
+    launch_screen_prep(s->X, N, s->D, p->precision == TG_PREC_BF16 ? s->Xb : nullptr, s->nrm2,
+                       s->nq, s->ne, st);
   launch_knn_init(a, st);
   if (sec.stop()) return TG_ERR_HIP;
   const uint32_t it1 = p->iters / 2, it2 = p->iters / 2;
@@ -498,7 +522,7 @@ int tg_knn_build(tg_state* s, const tg_knn_params* p) {
   for (uint32_t it = 0; it < it2; ++it) {
     if (sec.start(6)) return TG_ERR_HIP;
     pb_reset(a.pr, N, s->fill, st);
-    launch_knn_phase2(a, st);
+    launch_knn_phase2(a, p->precision, st);
     pb_group_overflow(a.pr, N, s->fill, s->scan_tmp, st);
     launch_knn_rupdate(a, st);
     if (sec.stop()) return TG_ERR_HIP;
@@ -525,6 +549,7 @@ int tg_knn_build(tg_state* s, const tg_knn_params* p) {
   unsigned long long cnt[8];
   HIPX(hipMemcpy(cnt, s->counters, sizeof(cnt), hipMemcpyDeviceToHost));
   s->stats.knn_evals = cnt[0];
+  s->stats.knn_rescored = cnt[1];
   if (check_pb_err(s, s->pn, "reverse sampling") || check_pb_err(s, s->po, "reverse sampling") ||
       check_pb_err(s, s->pc, "local join") || check_pb_err(s, s->pr, "reverse lists"))
     return TG_ERR_CAPACITY;

@@ -4,7 +4,7 @@
 //   non-negative distances used everywhere (+inf sorts after every finite distance).
 // * Canonical FP32 squared L2 evaluated by a 16-lane group with float4 loads: lane t owns
 //   the partial sums p[4t..4t+3] over i == l (mod 64); the tree p[l] += p[l+s] runs as
-//   shuffle-down steps 8,4,2,1 across the group and the last two levels inside lane 0.
+//   DPP row shifts 8,4,2,1 across the group and the last two levels inside lane 0.
 //   This is the same arithmetic, in the same order, as the CPU oracle (bit-identical).
 // * Wave-level (64-lane) bitonic sorts and scans in LDS.
 // * PushBuf: per-destination append lists with a bounded slot area and a global overflow
@@ -42,8 +42,8 @@ __device__ __forceinline__ int mask_rank(uint64_t m) {
 
 // ---------------------------------------------------------------------------------------
 // Canonical distance.  `MC` = number of 64-element chunks held (ceil(D/64) <= MC).
-// Row a is preloaded by the group (ra[]), row b is read from memory.  Every lane of the
-// group returns the same value.  Requires D % 4 == 0 (host checks).
+// Row a is preloaded by the group (ra[]), row b is read from memory.  The value is valid in
+// lane 0 of the group.  Requires D % 4 == 0 (host checks).
 // ---------------------------------------------------------------------------------------
 template <int MC>
 struct Row16 {
@@ -63,19 +63,70 @@ __device__ __forceinline__ void load_row16(Row16<MC>& r, const float* __restrict
   }
 }
 
-// tree reduction of the group's four partials (lane t holds p[4t..4t+3])
+// lane i of each 16-lane DPP row reads lane i+S of the same row (lanes past the row end read
+// 0; only lanes whose partner exists are used)
+template <int S>
+__device__ __forceinline__ float row_shl(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x100 + S, 0xF, 0xF,
+                                                    false));
+}
+
+// Tree reduction of the group's partials (lane t holds p[4t..4t+3]): the steps s = 32,16,8,4
+// pair lane t with lane t+8, t+4, t+2, t+1 (DPP row shifts), s = 2,1 run inside lane 0.
+// The result is valid in lane 0 of the group (t == 0) only.
 __device__ __forceinline__ float tree16(float p0, float p1, float p2, float p3) {
-#pragma unroll
-  for (int s = 8; s >= 1; s >>= 1) {
-    p0 = p0 + __shfl_down(p0, s, 16);
-    p1 = p1 + __shfl_down(p1, s, 16);
-    p2 = p2 + __shfl_down(p2, s, 16);
-    p3 = p3 + __shfl_down(p3, s, 16);
-  }
+  p0 = p0 + row_shl<8>(p0);
+  p1 = p1 + row_shl<8>(p1);
+  p2 = p2 + row_shl<8>(p2);
+  p3 = p3 + row_shl<8>(p3);
+  p0 = p0 + row_shl<4>(p0);
+  p1 = p1 + row_shl<4>(p1);
+  p2 = p2 + row_shl<4>(p2);
+  p3 = p3 + row_shl<4>(p3);
+  p0 = p0 + row_shl<2>(p0);
+  p1 = p1 + row_shl<2>(p1);
+  p2 = p2 + row_shl<2>(p2);
+  p3 = p3 + row_shl<2>(p3);
+  p0 = p0 + row_shl<1>(p0);
+  p1 = p1 + row_shl<1>(p1);
+  p2 = p2 + row_shl<1>(p2);
+  p3 = p3 + row_shl<1>(p3);
   p0 = p0 + p2;
   p1 = p1 + p3;
-  p0 = p0 + p1;
-  return __shfl(p0, 0, 16);
+  return p0 + p1;
+}
+
+// NB distances from row a (registers) to rows b[0..NB) (b[i] may be null: result unused);
+// all loads are issued before the arithmetic.  Results valid in lane t == 0.
+template <int MC, int NB>
+__device__ __forceinline__ void dist16_batch(const Row16<MC>& a, const float* const (&b)[NB],
+                                             int D, int t, float (&out)[NB]) {
+  float4 y[NB][MC];
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+#pragma unroll
+    for (int m = 0; m < MC; ++m) {
+      const int i = m * 64 + 4 * t;
+      y[k][m] = (b[k] && i < D) ? *reinterpret_cast<const float4*>(b[k] + i)
+                                : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    float p0 = 0.f, p1 = 0.f, p2 = 0.f, p3 = 0.f;
+#pragma unroll
+    for (int m = 0; m < MC; ++m) {
+      if (m * 64 + 4 * t < D) {
+        const float e0 = a.v[m].x - y[k][m].x, e1 = a.v[m].y - y[k][m].y,
+                    e2 = a.v[m].z - y[k][m].z, e3 = a.v[m].w - y[k][m].w;
+        p0 = __builtin_fmaf(e0, e0, p0);
+        p1 = __builtin_fmaf(e1, e1, p1);
+        p2 = __builtin_fmaf(e2, e2, p2);
+        p3 = __builtin_fmaf(e3, e3, p3);
+      }
+    }
+    out[k] = tree16(p0, p1, p2, p3);
+  }
 }
 
 template <int MC>
@@ -275,6 +326,61 @@ __device__ __forceinline__ int lower_bound_lds(const T* a, int n, T x) {
     if (a[mid] < x) lo = mid + 1; else hi = mid;
   }
   return lo;
+}
+
+// Is the key (d, id) an entry of kNN row v?  Rows are sorted by (distance, id) and hold
+// canonical distances, so a listed id always appears under exactly this key.
+__device__ __forceinline__ bool row_has_key(const uint32_t* G, const float* GD, int K, uint32_t v,
+                                            uint64_t key) {
+  const uint32_t* g = G + size_t(v) * K;
+  const float* gd = GD + size_t(v) * K;
+  int lo = 0, hi = K;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (mk_key(gd[mid], g[mid]) < key) lo = mid + 1; else hi = mid;
+  }
+  return lo < K && mk_key(gd[lo], g[lo]) == key;
+}
+
+template <class T>
+__device__ __forceinline__ int upper_bound_lds(const T* a, int n, T x) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (!(x < a[mid])) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// dst[0..na+nb) = stable merge of the sorted arrays A[0..na) and B[0..nb) (A first on ties)
+__device__ void merge_lds(const uint64_t* A, int na, const uint64_t* B, int nb, uint64_t* dst) {
+  for (int i = threadIdx.x; i < na; i += TG_WAVE) dst[i + lower_bound_lds(B, nb, A[i])] = A[i];
+  for (int j = threadIdx.x; j < nb; j += TG_WAVE) dst[j + upper_bound_lds(A, na, B[j])] = B[j];
+}
+
+// Evaluates the targets listed in ids[0..n) (selected through idx when given) against the
+// register row `tgt`, four candidates per 16-lane group at a time; calls emit(q, d) in lane
+// t == 0 of the group that owns item q.
+template <int MC, class IdOf, class Emit>
+__device__ __forceinline__ void eval_batch(const float* X, int D, const Row16<MC>& tgt, int n,
+                                           IdOf id_of, Emit emit) {
+  const int lane = threadIdx.x, g = lane >> 4, t = lane & 15;
+  for (int q0 = 0; q0 < n; q0 += 16) {
+    const float* rows[4];
+    int qs[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      qs[b] = q0 + g + 4 * b;
+      rows[b] = qs[b] < n ? X + size_t(id_of(qs[b])) * D : nullptr;
+    }
+    float dd[4];
+    dist16_batch<MC, 4>(tgt, rows, D, t, dd);
+    if (t == 0) {
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        if (qs[b] < n) emit(qs[b], dd[b]);
+    }
+  }
 }
 
 }  // namespace tg

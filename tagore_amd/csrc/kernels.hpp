@@ -64,7 +64,19 @@ struct KnnArgs {
   uint32_t* Rn1;    // [N]
   uint64_t seed;
   unsigned long long* counters;  // [8] or null
+  // screening (precision FP32 / BF16): a screen distance s = |a|^2 + |b|^2 - 2 a'.b' on the
+  // screen copies a', b' with a rigorous bound |s - dist(a,b)| <= E(a,b); only candidates
+  // whose bound interval can change a decision are re-scored with the canonical distance.
+  const uint16_t* Xb;  // [N*D] BF16 rows (precision BF16)
+  const float* nrm2;   // [N] |x|^2 (FP64 sum rounded once)
+  const float* nq;     // [N] |x'| of the screen copy (rounded up)
+  const float* ne;     // [N] |x' - x| (rounded up)
+  float eta;           // relative slack of the bound
 };
+
+// screen-mode preparation: BF16 copy (if Xb), norms and quantisation-error norms
+void launch_screen_prep(const float* X, uint32_t N, uint32_t D, uint16_t* Xb, float* nrm2,
+                        float* nq, float* ne, hipStream_t s);
 
 void launch_knn_init(const KnnArgs& a, hipStream_t s);
 void launch_knn_sample(const KnnArgs& a, hipStream_t s);
@@ -73,7 +85,7 @@ void launch_knn_localjoin(const KnnArgs& a, int precision, hipStream_t s);
 void launch_knn_merge(const KnnArgs& a, hipStream_t s);
 void launch_knn_rev_build(const KnnArgs& a, hipStream_t s);
 void launch_knn_rev_reduce(const KnnArgs& a, hipStream_t s);
-void launch_knn_phase2(const KnnArgs& a, hipStream_t s);
+void launch_knn_phase2(const KnnArgs& a, int precision, hipStream_t s);
 void launch_knn_rupdate(const KnnArgs& a, hipStream_t s);
 void launch_knn_final(const KnnArgs& a, hipStream_t s);
 

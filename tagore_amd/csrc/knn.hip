@@ -38,20 +38,34 @@ __device__ __forceinline__ uint32_t sm_bounded(uint64_t x, uint32_t n) {
 }
 
 // ------------------------------------------------------------------------------- init
-__global__ void k_init_ids(KnnArgs a) {
-  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+// The i-th draw of node u's SplitMix64 stream is a pure function of (stream start, i), so a
+// wave evaluates 64 draws at once and accepts them in stream order: a draw is taken unless it
+// names u, repeats an accepted id, or repeats an earlier draw of the same batch (an earlier
+// equal draw is either accepted or rejected for a reason that applies to this one too).
+__global__ __launch_bounds__(64) void k_init_ids(KnnArgs a) {
+  __shared__ uint32_t acc[128];
+  const uint32_t u = blockIdx.x;
   if (u >= a.N) return;
-  uint32_t* row = a.G + size_t(u) * a.K;
-  uint64_t st = sm_stream(a.seed, u);
-  uint32_t cnt = 0;
-  while (cnt < a.K) {
-    const uint32_t c = sm_bounded(sm_next(st), a.N);
-    if (c == u) continue;
-    bool dup = false;
-    for (uint32_t j = 0; j < cnt; ++j) dup |= (row[j] == c);
-    if (dup) continue;
-    row[cnt++] = c;
+  const int lane = threadIdx.x, K = a.K;
+  const uint64_t st0 = sm_stream(a.seed, u);
+  int cnt = 0;
+  for (uint64_t base = 0; cnt < K; base += TG_WAVE) {
+    uint64_t s = st0 + (base + lane) * 0x9E3779B97F4A7C15ull;
+    const uint32_t c = sm_bounded(sm_next(s), a.N);
+    bool ok = c != u;
+    for (int j = 0; j < cnt && ok; ++j) ok = acc[j] != c;
+    for (int j = 0; j < TG_WAVE; ++j) {
+      const uint32_t cj = __shfl(c, j, TG_WAVE);
+      if (j < lane && cj == c) ok = false;
+    }
+    const uint64_t m = ballot(ok);
+    const int rk = cnt + mask_rank(m);
+    __syncthreads();
+    if (ok && rk < K) acc[rk] = c;
+    cnt = min(K, cnt + int(__popcll(m)));
+    __syncthreads();
   }
+  for (int j = lane; j < K; j += TG_WAVE) a.G[size_t(u) * K + j] = acc[j];
 }
 
 template <int MC>
@@ -65,11 +79,9 @@ __global__ __launch_bounds__(64) void k_init_sort(KnnArgs a) {
   load_row16<MC>(xu, a.X + size_t(u) * D, D, t);
   for (int j = lane; j < n; j += TG_WAVE) kk[j] = TG_KEY_INF;
   __syncthreads();
-  for (int j = g; j < K; j += 4) {
-    const uint32_t y = a.G[size_t(u) * K + j];
-    const float d = dist16<MC>(xu, a.X + size_t(y) * D, D, t);
-    if (t == 0) kk[j] = mk_key(d, y);
-  }
+  const uint32_t* row = a.G + size_t(u) * K;
+  eval_batch<MC>(a.X, D, xu, K, [&](int q) { return row[q]; },
+                 [&](int q, float d) { kk[q] = mk_key(d, row[q]); });
   __syncthreads();
   wave_bitonic(kk, n);
   for (int j = lane; j < K; j += TG_WAVE) {
@@ -173,16 +185,34 @@ __global__ __launch_bounds__(256) void k_localjoin_exact(KnnArgs a) {
   for (int i = gg; i < nA; i += 16) {
     Row16<MC> ra;
     load_row16<MC>(ra, a.X + size_t(A[i]) * D, D, t);
-    for (int j = 0; j < nB; ++j) {
-      float d = INF;
-      if (A[i] != B[j]) d = dist16<MC>(ra, a.X + size_t(B[j]) * D, D, t);
-      if (t == 0) MAB[i * 64 + j] = d;
-    }
-    for (int j = i + 1; j < nA; ++j) {
-      const float d = dist16<MC>(ra, a.X + size_t(A[j]) * D, D, t);
+    // row i against B[0..nB) then A[i+1..nA), four rows in flight per group
+    const int nj = nB + (nA - i - 1);
+    for (int j0 = 0; j0 < nj; j0 += 4) {
+      const float* rows[4];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int j = j0 + b;
+        rows[b] = nullptr;
+        if (j < nB) {
+          if (A[i] != B[j]) rows[b] = a.X + size_t(B[j]) * D;
+        } else if (j < nj) {
+          rows[b] = a.X + size_t(A[i + 1 + (j - nB)]) * D;
+        }
+      }
+      float dd[4];
+      dist16_batch<MC, 4>(ra, rows, D, t, dd);
       if (t == 0) {
-        MAA[i * 64 + j] = d;
-        MAA[j * 64 + i] = d;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const int j = j0 + b;
+          if (j < nB) {
+            MAB[i * 64 + j] = rows[b] ? dd[b] : INF;
+          } else if (j < nj) {
+            const int jj = i + 1 + (j - nB);
+            MAA[i * 64 + jj] = dd[b];
+            MAA[jj * 64 + i] = dd[b];
+          }
+        }
       }
     }
     if (t == 0) MAA[i * 64 + i] = INF;
@@ -196,7 +226,8 @@ __global__ __launch_bounds__(256) void k_localjoin_exact(KnnArgs a) {
       const float d = MAB[r * 64 + j];
       if (d < INF) best = min(best, mk_key(d, B[j]));
     }
-    if (best != TG_KEY_INF && key_d(best) < wA[r]) pb_push(a.pc, A[r], best);
+    if (best != TG_KEY_INF && key_d(best) < wA[r] && !row_has_key(a.G, a.GD, K, A[r], best))
+      pb_push(a.pc, A[r], best);
   }
   // columns of B against tiles of A
   if (r < nB && lo < nA) {
@@ -205,7 +236,8 @@ __global__ __launch_bounds__(256) void k_localjoin_exact(KnnArgs a) {
       const float d = MAB[i * 64 + r];
       if (d < INF) best = min(best, mk_key(d, A[i]));
     }
-    if (best != TG_KEY_INF && key_d(best) < wB[r]) pb_push(a.pc, B[r], best);
+    if (best != TG_KEY_INF && key_d(best) < wB[r] && !row_has_key(a.G, a.GD, K, B[r], best))
+      pb_push(a.pc, B[r], best);
   }
   // rows of A against tiles of A
   if (r < nA && lo < nA) {
@@ -214,7 +246,8 @@ __global__ __launch_bounds__(256) void k_localjoin_exact(KnnArgs a) {
       const float d = MAA[r * 64 + j];
       if (d < INF) best = min(best, mk_key(d, A[j]));
     }
-    if (best != TG_KEY_INF && key_d(best) < wA[r]) pb_push(a.pc, A[r], best);
+    if (best != TG_KEY_INF && key_d(best) < wA[r] && !row_has_key(a.G, a.GD, K, A[r], best))
+      pb_push(a.pc, A[r], best);
   }
   if (a.counters && tid == 0)
     atomicAdd(&a.counters[0], (unsigned long long)(nA * nB + nA * (nA - 1) / 2));
@@ -316,7 +349,56 @@ __global__ __launch_bounds__(64) void k_rev_reduce(KnnArgs a) {
 }
 
 // ------------------------------------------------------------------------- phase 2
-template <int MC>
+// BF16 screen of one candidate: lane t of the 16-lane group holds u' elements 128m + 8t..+7.
+template <int MB>
+struct RowBf {
+  float v[MB][8];
+};
+__device__ __forceinline__ float bfbits2f(uint32_t bits16) { return __uint_as_float(bits16 << 16); }
+
+template <int MB>
+__device__ __forceinline__ void load_rowbf(RowBf<MB>& r, const uint16_t* x, int D, int t) {
+#pragma unroll
+  for (int m = 0; m < MB; ++m) {
+    const int i = 128 * m + 8 * t;
+    uint4 q = make_uint4(0, 0, 0, 0);
+    if (i < D) q = *reinterpret_cast<const uint4*>(x + i);
+    const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      r.v[m][2 * k] = bfbits2f(w[k] & 0xFFFFu);
+      r.v[m][2 * k + 1] = bfbits2f(w[k] >> 16);
+    }
+  }
+}
+
+template <int MB>
+__device__ __forceinline__ float dot_bf16(const RowBf<MB>& a, const uint16_t* y, int D, int t) {
+  float s = 0.f;
+#pragma unroll
+  for (int m = 0; m < MB; ++m) {
+    const int i = 128 * m + 8 * t;
+    if (i < D) {
+      const uint4 q = *reinterpret_cast<const uint4*>(y + i);
+      const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        s = __builtin_fmaf(a.v[m][2 * k], bfbits2f(w[k] & 0xFFFFu), s);
+        s = __builtin_fmaf(a.v[m][2 * k + 1], bfbits2f(w[k] >> 16), s);
+      }
+    }
+  }
+#pragma unroll
+  for (int o = 8; o >= 1; o >>= 1) s += __shfl_xor(s, o, 16);
+  return s;
+}
+
+__device__ __forceinline__ float screen_bound2(float na, float qa, float ea, float nb, float qb,
+                                               float eb, float eta) {
+  return 2.0f * (ea * qb + qa * eb + ea * eb) + eta * (na + nb + 2.0f * qa * qb);
+}
+
+template <int MC, int PREC>
 __global__ __launch_bounds__(64) void k_phase2(KnnArgs a) {
   __shared__ RowLds row;
   __shared__ uint32_t ids[512];
@@ -374,15 +456,46 @@ __global__ __launch_bounds__(64) void k_phase2(KnnArgs a) {
   load_row16<MC>(xu, a.X + size_t(u) * D, D, t);
   const float worst = key_d(row.gk[K - 1]);
   const int nk = pow2_at_least(max(cnt, 1));
-  for (int q = g; q < nk; q += 4) {
-    uint64_t k = TG_KEY_INF;
-    if (q < cnt) {
-      const uint32_t y = ids[cl[q]];
-      const float d = dist16<MC>(xu, a.X + size_t(y) * D, D, t);
-      if (d < worst) k = mk_key(d, y);
-    }
-    if (t == 0) kv[q] = k;
+  constexpr int MB = (MC + 1) / 2;
+  RowBf<PREC == KP_BF16 ? MB : 1> ub;
+  float nu = 0.f, qu = 0.f, eu = 0.f;
+  if (PREC == KP_BF16) {
+    load_rowbf(ub, a.Xb + size_t(u) * D, D, t);
+    nu = a.nrm2[u];
+    qu = a.nq[u];
+    eu = a.ne[u];
   }
+  for (int q = lane; q < nk; q += TG_WAVE) kv[q] = TG_KEY_INF;
+  // BF16 screen: keep (in cl order) only the candidates whose bound interval reaches below
+  // the worst distance; everything kept is re-scored exactly
+  int nre = cnt;
+  if (PREC == KP_BF16) {
+    __shared__ uint16_t keep[512];
+    for (int q = g; q < cnt; q += 4) {
+      const uint32_t y = ids[cl[q]];
+      const float s = (nu + a.nrm2[y]) - 2.0f * dot_bf16(ub, a.Xb + size_t(y) * D, D, t);
+      const float E = screen_bound2(nu, qu, eu, a.nrm2[y], a.nq[y], a.ne[y], a.eta);
+      if (t == 0) keep[q] = (s - E < worst) ? 1 : 0;
+    }
+    __syncthreads();
+    nre = 0;
+    for (int base = 0; base < cnt; base += TG_WAVE) {
+      const int q = base + lane;
+      const bool k = q < cnt && keep[q];
+      const uint16_t c = q < cnt ? cl[q] : 0;
+      const uint64_t m = ballot(k);
+      __syncthreads();
+      if (k) cl[nre + mask_rank(m)] = c;
+      nre += __popcll(m);
+      __syncthreads();
+    }
+  }
+  __syncthreads();
+  eval_batch<MC>(a.X, D, xu, nre, [&](int q) { return ids[cl[q]]; },
+                 [&](int q, float d) {
+                   if (d < worst) kv[q] = mk_key(d, ids[cl[q]]);
+                 });
+  if (a.counters && lane == 0 && nre) atomicAdd(&a.counters[1], (unsigned long long)nre);
   __syncthreads();
   wave_bitonic(kv, nk);
   int nc = 0;
@@ -457,19 +570,34 @@ __global__ __launch_bounds__(64) void k_final(KnnArgs a) {
   } while (0)
 
 void launch_knn_init(const KnnArgs& a, hipStream_t s) {
-  k_init_ids<<<(a.N + 63) / 64, 64, 0, s>>>(a);
+  k_init_ids<<<a.N, 64, 0, s>>>(a);
   TG_MC_DISPATCH(k_init_sort, a.N, 64, s, (a));
 }
 void launch_knn_sample(const KnnArgs& a, hipStream_t s) { k_sample<<<a.N, 64, 0, s>>>(a); }
 void launch_knn_rsample(const KnnArgs& a, hipStream_t s) { k_rsample<<<a.N, 64, 0, s>>>(a); }
+void launch_knn_localjoin_screen(const KnnArgs& a, int precision, hipStream_t s);
+
 void launch_knn_localjoin(const KnnArgs& a, int precision, hipStream_t s) {
-  (void)precision;
-  TG_MC_DISPATCH(k_localjoin_exact, a.N, 256, s, (a));
+  if (precision == KP_EXACT)
+    TG_MC_DISPATCH(k_localjoin_exact, a.N, 256, s, (a));
+  else
+    launch_knn_localjoin_screen(a, precision, s);
 }
 void launch_knn_merge(const KnnArgs& a, hipStream_t s) { k_merge<<<a.N, 64, 0, s>>>(a); }
 void launch_knn_rev_build(const KnnArgs& a, hipStream_t s) { k_rev_build<<<a.N, 64, 0, s>>>(a); }
 void launch_knn_rev_reduce(const KnnArgs& a, hipStream_t s) { k_rev_reduce<<<a.N, 64, 0, s>>>(a); }
-void launch_knn_phase2(const KnnArgs& a, hipStream_t s) { TG_MC_DISPATCH(k_phase2, a.N, 64, s, (a)); }
+void launch_knn_phase2(const KnnArgs& a, int precision, hipStream_t s) {
+  const int mc = (int(a.D) + 63) / 64;
+  if (precision == KP_BF16) {
+    if (mc <= 2) k_phase2<2, KP_BF16><<<a.N, 64, 0, s>>>(a);
+    else if (mc <= 4) k_phase2<4, KP_BF16><<<a.N, 64, 0, s>>>(a);
+    else k_phase2<16, KP_BF16><<<a.N, 64, 0, s>>>(a);
+  } else {
+    if (mc <= 2) k_phase2<2, KP_EXACT><<<a.N, 64, 0, s>>>(a);
+    else if (mc <= 4) k_phase2<4, KP_EXACT><<<a.N, 64, 0, s>>>(a);
+    else k_phase2<16, KP_EXACT><<<a.N, 64, 0, s>>>(a);
+  }
+}
 void launch_knn_rupdate(const KnnArgs& a, hipStream_t s) { k_rupdate<<<a.N, 64, 0, s>>>(a); }
 void launch_knn_final(const KnnArgs& a, hipStream_t s) { k_final<<<a.N, 64, 0, s>>>(a); }
 

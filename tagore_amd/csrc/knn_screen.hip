@@ -1,0 +1,265 @@
+// knn_screen.hip -- screened local join on MFMA (FP32 or BF16) with exact FP32 re-scoring.
+//
+// The local join of GNN-Descent (nn_descent_opt_cal, Tagore_src.cu:684-1033) needs, for every
+// sampled node list A = new'(u) (<= 64 rows) and B = old'(u) (<= 64 rows), the per-16-wide-
+// tile minimum of the pair distances A x B, B x A and A x A.  Here the Gram matrices A.B^T and
+// A.A^T are contracted on the matrix cores from LDS-staged, coalesced row loads:
+//   BF16: v_mfma_f32_16x16x32_bf16 on a BF16 copy of the vectors,
+//   FP32: v_mfma_f32_16x16x4_f32 on the FP32 vectors,
+// and turned into screen distances s = |a|^2 + |b|^2 - 2 a'.b'.  A rigorous bound E(a,b) on
+// |s - dist(a,b)| (quantisation + accumulation + canonical-distance rounding) selects, per
+// tile, the only candidates that can be the exact (distance, id) minimum and pass the
+// destination's worst distance; those are re-scored with the canonical FP32 distance.  The
+// pushes are therefore bit-identical to the exact kernel's: graph decisions stay FP32.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace tg {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int LJ_TILE = 16;
+constexpr int HROW_S = 128;
+
+// round-to-nearest-even FP32 -> BF16 bits
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return uint16_t((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }
+
+// per-vector screen statistics; one wave per row, FP64 sums
+__global__ void k_screen_prep(const float* X, uint32_t N, uint32_t D, uint16_t* Xb, float* nrm2,
+                              float* nq, float* ne) {
+  const uint32_t row = blockIdx.x;
+  if (row >= N) return;
+  const float* x = X + size_t(row) * D;
+  double s2 = 0.0, q2 = 0.0, e2 = 0.0;
+  for (uint32_t j = threadIdx.x; j < D; j += TG_WAVE) {
+    const float v = x[j];
+    float w = v;
+    if (Xb) {
+      const uint16_t b = f2bf(v);
+      Xb[size_t(row) * D + j] = b;
+      w = bf2f(b);
+    }
+    s2 += double(v) * double(v);
+    q2 += double(w) * double(w);
+    e2 += (double(w) - double(v)) * (double(w) - double(v));
+  }
+  for (int o = 32; o >= 1; o >>= 1) {
+    s2 += __shfl_xor(s2, o, TG_WAVE);
+    q2 += __shfl_xor(q2, o, TG_WAVE);
+    e2 += __shfl_xor(e2, o, TG_WAVE);
+  }
+  if (threadIdx.x == 0) {
+    nrm2[row] = float(s2);
+    // norms rounded up, so the bound stays an upper bound
+    nq[row] = float(sqrt(q2) * (1.0 + 1e-6));
+    ne[row] = float(sqrt(e2) * (1.0 + 1e-6));
+  }
+}
+
+void launch_screen_prep(const float* X, uint32_t N, uint32_t D, uint16_t* Xb, float* nrm2,
+                        float* nq, float* ne, hipStream_t s) {
+  k_screen_prep<<<N, TG_WAVE, 0, s>>>(X, N, D, Xb, nrm2, nq, ne);
+}
+
+__device__ __forceinline__ float screen_bound(float na, float qa, float ea, float nb, float qb,
+                                              float eb, float eta) {
+  return 2.0f * (ea * qb + qa * eb + ea * eb) + eta * (na + nb + 2.0f * qa * qb);
+}
+
+// -------------------------------------------------------------------------------------
+// Screened local join.  PREC 1: FP32 MFMA (16x16x4), PREC 2: BF16 MFMA (16x16x32).
+// 256 threads: wave w owns row block w (16 rows of A) against the 4 column blocks of B and
+// of A.  Staging chunk: DC elements per row (128 BF16 or 64 FP32 = 256 B).
+// -------------------------------------------------------------------------------------
+template <int PREC, int MC>
+__global__ __launch_bounds__(256) void k_localjoin_screen(KnnArgs a) {
+  constexpr int DC = PREC == 2 ? 128 : 64;          // elements per staged chunk
+  constexpr int ESZ = PREC == 2 ? 2 : 4;            // element bytes
+  constexpr int RS = DC * ESZ + 16;                 // padded row stride in bytes
+  constexpr int STAGE = 64 * RS;                    // one operand
+  constexpr int MROW = 65;                          // padded distance-matrix row (floats)
+  static_assert(2 * STAGE >= 2 * 64 * MROW * 4, "distance matrices alias the staging area");
+  __shared__ __attribute__((aligned(16))) unsigned char smem[2 * STAGE];
+  __shared__ uint32_t A[64], B[64];
+  __shared__ float naA[64], qA[64], eA[64], wA[64];
+  __shared__ float naB[64], qB[64], eB[64], wB[64];
+  unsigned char* sA = smem;
+  unsigned char* sB = smem + STAGE;
+  float* MAB = reinterpret_cast<float*>(smem);
+  float* MAA = MAB + 64 * MROW;
+
+  const uint32_t u = blockIdx.x;
+  if (u >= a.N) return;
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+  const int K = a.K, D = a.D;
+  const int nA = a.na[u], nB = a.nb[u];
+  if (tid < 64) {
+    if (tid < nA) {
+      const uint32_t v = a.H[size_t(u) * HROW_S + tid];
+      A[tid] = v;
+      naA[tid] = a.nrm2[v];
+      qA[tid] = a.nq[v];
+      eA[tid] = a.ne[v];
+      wA[tid] = a.GD[size_t(v) * K + K - 1];
+    }
+  } else if (tid < 128) {
+    const int j = tid - 64;
+    if (j < nB) {
+      const uint32_t v = a.H[size_t(u) * HROW_S + 64 + j];
+      B[j] = v;
+      naB[j] = a.nrm2[v];
+      qB[j] = a.nq[v];
+      eB[j] = a.ne[v];
+      wB[j] = a.GD[size_t(v) * K + K - 1];
+    }
+  }
+  __syncthreads();
+
+  f32x4 accB[4], accA[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    accB[c] = f32x4{0.f, 0.f, 0.f, 0.f};
+    accA[c] = f32x4{0.f, 0.f, 0.f, 0.f};
+  }
+  const unsigned char* src = PREC == 2 ? reinterpret_cast<const unsigned char*>(a.Xb)
+                                       : reinterpret_cast<const unsigned char*>(a.X);
+  for (int d0 = 0; d0 < D; d0 += DC) {
+    // stage rows [d0, d0+DC) of A and B: 16 B per thread-load, 16 loads per 256-B row
+    for (int e = tid; e < 2 * 64 * 16; e += 256) {
+      const int op = e >> 10, r = (e >> 4) & 63, c16 = e & 15;
+      const int n = op == 0 ? nA : nB;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      const int col = d0 + c16 * (16 / ESZ);
+      if (r < n && col < D) {
+        const uint32_t id = op == 0 ? A[r] : B[r];
+        v = *reinterpret_cast<const uint4*>(src + (size_t(id) * D + col) * ESZ);
+      }
+      *reinterpret_cast<uint4*>((op == 0 ? sA : sB) + r * RS + c16 * 16) = v;
+    }
+    __syncthreads();
+    if (PREC == 2) {
+#pragma unroll
+      for (int k0 = 0; k0 < DC; k0 += 32) {
+        const int kk = k0 + 8 * (l >> 4);
+        const bf16x8 fa = *reinterpret_cast<const bf16x8*>(sA + (16 * w + (l & 15)) * RS + kk * 2);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const bf16x8 fb = *reinterpret_cast<const bf16x8*>(sB + (16 * c + (l & 15)) * RS + kk * 2);
+          const bf16x8 fc = *reinterpret_cast<const bf16x8*>(sA + (16 * c + (l & 15)) * RS + kk * 2);
+          accB[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fb, accB[c], 0, 0, 0);
+          accA[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fc, accA[c], 0, 0, 0);
+        }
+      }
+    } else {
+#pragma unroll 4
+      for (int k0 = 0; k0 < DC; k0 += 4) {
+        const int kk = k0 + (l >> 4);
+        const float fa = *reinterpret_cast<const float*>(sA + (16 * w + (l & 15)) * RS + kk * 4);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const float fb = *reinterpret_cast<const float*>(sB + (16 * c + (l & 15)) * RS + kk * 4);
+          const float fc = *reinterpret_cast<const float*>(sA + (16 * c + (l & 15)) * RS + kk * 4);
+          accB[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa, fb, accB[c], 0, 0, 0);
+          accA[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa, fc, accA[c], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // screen distances into the (aliased) matrices: C[i][j], i = 4*(l/16)+r, j = l%16
+  const float INF = __uint_as_float(0x7F800000u);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = 16 * w + 4 * (l >> 4) + r, j = 16 * c + (l & 15);
+      float sb = INF, sa = INF;
+      if (i < nA && j < nB && A[i] != B[j]) sb = (naA[i] + naB[j]) - 2.0f * accB[c][r];
+      if (i < nA && j < nA && i != j) sa = (naA[i] + naA[j]) - 2.0f * accA[c][r];
+      MAB[i * MROW + j] = sb;
+      MAA[i * MROW + j] = sa;
+    }
+  }
+  __syncthreads();
+
+  // per (row or column, tile): the exact (d, id) minimum among the candidates whose bound
+  // interval reaches below min(worst, min upper bound); pushed when d < worst
+  __shared__ uint32_t n_exact_wg;
+  if (tid == 0) n_exact_wg = 0;
+  __syncthreads();
+  // one 16-lane group per job (destination row, 16-wide candidate tile); lane t screens
+  // candidate lo + t, the group re-scores the candidates inside the band with dist16
+  const int gg = tid >> 4, t = tid & 15, gshift = 16 * ((tid >> 4) & 3);
+  const float eta = a.eta;
+  uint32_t n_exact = 0;
+  for (int kind = 0; kind < 3; ++kind) {
+    const bool col = kind == 1;
+    const bool cb = kind == 0;  // candidates come from B only for the A-row x B-tile jobs
+    const int nr = col ? nB : nA, nc = kind == 0 ? nB : nA;
+    const float* M = kind == 2 ? MAA : MAB;
+    for (int r = gg; r < nr; r += 16) {
+      const uint32_t dest = col ? B[r] : A[r];
+      const float worst = col ? wB[r] : wA[r];
+      const float nd = col ? naB[r] : naA[r], qd = col ? qB[r] : qA[r];
+      const float ed = col ? eB[r] : eA[r];
+      Row16<MC> xd;
+      load_row16<MC>(xd, a.X + size_t(dest) * D, D, t);
+      for (int lo = 0; lo < nc; lo += LJ_TILE) {
+        const int j = lo + t;
+        float s = INF, E = 0.f;
+        if (j < nc) {
+          s = col ? M[j * MROW + r] : M[r * MROW + j];
+          if (s < INF)
+            E = screen_bound(nd, qd, ed, cb ? naB[j] : naA[j], cb ? qB[j] : qA[j],
+                             cb ? eB[j] : eA[j], eta);
+        }
+        float tau = s < INF ? fminf(worst, s + E) : worst;
+#pragma unroll
+        for (int o = 8; o >= 1; o >>= 1) tau = fminf(tau, __shfl_xor(tau, o, 16));
+        const bool in_band = s < INF && !(s - E > tau);
+        uint32_t mask = uint32_t(ballot(in_band) >> gshift) & 0xFFFFu;
+        uint64_t best = TG_KEY_INF;
+        while (mask) {
+          const int jj = lo + __ffs(mask) - 1;
+          mask &= mask - 1;
+          const uint32_t cid = cb ? B[jj] : A[jj];
+          const float dd = dist16<MC>(xd, a.X + size_t(cid) * D, D, t);
+          ++n_exact;
+          best = min(best, mk_key(dd, cid));
+        }
+        // candidates already in the destination's list would be dropped by the merge
+        if (t == 0 && best != TG_KEY_INF && key_d(best) < worst &&
+            !row_has_key(a.G, a.GD, K, dest, best))
+          pb_push(a.pc, dest, best);
+      }
+    }
+  }
+  if (t != 0) n_exact = 0;
+  if (a.counters) {
+    // [0] pair evaluations (screen), [1] exact re-scores
+    if (n_exact) atomicAdd(&n_exact_wg, n_exact);
+    __syncthreads();
+    if (tid == 0) {
+      atomicAdd(&a.counters[0], (unsigned long long)(nA * nB + nA * (nA - 1) / 2));
+      atomicAdd(&a.counters[1], (unsigned long long)n_exact_wg);
+    }
+  }
+}
+
+void launch_knn_localjoin_screen(const KnnArgs& a, int precision, hipStream_t s) {
+  const int mc = (int(a.D) + 63) / 64;
+#define LJS(P, M) k_localjoin_screen<P, M><<<a.N, 256, 0, s>>>(a)
+  if (precision == KP_BF16) {
+    if (mc <= 2) LJS(2, 2); else if (mc <= 4) LJS(2, 4); else LJS(2, 16);
+  } else {
+    if (mc <= 2) LJS(1, 2); else if (mc <= 4) LJS(1, 4); else LJS(1, 16);
+  }
+#undef LJS
+}
+
+}  // namespace tg

@@ -23,46 +23,74 @@ constexpr int W_MAX = 256;      // TOPM + K <= 256
 constexpr int EXPL_MAX = 32;
 
 // -------------------------------------------------------------------------------------
+// Id set in LDS: open addressing over 512 slots (<= 256 members), INV marks an empty slot.
+// -------------------------------------------------------------------------------------
+constexpr int HSET = 512;
+__device__ __forceinline__ uint32_t hslot(uint32_t id) { return (id * 0x9E3779B1u) >> 23; }
+__device__ void hset_clear(uint32_t* tab) {
+  for (int i = threadIdx.x; i < HSET; i += TG_WAVE) tab[i] = TG_INV;
+}
+__device__ __forceinline__ void hset_insert(uint32_t* tab, uint32_t id) {
+  if (id == TG_INV) return;
+  uint32_t h = hslot(id);
+  for (;;) {
+    const uint32_t old = atomicCAS(&tab[h], TG_INV, id);
+    if (old == TG_INV || old == id) return;
+    h = (h + 1) & (HSET - 1);
+  }
+}
+__device__ __forceinline__ bool hset_has(const uint32_t* tab, uint32_t id) {
+  uint32_t h = hslot(id);
+  for (;;) {
+    const uint32_t v = tab[h];
+    if (v == id) return true;
+    if (v == TG_INV) return false;
+    h = (h + 1) & (HSET - 1);
+  }
+}
+
+// -------------------------------------------------------------------------------------
 // Beam search shared by cal_ep and path Collect (spec in the oracle's beam_search).
-// W: beam (sorted keys, TOPM) followed by the K newest candidates; dset: sorted ids of W.
+// M0/M1 (ping-pong, TOPM + K sorted keys): the beam is the first TOPM; S: the K newest
+// candidates; tab: id set of the previous round's beam and candidates (the dedup set).
 // The explored flag of a beam entry is represented by membership of its id in `expl`:
 // the beam never holds two finite entries with one id, and an expanded node that falls
 // out of the beam can never re-enter it (the beam's TOPM-th key only decreases), so the
-// two representations agree.
+// two representations agree.  Returns the final sorted array (beam = first TOPM).
 // -------------------------------------------------------------------------------------
 template <int MC>
-__device__ void beam_search(const PruneArgs& a, const Row16<MC>& tgt, uint32_t start,
-                            int max_exp, uint64_t* W, uint32_t* dset, uint32_t* expl,
-                            uint16_t* cl, uint64_t* ring, uint32_t& ring_c,
-                            uint32_t& n_eval, uint32_t& n_adj) {
-  const int lane = threadIdx.x, g = lane >> 4, t = lane & 15;
-  const int K = a.K, TOPM = a.TOPM, D = a.D;
+__device__ uint64_t* beam_search(const PruneArgs& a, const Row16<MC>& tgt, uint32_t start,
+                                 int max_exp, uint64_t* M0, uint64_t* M1, uint64_t* S,
+                                 uint32_t* tab, uint32_t* expl, uint16_t* cl, uint64_t* ring,
+                                 uint32_t& ring_c, uint32_t& n_eval, uint32_t& n_adj) {
+  const int lane = threadIdx.x;
+  const int K = a.K, TOPM = a.TOPM, D = a.D, nK = pow2_at_least(K);
   const float* X = a.X;
-  // start list
-  for (int j = lane; j < W_MAX; j += TG_WAVE) W[j] = TG_KEY_INF;
+  const uint64_t INFKEY = mk_key(__uint_as_float(0x7F800000u), 0u);
+  // start list S = G[start] with distances (ring slots 0..K-1 in list order), sorted
+  for (int j = lane; j < nK; j += TG_WAVE)
+    S[j] = j < K ? (INFKEY | a.G[size_t(start) * K + j]) : TG_KEY_INF;
   __syncthreads();
-  for (int j = lane; j < K; j += TG_WAVE) dset[j] = a.G[size_t(start) * K + j];
-  __syncthreads();
-  for (int j = g; j < K; j += 4) {
-    const uint32_t y = dset[j];
-    const float d = dist16<MC>(tgt, X + size_t(y) * D, D, t);
-    if (t == 0) {
-      const uint64_t k = mk_key(d, y);
-      W[j] = k;
-      if (ring) ring[j % RING] = k;
-    }
-  }
+  eval_batch<MC>(X, D, tgt, K, [&](int q) { return key_id(S[q]); },
+                 [&](int q, float d) {
+                   const uint64_t k = mk_key(d, key_id(S[q]));
+                   S[q] = k;
+                   if (ring) ring[q % RING] = k;
+                 });
   n_eval += K;
   n_adj += 1;
   ring_c = K;
+  hset_clear(tab);
   __syncthreads();
-  wave_bitonic(W, W_MAX);
-  // B = W[0..min(TOPM,K)); W[min(TOPM,K)..TOPM) already padding.  W beyond TOPM keeps S's
-  // tail until overwritten; it is part of the first dedup set, as in the spec.
-  for (int i = lane; i < W_MAX; i += TG_WAVE)
-    dset[i] = (i < TOPM + K) ? key_id(W[i]) : TG_INV;
+  wave_bitonic(S, nK);
+  // beam = S[0..min(TOPM,K)) padded; the first dedup set is ids(S)
+  for (int i = lane; i < TOPM + K; i += TG_WAVE) {
+    M0[i] = i < TOPM ? (i < K ? S[i] : TG_KEY_INF) : TG_KEY_INF;
+    if (i < K) hset_insert(tab, key_id(S[i]));
+  }
   __syncthreads();
-  wave_bitonic(dset, W_MAX);
+  uint64_t* cur = M0;
+  uint64_t* nxt = M1;
   int nexpl = 0;
   for (int r = 0; r < max_exp; ++r) {
     // first explorable beam entry
@@ -71,7 +99,7 @@ __device__ void beam_search(const PruneArgs& a, const Row16<MC>& tgt, uint32_t s
       const int i = base + lane;
       bool ok = false;
       if (i < TOPM) {
-        const uint64_t k = W[i];
+        const uint64_t k = cur[i];
         ok = key_finite(k);
         if (ok) {
           const uint32_t id = key_id(k);
@@ -82,52 +110,50 @@ __device__ void beam_search(const PruneArgs& a, const Row16<MC>& tgt, uint32_t s
       if (m) first = base + __ffsll((unsigned long long)m) - 1;
     }
     if (first < 0) break;
-    const uint32_t x = key_id(W[first]);
+    const uint32_t x = key_id(cur[first]);
     __syncthreads();
     if (lane == 0) expl[nexpl] = x;
     ++nexpl;
     n_adj += 1;
-    // new candidates: dedup against dset, compact the non-duplicates in j order
+    // new candidates: duplicates of the dedup set get +inf; the rest compacted in j order
     int cnt = 0;
-    for (int base = 0; base < K; base += TG_WAVE) {
+    for (int base = 0; base < nK; base += TG_WAVE) {
       const int j = base + lane;
       bool fresh = false;
-      uint32_t y = 0;
       if (j < K) {
-        y = a.G[size_t(x) * K + j];
-        // lower_bound over dset[0..TOPM+K)
-        int lo = 0, hi = TOPM + K;
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (dset[mid] < y) lo = mid + 1; else hi = mid;
-        }
-        fresh = !(lo < TOPM + K && dset[lo] == y);
-        W[TOPM + j] = mk_key(__uint_as_float(0x7F800000u), y);
+        const uint32_t y = a.G[size_t(x) * K + j];
+        fresh = !hset_has(tab, y);
+        S[j] = INFKEY | y;
+      } else if (j < nK) {
+        S[j] = TG_KEY_INF;
       }
       const uint64_t m = ballot(fresh);
       if (fresh) cl[cnt + mask_rank(m)] = uint16_t(j);
       cnt += __popcll(m);
     }
     __syncthreads();
-    for (int q = g; q < cnt; q += 4) {
-      const int j = cl[q];
-      const uint32_t y = key_id(W[TOPM + j]);
-      const float d = dist16<MC>(tgt, X + size_t(y) * D, D, t);
-      if (t == 0) {
-        const uint64_t k = mk_key(d, y);
-        W[TOPM + j] = k;
-        if (ring) ring[(ring_c + q) % RING] = k;
-      }
-    }
+    const uint32_t c0 = ring_c;
+    eval_batch<MC>(X, D, tgt, cnt, [&](int q) { return key_id(S[cl[q]]); },
+                   [&](int q, float d) {
+                     const int j = cl[q];
+                     const uint64_t k = mk_key(d, key_id(S[j]));
+                     S[j] = k;
+                     if (ring) ring[(c0 + q) % RING] = k;
+                   });
     ring_c += cnt;
     n_eval += cnt;
     __syncthreads();
-    wave_bitonic(W, W_MAX);
-    for (int i = lane; i < W_MAX; i += TG_WAVE)
-      dset[i] = (i < TOPM + K) ? key_id(W[i]) : TG_INV;
+    wave_bitonic(S, nK);
+    merge_lds(cur, TOPM, S, K, nxt);
+    hset_clear(tab);
     __syncthreads();
-    wave_bitonic(dset, W_MAX);
+    for (int i = lane; i < TOPM + K; i += TG_WAVE) hset_insert(tab, key_id(nxt[i]));
+    __syncthreads();
+    uint64_t* tmp = cur;
+    cur = nxt;
+    nxt = tmp;
   }
+  return cur;
 }
 
 // -------------------------------------------------------------------------------------
@@ -175,7 +201,7 @@ __device__ int next_valid(const uint64_t* P, int from, int L, uint32_t skip) {
 template <int MC>
 __device__ int filter_pool(const PruneArgs& a, uint64_t* P, int L, uint32_t u, int rule,
                            uint64_t* sel, uint16_t* cl, uint32_t& n_eval) {
-  const int lane = threadIdx.x, g = lane >> 4, t = lane & 15;
+  const int lane = threadIdx.x, t = lane & 15;
   const int D = a.D, FD = a.FD;
   int tix = next_valid(P, 0, L, u);
   if (tix >= L) return 0;
@@ -203,13 +229,13 @@ __device__ int filter_pool(const PruneArgs& a, uint64_t* P, int L, uint32_t u, i
       cnt += __popcll(m);
     }
     __syncthreads();
-    for (int q = g; q < cnt; q += 4) {
-      const int i = cl[q];
-      const uint64_t k = P[i];
-      const float res = dist16<MC>(prow, a.X + size_t(key_id(k)) * D, D, t);
-      if (t == 0 && rule_eval(rule, res, key_d(k), cur, a.th) < 0.0f)
-        P[i] = key_with_id(k, TG_INV);
-    }
+    eval_batch<MC>(a.X, D, prow, cnt, [&](int q) { return key_id(P[cl[q]]); },
+                   [&](int q, float res) {
+                     const int i = cl[q];
+                     const uint64_t k = P[i];
+                     if (rule_eval(rule, res, key_d(k), cur, a.th) < 0.0f)
+                       P[i] = key_with_id(k, TG_INV);
+                   });
     n_eval += cnt;
     __syncthreads();
     tix = next_valid(P, tix, L, TG_INV);
@@ -236,14 +262,14 @@ __device__ __forceinline__ void count_add(unsigned long long* c, int idx, uint32
 template <int MC, int KIND>
 __global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
   __shared__ uint64_t P[POOL_MAX];
-  __shared__ uint64_t W[W_MAX];
-  __shared__ uint32_t dset[W_MAX];
+  __shared__ uint64_t M0[W_MAX], M1[W_MAX], S[128];
+  __shared__ uint32_t tab[HSET];
   __shared__ uint32_t expl[EXPL_MAX];
   __shared__ uint16_t cl[POOL_MAX];
   __shared__ uint64_t sel[129];
   const uint32_t u = blockIdx.x;
   if (u >= a.N) return;
-  const int lane = threadIdx.x, g = lane >> 4, t = lane & 15;
+  const int lane = threadIdx.x, t = lane & 15;
   const int K = a.K, D = a.D;
   Row16<MC> xu;
   load_row16<MC>(xu, a.X + size_t(u) * D, D, t);
@@ -255,7 +281,7 @@ __global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
   uint32_t n_col = 0, n_adj = 0, n_flt = 0;
   if (KIND == 0) {
     uint32_t ring_c = 0;
-    beam_search<MC>(a, xu, *a.ep, 10, W, dset, expl, cl, P + K, ring_c, n_col, n_adj);
+    beam_search<MC>(a, xu, *a.ep, 10, M0, M1, S, tab, expl, cl, P + K, ring_c, n_col, n_adj);
   } else {
     // two-hop ids y_i = G[G[u][i/K]][i%K], i < 480
     for (int i = lane; i < RING; i += TG_WAVE) {
@@ -263,11 +289,8 @@ __global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
       P[K + i] = mk_key(0.f, a.G[size_t(v) * K + (i % K)]);
     }
     __syncthreads();
-    for (int i = g; i < RING; i += 4) {
-      const uint32_t y = key_id(P[K + i]);
-      const float d = dist16<MC>(xu, a.X + size_t(y) * D, D, t);
-      if (t == 0) P[K + i] = mk_key(d, y);
-    }
+    eval_batch<MC>(a.X, D, xu, RING, [&](int q) { return key_id(P[K + q]); },
+                   [&](int q, float d) { P[K + q] = mk_key(d, key_id(P[K + q])); });
     n_col = RING;
     n_adj = (RING + K - 1) / K;
   }
@@ -477,15 +500,15 @@ __global__ __launch_bounds__(64) void k_rank_reverse(PruneArgs a) {
 // -------------------------------------------------------------------------------------
 template <int MC>
 __global__ __launch_bounds__(64) void k_cal_ep(PruneArgs a, const float* centroid, uint32_t* ep) {
-  __shared__ uint64_t W[W_MAX];
-  __shared__ uint32_t dset[W_MAX];
+  __shared__ uint64_t M0[W_MAX], M1[W_MAX], S[128];
+  __shared__ uint32_t tab[HSET];
   __shared__ uint32_t expl[EXPL_MAX];
-  __shared__ uint16_t cl[W_MAX];
+  __shared__ uint16_t cl[128];
   Row16<MC> c;
   load_row16<MC>(c, centroid, a.D, threadIdx.x & 15);
   uint32_t rc = 0, ne = 0, na = 0;
-  beam_search<MC>(a, c, 0u, 30, W, dset, expl, cl, nullptr, rc, ne, na);
-  if (threadIdx.x == 0) *ep = key_id(W[0]);
+  const uint64_t* B = beam_search<MC>(a, c, 0u, 30, M0, M1, S, tab, expl, cl, nullptr, rc, ne, na);
+  if (threadIdx.x == 0) *ep = key_id(B[0]);
   count_add(a.counters, 4, ne);
 }
 

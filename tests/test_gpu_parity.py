@@ -126,18 +126,21 @@ def test_index_files_are_byte_identical(int_case, tmp_path):
             assert open(p_gpu, "rb").read() == open(p_ref, "rb").read(), name
 
 
+@pytest.mark.parametrize("precision", ["exact", "fp32", "bf16"])
 @pytest.mark.parametrize("N,D,K,iters,kind", [(3000, 128, 32, 6, "int"), (2500, 96, 48, 8, "unit"),
                                               (1200, 64, 64, 4, "int")])
-def test_knn_exact_bit_identical(N, D, K, iters, kind):
-    _say(f"kNN exact N={N} D={D} K={K} iters={iters}")
+def test_knn_bit_identical(N, D, K, iters, kind, precision):
+    """Every precision gives the oracle's graph: the screens only decide what to re-score."""
+    _say(f"kNN {precision} N={N} D={D} K={K} iters={iters} data={kind}")
     X = gmm_int(N, D, 16, 5) if kind == "int" else gmm_unit(N, D, 16, 5)
     seed = 0x1234
     Gr, GDr, ev = oracle.gnn_descent(X, K, iters, seed)
     with State(N, D) as s:
         s.set_data(X)
-        s.knn_build(K, iters, seed, "exact")
+        s.knn_build(K, iters, seed, precision)
         G, GD = s.get_knn()
         st = s.stats()
+    _say(f"  re-scored {st['knn_rescored']} of {st['knn_evals']} evaluations")
     bad = np.nonzero((G != Gr).any(1))[0]
     assert bad.size == 0, f"{bad.size} rows differ, first {bad[:5]}"
     assert np.array_equal(GD, GDr)
