@@ -129,6 +129,40 @@ __device__ __forceinline__ void dist16_batch(const Row16<MC>& a, const float* co
   }
 }
 
+// NB distances between the row pairs (a[k], b[k]), both read from memory (null: unused).
+template <int MC, int NB>
+__device__ __forceinline__ void dist16_pairs(const float* const (&a)[NB],
+                                             const float* const (&b)[NB], int D, int t,
+                                             float (&out)[NB]) {
+  float4 x[NB][MC], y[NB][MC];
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+#pragma unroll
+    for (int m = 0; m < MC; ++m) {
+      const int i = m * 64 + 4 * t;
+      const bool ok = a[k] && i < D;
+      x[k][m] = ok ? *reinterpret_cast<const float4*>(a[k] + i) : make_float4(0.f, 0.f, 0.f, 0.f);
+      y[k][m] = ok ? *reinterpret_cast<const float4*>(b[k] + i) : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    float p0 = 0.f, p1 = 0.f, p2 = 0.f, p3 = 0.f;
+#pragma unroll
+    for (int m = 0; m < MC; ++m) {
+      if (m * 64 + 4 * t < D) {
+        const float e0 = x[k][m].x - y[k][m].x, e1 = x[k][m].y - y[k][m].y,
+                    e2 = x[k][m].z - y[k][m].z, e3 = x[k][m].w - y[k][m].w;
+        p0 = __builtin_fmaf(e0, e0, p0);
+        p1 = __builtin_fmaf(e1, e1, p1);
+        p2 = __builtin_fmaf(e2, e2, p2);
+        p3 = __builtin_fmaf(e3, e3, p3);
+      }
+    }
+    out[k] = tree16(p0, p1, p2, p3);
+  }
+}
+
 template <int MC>
 __device__ __forceinline__ float dist16(const Row16<MC>& a, const float* __restrict__ b, int D,
                                         int t) {

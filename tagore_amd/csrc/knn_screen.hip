@@ -189,65 +189,124 @@ __global__ __launch_bounds__(256) void k_localjoin_screen(KnnArgs a) {
 
   // per (row or column, tile): the exact (d, id) minimum among the candidates whose bound
   // interval reaches below min(worst, min upper bound); pushed when d < worst
-  __shared__ uint32_t n_exact_wg;
-  if (tid == 0) n_exact_wg = 0;
-  __syncthreads();
-  // one 16-lane group per job (destination row, 16-wide candidate tile); lane t screens
-  // candidate lo + t, the group re-scores the candidates inside the band with dist16
-  const int gg = tid >> 4, t = tid & 15, gshift = 16 * ((tid >> 4) & 3);
+  // Jobs = (destination row, 16-wide candidate tile) of the three kinds (A rows x B tiles,
+  // B columns x A tiles, A rows x A tiles), processed in chunks of QJ: one thread per job
+  // computes the band mask from the screen matrix, the band candidates are queued, the 16
+  // lane groups re-score the queue four pairs at a time, and each job keeps the exact
+  // (d, id) minimum.  Candidates already in the destination's list would be dropped by the
+  // merge and are not pushed.
+  constexpr int QJ = 64;
+  __shared__ uint32_t qent[QJ * LJ_TILE];  // (job in chunk << 8) | candidate column
+  __shared__ float qdist[QJ * LJ_TILE];
+  __shared__ int qoff[QJ + 1];
+  const int gg = tid >> 4, t = tid & 15;
   const float eta = a.eta;
+  const int tB = (nB + LJ_TILE - 1) / LJ_TILE, tA = (nA + LJ_TILE - 1) / LJ_TILE;
+  const int nj0 = nA * tB, nj1 = nB * tA, njobs = nj0 + nj1 + nA * tA;
   uint32_t n_exact = 0;
-  for (int kind = 0; kind < 3; ++kind) {
-    const bool col = kind == 1;
-    const bool cb = kind == 0;  // candidates come from B only for the A-row x B-tile jobs
-    const int nr = col ? nB : nA, nc = kind == 0 ? nB : nA;
-    const float* M = kind == 2 ? MAA : MAB;
-    for (int r = gg; r < nr; r += 16) {
-      const uint32_t dest = col ? B[r] : A[r];
+  for (int base = 0; base < njobs; base += QJ) {
+    // ---- band masks (threads 0..QJ-1 = wave 0)
+    int kind = 0, r = 0, lo = 0;
+    uint32_t mask = 0;
+    const int J = base + tid;
+    if (tid < QJ && J < njobs) {
+      if (J < nj0) {
+        kind = 0; r = J / tB; lo = (J % tB) * LJ_TILE;
+      } else if (J < nj0 + nj1) {
+        kind = 1; r = (J - nj0) / tA; lo = ((J - nj0) % tA) * LJ_TILE;
+      } else {
+        kind = 2; r = (J - nj0 - nj1) / tA; lo = ((J - nj0 - nj1) % tA) * LJ_TILE;
+      }
+      const bool col = kind == 1, cb = kind == 0;
+      const int nc = cb ? nB : nA, hi = min(nc, lo + LJ_TILE);
+      const float* M = kind == 2 ? MAA : MAB;
       const float worst = col ? wB[r] : wA[r];
       const float nd = col ? naB[r] : naA[r], qd = col ? qB[r] : qA[r];
       const float ed = col ? eB[r] : eA[r];
-      Row16<MC> xd;
-      load_row16<MC>(xd, a.X + size_t(dest) * D, D, t);
-      for (int lo = 0; lo < nc; lo += LJ_TILE) {
-        const int j = lo + t;
-        float s = INF, E = 0.f;
-        if (j < nc) {
-          s = col ? M[j * MROW + r] : M[r * MROW + j];
-          if (s < INF)
-            E = screen_bound(nd, qd, ed, cb ? naB[j] : naA[j], cb ? qB[j] : qA[j],
-                             cb ? eB[j] : eA[j], eta);
-        }
-        float tau = s < INF ? fminf(worst, s + E) : worst;
+      float sv[LJ_TILE], ev[LJ_TILE];
+      float tau = worst;
 #pragma unroll
-        for (int o = 8; o >= 1; o >>= 1) tau = fminf(tau, __shfl_xor(tau, o, 16));
-        const bool in_band = s < INF && !(s - E > tau);
-        uint32_t mask = uint32_t(ballot(in_band) >> gshift) & 0xFFFFu;
-        uint64_t best = TG_KEY_INF;
-        while (mask) {
-          const int jj = lo + __ffs(mask) - 1;
-          mask &= mask - 1;
-          const uint32_t cid = cb ? B[jj] : A[jj];
-          const float dd = dist16<MC>(xd, a.X + size_t(cid) * D, D, t);
-          ++n_exact;
-          best = min(best, mk_key(dd, cid));
+      for (int q = 0; q < LJ_TILE; ++q) {
+        const int j = lo + q;
+        sv[q] = INF;
+        ev[q] = 0.f;
+        if (j < hi) {
+          sv[q] = col ? M[j * MROW + r] : M[r * MROW + j];
+          if (sv[q] < INF) {
+            ev[q] = screen_bound(nd, qd, ed, cb ? naB[j] : naA[j], cb ? qB[j] : qA[j],
+                                 cb ? eB[j] : eA[j], eta);
+            tau = fminf(tau, sv[q] + ev[q]);
+          }
         }
-        // candidates already in the destination's list would be dropped by the merge
-        if (t == 0 && best != TG_KEY_INF && key_d(best) < worst &&
-            !row_has_key(a.G, a.GD, K, dest, best))
-          pb_push(a.pc, dest, best);
+      }
+#pragma unroll
+      for (int q = 0; q < LJ_TILE; ++q)
+        if (sv[q] < INF && !(sv[q] - ev[q] > tau)) mask |= 1u << q;
+    }
+    // ---- queue offsets (wave 0 scan)
+    if (tid < QJ) {
+      const int c = __popc(mask);
+      const int incl = wave_incl_scan(c);
+      qoff[tid + 1] = incl;
+      if (tid == 0) qoff[0] = 0;
+      int pos = incl - c;
+      uint32_t m = mask;
+      while (m) {
+        const int q = __ffs(m) - 1;
+        m &= m - 1;
+        qent[pos++] = (uint32_t(tid) << 8) | uint32_t(lo + q) | (uint32_t(kind) << 6);
       }
     }
-  }
-  if (t != 0) n_exact = 0;
-  if (a.counters) {
-    // [0] pair evaluations (screen), [1] exact re-scores
-    if (n_exact) atomicAdd(&n_exact_wg, n_exact);
     __syncthreads();
-    if (tid == 0) {
-      atomicAdd(&a.counters[0], (unsigned long long)(nA * nB + nA * (nA - 1) / 2));
-      atomicAdd(&a.counters[1], (unsigned long long)n_exact_wg);
+    const int nq = qoff[QJ];
+    // ---- exact re-scores, four pairs in flight per 16-lane group
+    for (int e0 = 0; e0 < nq; e0 += 64) {
+      const float* ra[4];
+      const float* rb[4];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int e = e0 + gg + 16 * b;
+        ra[b] = rb[b] = nullptr;
+        if (e < nq) {
+          const uint32_t q = qent[e];
+          const int jk = (q >> 6) & 3, jc = q & 63;
+          const int J2 = base + int(q >> 8);
+          const int rr = jk == 0 ? J2 / tB : (jk == 1 ? (J2 - nj0) / tA : (J2 - nj0 - nj1) / tA);
+          const uint32_t dest = jk == 1 ? B[rr] : A[rr];
+          const uint32_t cid = jk == 0 ? B[jc] : A[jc];
+          ra[b] = a.X + size_t(dest) * D;
+          rb[b] = a.X + size_t(cid) * D;
+        }
+      }
+      float dd[4];
+      dist16_pairs<MC, 4>(ra, rb, D, t, dd);
+      if (t == 0) {
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const int e = e0 + gg + 16 * b;
+          if (e < nq) qdist[e] = dd[b];
+        }
+      }
     }
+    n_exact += nq;
+    __syncthreads();
+    // ---- per job: exact minimum of the band, push
+    if (tid < QJ && mask) {
+      const uint32_t dest = kind == 1 ? B[r] : A[r];
+      const float worst = kind == 1 ? wB[r] : wA[r];
+      uint64_t best = TG_KEY_INF;
+      for (int e = qoff[tid]; e < qoff[tid + 1]; ++e) {
+        const int jc = qent[e] & 63;
+        best = min(best, mk_key(qdist[e], kind == 0 ? B[jc] : A[jc]));
+      }
+      if (key_d(best) < worst && !row_has_key(a.G, a.GD, K, dest, best)) pb_push(a.pc, dest, best);
+    }
+    __syncthreads();
+  }
+  if (a.counters && tid == 0) {
+    // [0] pair evaluations (screen), [1] exact re-scores
+    atomicAdd(&a.counters[0], (unsigned long long)(nA * nB + nA * (nA - 1) / 2));
+    atomicAdd(&a.counters[1], (unsigned long long)n_exact);
   }
 }
 
