@@ -38,7 +38,8 @@ def measure(st, args, precision, iters, label):
 
     def step():
         st.knn_build(args.k, iters, args.seed, precision)
-        st.prune("path", "dist", "dist", args.fd, args.topm, 1.0)
+        st.prune("path", "dist", "dist", args.fd, args.topm, 1.0,
+                 args.prune_rows if precision == "bf16" else "fp32")
 
     for w in range(args.warmup):
         t = time.perf_counter()
@@ -86,6 +87,8 @@ def main():
     ap.add_argument("--ncomp", type=int, default=1024)
     ap.add_argument("--precision", default="bf16", help="headline precision")
     ap.add_argument("--plain-precision", default="fp32")
+    ap.add_argument("--prune-rows", default="fp32", choices=["fp32", "bf16"],
+                    help="rows the headline prune reads (bf16 only when the data is exact in BF16)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--seed", type=int, default=0x7A600002)
     ap.add_argument("--recall-queries", type=int, default=10000)

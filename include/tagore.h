@@ -60,6 +60,9 @@ typedef struct {
   uint32_t FD;       /* FINAL_DEGREE */
   uint32_t TOPM;     /* beam width m of the path Collect */
   float thre;        /* threshold: alpha for dist, cos(angle) for angle */
+  int32_t precision; /* TG_PREC_BF16: read rows from the BF16 copy when every value of the
+                        data set is exactly representable in BF16 (bit-identical result);
+                        otherwise FP32 rows */
 } tg_prune_params;
 
 /* Last error message of the calling thread (never NULL). */

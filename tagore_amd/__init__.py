@@ -50,7 +50,8 @@ class KnnParams(ctypes.Structure):
 class PruneParams(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("fwd_rule", ctypes.c_int32),
                 ("rev_rule", ctypes.c_int32), ("FD", ctypes.c_uint32),
-                ("TOPM", ctypes.c_uint32), ("thre", ctypes.c_float)]
+                ("TOPM", ctypes.c_uint32), ("thre", ctypes.c_float),
+                ("precision", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
@@ -182,11 +183,12 @@ class State:
         return ids, d
 
     # pruning
-    def prune(self, kind, fwd_rule, rev_rule, FD: int, TOPM: int = 64, thre: float = 1.0):
+    def prune(self, kind, fwd_rule, rev_rule, FD: int, TOPM: int = 64, thre: float = 1.0,
+              precision: str = "fp32"):
         kind = KIND[kind] if isinstance(kind, str) else kind
         fwd_rule = RULE[fwd_rule] if isinstance(fwd_rule, str) else fwd_rule
         rev_rule = RULE[rev_rule] if isinstance(rev_rule, str) else rev_rule
-        p = PruneParams(kind, fwd_rule, rev_rule, FD, TOPM, thre)
+        p = PruneParams(kind, fwd_rule, rev_rule, FD, TOPM, thre, PREC[precision])
         _check(lib().tg_prune(self._h, ctypes.byref(p)))
         self.FD = FD
 

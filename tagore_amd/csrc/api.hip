@@ -101,6 +101,10 @@ struct tg_state {
   PB pn, po, pc, pr;
   uint16_t* Xb = nullptr;  // BF16 copy of X (screen)
   float *nrm2 = nullptr, *nq = nullptr, *ne = nullptr;
+  uint32_t* cert_flags = nullptr;  // [4], see k_screen_prep
+  bool prep_valid = false;         // Xb / norms / certificates match the current X
+  bool cert_bf16 = false;          // every value of X is exactly representable in BF16
+  bool cert_int = false;           // integer data whose screen distances are exact
   uint32_t* fill = nullptr;
   uint32_t* scan_tmp = nullptr;
   unsigned long long* counters = nullptr;  // [16]
@@ -188,6 +192,31 @@ int ensure_knn_scratch(tg_state* s, uint32_t K) {
   if (alloc_pb(s, s->pn, 64, N * 32) || alloc_pb(s, s->po, 64, N * 32) ||
       alloc_pb(s, s->pc, 256, N * 256) || alloc_pb(s, s->pr, 128, N * size_t(K)))
     return TG_ERR_HIP;
+  return TG_OK;
+}
+
+// BF16 copy, norms and data-set certificates of the current vectors (computed once per data
+// set, like the reference's host-side FP16 copy, pybind.cu:58-61).
+int ensure_prep(tg_state* s) {
+  if (s->prep_valid) return TG_OK;
+  const size_t N = s->N;
+  if (!s->nrm2) {
+    if (dalloc(&s->nrm2, N) || dalloc(&s->nq, N) || dalloc(&s->ne, N) ||
+        dalloc(&s->cert_flags, 4) || dalloc(&s->Xb, N * s->D))
+      return TG_ERR_HIP;
+  }
+  fill_u32(s->cert_flags, 0, 4, s->stream);
+  launch_screen_prep(s->X, s->N, s->D, s->Xb, s->nrm2, s->nq, s->ne, s->cert_flags, s->stream);
+  uint32_t f[4];
+  HIPX(hipMemcpyAsync(f, s->cert_flags, sizeof(f), hipMemcpyDeviceToHost, s->stream));
+  HIPX(hipStreamSynchronize(s->stream));
+  s->cert_bf16 = f[0] == 0;
+  float qmax;
+  std::memcpy(&qmax, &f[3], 4);
+  // every MFMA partial sum, |a|^2+|b|^2 and |a-b|^2 stay integers below 2^24
+  const double lim = 16777216.0;
+  s->cert_int = f[1] == 0 && (f[2] == 0 ? 2.0 * qmax < lim : 4.0 * qmax < lim);
+  s->prep_valid = true;
   return TG_OK;
 }
 
@@ -347,6 +376,7 @@ int tg_free(tg_state* s) {
   dfree(s->nrm2);
   dfree(s->nq);
   dfree(s->ne);
+  dfree(s->cert_flags);
   dfree(s->fill);
   dfree(s->scan_tmp);
   dfree(s->counters);
@@ -370,6 +400,7 @@ int tg_set_data(tg_state* s, const float* rows) {
   HIPX(hipMemcpyAsync(s->X, rows, size_t(s->N) * s->D * 4, hipMemcpyHostToDevice, s->stream));
   HIPX(hipStreamSynchronize(s->stream));
   s->has_knn = false;
+  s->prep_valid = false;
   s->has_prune = false;
   return TG_OK;
 }
@@ -387,6 +418,7 @@ int tg_gen_data(tg_state* s, int32_t kind, uint32_t ncomp, uint64_t seed, float 
   if (int rc = check_launch("gen_data")) return rc;
   HIPX(hipStreamSynchronize(s->stream));
   s->has_knn = false;
+  s->prep_valid = false;
   s->has_prune = false;
   return TG_OK;
 }
@@ -465,12 +497,8 @@ int tg_knn_build(tg_state* s, const tg_knn_params* p) {
     return fail(TG_ERR_ARG, "BF16 screening needs D % 8 == 0");
   HIPX(hipSetDevice(s->device));
   if (ensure_graph(s, K) || ensure_knn_scratch(s, K)) return TG_ERR_HIP;
-  if (p->precision != TG_PREC_EXACT && !s->nrm2) {
-    if (dalloc(&s->nrm2, s->N) || dalloc(&s->nq, s->N) || dalloc(&s->ne, s->N)) return TG_ERR_HIP;
-  }
-  if (p->precision == TG_PREC_BF16 && !s->Xb) {
-    if (dalloc(&s->Xb, size_t(s->N) * s->D)) return TG_ERR_HIP;
-  }
+  if (p->precision != TG_PREC_EXACT)
+    if (int rc = ensure_prep(s)) return rc;
   hipStream_t st = s->stream;
   if (ensure_events(s, 2)) return TG_ERR_HIP;
   // window events (separate from the per-section pool)
@@ -482,12 +510,18 @@ int tg_knn_build(tg_state* s, const tg_knn_params* p) {
   fill_u32(reinterpret_cast<uint32_t*>(s->counters), 0, 32, st);
   clear_errors(s, {&s->pn, &s->po, &s->pc, &s->pr});
   HIPX(hipEventRecord(w0, st));
-  KnnArgs a = knn_args(s, p->seed);
+  // precision-dependent parts of the argument block (re-applied after every swap)
+  const bool exact_screen = p->precision != TG_PREC_EXACT && s->cert_int;
+  const uint16_t* rows16 = (p->precision == TG_PREC_BF16 && s->cert_bf16) ? s->Xb : nullptr;
+  auto args = [&]() {
+    KnnArgs k = knn_args(s, p->seed);
+    k.exact_screen = exact_screen ? 1 : 0;
+    k.Xr16 = rows16;
+    return k;
+  };
+  KnnArgs a = args();
   const uint32_t N = s->N;
   if (sec.start(0)) return TG_ERR_HIP;
-  if (p->precision != TG_PREC_EXACT)
-    launch_screen_prep(s->X, N, s->D, p->precision == TG_PREC_BF16 ? s->Xb : nullptr, s->nrm2,
-                       s->nq, s->ne, st);
   launch_knn_init(a, st);
   if (sec.stop()) return TG_ERR_HIP;
   const uint32_t it1 = p->iters / 2, it2 = p->iters / 2;
@@ -530,7 +564,7 @@ int tg_knn_build(tg_state* s, const tg_knn_params* p) {
     std::swap(s->GD, s->GD1);
     std::swap(s->R0, s->R1);
     std::swap(s->Rn0, s->Rn1);
-    a = knn_args(s, p->seed);
+    a = args();
     if (int rc = check_launch("GNN-Descent phase 2")) return rc;
   }
   if (sec.start(7)) return TG_ERR_HIP;
@@ -638,6 +672,15 @@ int tg_prune(tg_state* s, const tg_prune_params* p) {
   a.out_ids = s->out_ids;
   a.rev = s->rev.d;
   a.counters = s->counters;
+  a.rows = s->X;
+  a.rows_bf16 = 0;
+  if (p->precision == TG_PREC_BF16) {
+    if (int rc = ensure_prep(s)) return rc;
+    if (s->cert_bf16) {
+      a.rows = s->Xb;
+      a.rows_bf16 = 1;
+    }
+  }
   hipEvent_t w0, w1;
   HIPX(hipEventCreate(&w0));
   HIPX(hipEventCreate(&w1));

@@ -50,14 +50,25 @@ struct Row16 {
   float4 v[MC];
 };
 
-template <int MC>
-__device__ __forceinline__ void load_row16(Row16<MC>& r, const float* __restrict__ x, int D,
-                                           int t) {
+// Four consecutive elements as FP32: rows are stored as FP32, or as BF16 bits when every
+// value of the data set is exactly representable in BF16 (then the widened values are the
+// FP32 values themselves and every distance below is bit-identical).
+__device__ __forceinline__ float4 load4(const float* p) {
+  return *reinterpret_cast<const float4*>(p);
+}
+__device__ __forceinline__ float4 load4(const uint16_t* p) {
+  const uint2 q = *reinterpret_cast<const uint2*>(p);
+  return make_float4(__uint_as_float(q.x << 16), __uint_as_float(q.x & 0xFFFF0000u),
+                     __uint_as_float(q.y << 16), __uint_as_float(q.y & 0xFFFF0000u));
+}
+
+template <int MC, class T>
+__device__ __forceinline__ void load_row16(Row16<MC>& r, const T* __restrict__ x, int D, int t) {
 #pragma unroll
   for (int m = 0; m < MC; ++m) {
     const int i = m * 64 + 4 * t;
     if (i < D)
-      r.v[m] = *reinterpret_cast<const float4*>(x + i);
+      r.v[m] = load4(x + i);
     else
       r.v[m] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
@@ -98,17 +109,16 @@ __device__ __forceinline__ float tree16(float p0, float p1, float p2, float p3) 
 
 // NB distances from row a (registers) to rows b[0..NB) (b[i] may be null: result unused);
 // all loads are issued before the arithmetic.  Results valid in lane t == 0.
-template <int MC, int NB>
-__device__ __forceinline__ void dist16_batch(const Row16<MC>& a, const float* const (&b)[NB],
-                                             int D, int t, float (&out)[NB]) {
+template <int MC, int NB, class T>
+__device__ __forceinline__ void dist16_batch(const Row16<MC>& a, const T* const (&b)[NB], int D,
+                                             int t, float (&out)[NB]) {
   float4 y[NB][MC];
 #pragma unroll
   for (int k = 0; k < NB; ++k) {
 #pragma unroll
     for (int m = 0; m < MC; ++m) {
       const int i = m * 64 + 4 * t;
-      y[k][m] = (b[k] && i < D) ? *reinterpret_cast<const float4*>(b[k] + i)
-                                : make_float4(0.f, 0.f, 0.f, 0.f);
+      y[k][m] = (b[k] && i < D) ? load4(b[k] + i) : make_float4(0.f, 0.f, 0.f, 0.f);
     }
   }
 #pragma unroll
@@ -130,10 +140,9 @@ __device__ __forceinline__ void dist16_batch(const Row16<MC>& a, const float* co
 }
 
 // NB distances between the row pairs (a[k], b[k]), both read from memory (null: unused).
-template <int MC, int NB>
-__device__ __forceinline__ void dist16_pairs(const float* const (&a)[NB],
-                                             const float* const (&b)[NB], int D, int t,
-                                             float (&out)[NB]) {
+template <int MC, int NB, class T>
+__device__ __forceinline__ void dist16_pairs(const T* const (&a)[NB], const T* const (&b)[NB],
+                                             int D, int t, float (&out)[NB]) {
   float4 x[NB][MC], y[NB][MC];
 #pragma unroll
   for (int k = 0; k < NB; ++k) {
@@ -141,8 +150,8 @@ __device__ __forceinline__ void dist16_pairs(const float* const (&a)[NB],
     for (int m = 0; m < MC; ++m) {
       const int i = m * 64 + 4 * t;
       const bool ok = a[k] && i < D;
-      x[k][m] = ok ? *reinterpret_cast<const float4*>(a[k] + i) : make_float4(0.f, 0.f, 0.f, 0.f);
-      y[k][m] = ok ? *reinterpret_cast<const float4*>(b[k] + i) : make_float4(0.f, 0.f, 0.f, 0.f);
+      x[k][m] = ok ? load4(a[k] + i) : make_float4(0.f, 0.f, 0.f, 0.f);
+      y[k][m] = ok ? load4(b[k] + i) : make_float4(0.f, 0.f, 0.f, 0.f);
     }
   }
 #pragma unroll
@@ -163,15 +172,15 @@ __device__ __forceinline__ void dist16_pairs(const float* const (&a)[NB],
   }
 }
 
-template <int MC>
-__device__ __forceinline__ float dist16(const Row16<MC>& a, const float* __restrict__ b, int D,
+template <int MC, class T>
+__device__ __forceinline__ float dist16(const Row16<MC>& a, const T* __restrict__ b, int D,
                                         int t) {
   float p0 = 0.f, p1 = 0.f, p2 = 0.f, p3 = 0.f;
 #pragma unroll
   for (int m = 0; m < MC; ++m) {
     const int i = m * 64 + 4 * t;
     if (i < D) {
-      const float4 y = *reinterpret_cast<const float4*>(b + i);
+      const float4 y = load4(b + i);
       const float e0 = a.v[m].x - y.x, e1 = a.v[m].y - y.y, e2 = a.v[m].z - y.z,
                   e3 = a.v[m].w - y.w;
       p0 = __builtin_fmaf(e0, e0, p0);
@@ -184,16 +193,16 @@ __device__ __forceinline__ float dist16(const Row16<MC>& a, const float* __restr
 }
 
 // both rows from memory
-template <int MC>
-__device__ __forceinline__ float dist16_mm(const float* __restrict__ a,
-                                           const float* __restrict__ b, int D, int t) {
+template <int MC, class T>
+__device__ __forceinline__ float dist16_mm(const T* __restrict__ a, const T* __restrict__ b,
+                                           int D, int t) {
   float p0 = 0.f, p1 = 0.f, p2 = 0.f, p3 = 0.f;
 #pragma unroll
   for (int m = 0; m < MC; ++m) {
     const int i = m * 64 + 4 * t;
     if (i < D) {
-      const float4 x = *reinterpret_cast<const float4*>(a + i);
-      const float4 y = *reinterpret_cast<const float4*>(b + i);
+      const float4 x = load4(a + i);
+      const float4 y = load4(b + i);
       const float e0 = x.x - y.x, e1 = x.y - y.y, e2 = x.z - y.z, e3 = x.w - y.w;
       p0 = __builtin_fmaf(e0, e0, p0);
       p1 = __builtin_fmaf(e1, e1, p1);
@@ -395,12 +404,12 @@ __device__ void merge_lds(const uint64_t* A, int na, const uint64_t* B, int nb, 
 // Evaluates the targets listed in ids[0..n) (selected through idx when given) against the
 // register row `tgt`, four candidates per 16-lane group at a time; calls emit(q, d) in lane
 // t == 0 of the group that owns item q.
-template <int MC, class IdOf, class Emit>
-__device__ __forceinline__ void eval_batch(const float* X, int D, const Row16<MC>& tgt, int n,
+template <int MC, class T, class IdOf, class Emit>
+__device__ __forceinline__ void eval_batch(const T* X, int D, const Row16<MC>& tgt, int n,
                                            IdOf id_of, Emit emit) {
   const int lane = threadIdx.x, g = lane >> 4, t = lane & 15;
   for (int q0 = 0; q0 < n; q0 += 16) {
-    const float* rows[4];
+    const T* rows[4];
     int qs[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) {

@@ -34,6 +34,10 @@ struct PruneArgs {
   uint32_t* out_ids;       // [N*FD]
   PushBuf rev;             // reverse pushes
   unsigned long long* counters;  // [8] evaluation counters or null
+  // rows the distances are read from: X itself, or its BF16 copy when every value of the
+  // data set is exactly representable in BF16 (identical FP32 values, half the bytes)
+  const void* rows;
+  int rows_bf16;
 };
 
 void launch_prune(const PruneArgs& a, int phase, const float* centroid, uint32_t* ep,
@@ -72,11 +76,18 @@ struct KnnArgs {
   const float* nq;     // [N] |x'| of the screen copy (rounded up)
   const float* ne;     // [N] |x' - x| (rounded up)
   float eta;           // relative slack of the bound
+  // Integer data set (values in [-256, 256], |x|^2 small enough that every partial sum is
+  // an integer below 2^24): the MFMA screen distance IS the canonical distance, no bound
+  // and no re-score are needed.
+  int exact_screen;
+  // rows used for canonical re-scores: BF16 copy when the data set is BF16-exact, else X
+  const uint16_t* Xr16;
 };
 
-// screen-mode preparation: BF16 copy (if Xb), norms and quantisation-error norms
+// screen-mode preparation: BF16 copy (if Xb), norms, quantisation-error norms and the
+// data-set certificates in flags[4] (zeroed by the caller)
 void launch_screen_prep(const float* X, uint32_t N, uint32_t D, uint16_t* Xb, float* nrm2,
-                        float* nq, float* ne, hipStream_t s);
+                        float* nq, float* ne, uint32_t* flags, hipStream_t s);
 
 void launch_knn_init(const KnnArgs& a, hipStream_t s);
 void launch_knn_sample(const KnnArgs& a, hipStream_t s);

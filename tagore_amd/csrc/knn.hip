@@ -398,7 +398,7 @@ __device__ __forceinline__ float screen_bound2(float na, float qa, float ea, flo
   return 2.0f * (ea * qb + qa * eb + ea * eb) + eta * (na + nb + 2.0f * qa * qb);
 }
 
-template <int MC, int PREC>
+template <int MC, int PREC, class T>
 __global__ __launch_bounds__(64) void k_phase2(KnnArgs a) {
   __shared__ RowLds row;
   __shared__ uint32_t ids[512];
@@ -474,11 +474,17 @@ __global__ __launch_bounds__(64) void k_phase2(KnnArgs a) {
     for (int q = g; q < cnt; q += 4) {
       const uint32_t y = ids[cl[q]];
       const float s = (nu + a.nrm2[y]) - 2.0f * dot_bf16(ub, a.Xb + size_t(y) * D, D, t);
-      const float E = screen_bound2(nu, qu, eu, a.nrm2[y], a.nq[y], a.ne[y], a.eta);
-      if (t == 0) keep[q] = (s - E < worst) ? 1 : 0;
+      if (a.exact_screen) {
+        // integer data: s is the canonical distance itself
+        if (t == 0 && s < worst) kv[q] = mk_key(s, y);
+      } else {
+        const float E = screen_bound2(nu, qu, eu, a.nrm2[y], a.nq[y], a.ne[y], a.eta);
+        if (t == 0) keep[q] = (s - E < worst) ? 1 : 0;
+      }
     }
     __syncthreads();
     nre = 0;
+    if (!a.exact_screen)
     for (int base = 0; base < cnt; base += TG_WAVE) {
       const int q = base + lane;
       const bool k = q < cnt && keep[q];
@@ -491,7 +497,8 @@ __global__ __launch_bounds__(64) void k_phase2(KnnArgs a) {
     }
   }
   __syncthreads();
-  eval_batch<MC>(a.X, D, xu, nre, [&](int q) { return ids[cl[q]]; },
+  const T* Xr = a.Xr16 ? reinterpret_cast<const T*>(a.Xr16) : reinterpret_cast<const T*>(a.X);
+  eval_batch<MC>(Xr, D, xu, nre, [&](int q) { return ids[cl[q]]; },
                  [&](int q, float d) {
                    if (d < worst) kv[q] = mk_key(d, ids[cl[q]]);
                  });
@@ -588,15 +595,18 @@ void launch_knn_rev_build(const KnnArgs& a, hipStream_t s) { k_rev_build<<<a.N, 
 void launch_knn_rev_reduce(const KnnArgs& a, hipStream_t s) { k_rev_reduce<<<a.N, 64, 0, s>>>(a); }
 void launch_knn_phase2(const KnnArgs& a, int precision, hipStream_t s) {
   const int mc = (int(a.D) + 63) / 64;
+#define P2(M, P, T) k_phase2<M, P, T><<<a.N, 64, 0, s>>>(a)
+#define P2_MC(P, T)                                                  \
+  do {                                                               \
+    if (mc <= 2) P2(2, P, T); else if (mc <= 4) P2(4, P, T); else P2(16, P, T); \
+  } while (0)
   if (precision == KP_BF16) {
-    if (mc <= 2) k_phase2<2, KP_BF16><<<a.N, 64, 0, s>>>(a);
-    else if (mc <= 4) k_phase2<4, KP_BF16><<<a.N, 64, 0, s>>>(a);
-    else k_phase2<16, KP_BF16><<<a.N, 64, 0, s>>>(a);
+    if (a.Xr16) P2_MC(KP_BF16, uint16_t); else P2_MC(KP_BF16, float);
   } else {
-    if (mc <= 2) k_phase2<2, KP_EXACT><<<a.N, 64, 0, s>>>(a);
-    else if (mc <= 4) k_phase2<4, KP_EXACT><<<a.N, 64, 0, s>>>(a);
-    else k_phase2<16, KP_EXACT><<<a.N, 64, 0, s>>>(a);
+    if (a.Xr16) P2_MC(KP_EXACT, uint16_t); else P2_MC(KP_EXACT, float);
   }
+#undef P2_MC
+#undef P2
 }
 void launch_knn_rupdate(const KnnArgs& a, hipStream_t s) { k_rupdate<<<a.N, 64, 0, s>>>(a); }
 void launch_knn_final(const KnnArgs& a, hipStream_t s) { k_final<<<a.N, 64, 0, s>>>(a); }

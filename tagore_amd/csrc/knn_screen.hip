@@ -29,21 +29,25 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }
 
-// per-vector screen statistics; one wave per row, FP64 sums
+// Per-vector screen statistics (one wave per row, FP64 sums) and data-set certificates:
+// flags[0] |= some value is not exactly representable in BF16,
+// flags[1] |= some value is not an integer in [-256, 256], flags[2] |= some value is
+// negative, flags[3] = max |x|^2 (FP32 bits; exact for integer data below 2^24).
 __global__ void k_screen_prep(const float* X, uint32_t N, uint32_t D, uint16_t* Xb, float* nrm2,
-                              float* nq, float* ne) {
+                              float* nq, float* ne, uint32_t* flags) {
   const uint32_t row = blockIdx.x;
   if (row >= N) return;
   const float* x = X + size_t(row) * D;
   double s2 = 0.0, q2 = 0.0, e2 = 0.0;
+  uint32_t not_bf16 = 0, not_int = 0, neg = 0;
   for (uint32_t j = threadIdx.x; j < D; j += TG_WAVE) {
     const float v = x[j];
-    float w = v;
-    if (Xb) {
-      const uint16_t b = f2bf(v);
-      Xb[size_t(row) * D + j] = b;
-      w = bf2f(b);
-    }
+    const uint16_t b = f2bf(v);
+    if (Xb) Xb[size_t(row) * D + j] = b;
+    const float w = bf2f(b);
+    not_bf16 |= w != v;
+    not_int |= !(rintf(v) == v && fabsf(v) <= 256.f);
+    neg |= v < 0.f;
     s2 += double(v) * double(v);
     q2 += double(w) * double(w);
     e2 += (double(w) - double(v)) * (double(w) - double(v));
@@ -53,17 +57,22 @@ __global__ void k_screen_prep(const float* X, uint32_t N, uint32_t D, uint16_t* 
     q2 += __shfl_xor(q2, o, TG_WAVE);
     e2 += __shfl_xor(e2, o, TG_WAVE);
   }
+  const uint64_t fb = __ballot(not_bf16 != 0), fi = __ballot(not_int != 0), fn = __ballot(neg != 0);
   if (threadIdx.x == 0) {
     nrm2[row] = float(s2);
     // norms rounded up, so the bound stays an upper bound
     nq[row] = float(sqrt(q2) * (1.0 + 1e-6));
     ne[row] = float(sqrt(e2) * (1.0 + 1e-6));
+    if (fb) atomicOr(&flags[0], 1u);
+    if (fi) atomicOr(&flags[1], 1u);
+    if (fn) atomicOr(&flags[2], 1u);
+    atomicMax(&flags[3], __float_as_uint(float(s2)));
   }
 }
 
 void launch_screen_prep(const float* X, uint32_t N, uint32_t D, uint16_t* Xb, float* nrm2,
-                        float* nq, float* ne, hipStream_t s) {
-  k_screen_prep<<<N, TG_WAVE, 0, s>>>(X, N, D, Xb, nrm2, nq, ne);
+                        float* nq, float* ne, uint32_t* flags, hipStream_t s) {
+  k_screen_prep<<<N, TG_WAVE, 0, s>>>(X, N, D, Xb, nrm2, nq, ne, flags);
 }
 
 __device__ __forceinline__ float screen_bound(float na, float qa, float ea, float nb, float qb,
@@ -76,7 +85,7 @@ __device__ __forceinline__ float screen_bound(float na, float qa, float ea, floa
 // 256 threads: wave w owns row block w (16 rows of A) against the 4 column blocks of B and
 // of A.  Staging chunk: DC elements per row (128 BF16 or 64 FP32 = 256 B).
 // -------------------------------------------------------------------------------------
-template <int PREC, int MC>
+template <int PREC, int MC, class T>
 __global__ __launch_bounds__(256) void k_localjoin_screen(KnnArgs a) {
   constexpr int DC = PREC == 2 ? 128 : 64;          // elements per staged chunk
   constexpr int ESZ = PREC == 2 ? 2 : 4;            // element bytes
@@ -239,10 +248,23 @@ __global__ __launch_bounds__(256) void k_localjoin_screen(KnnArgs a) {
           }
         }
       }
+      if (a.exact_screen) {
+        // the screen distances are the canonical distances: the job's answer is the
+        // (s, id) minimum of the tile, no bound and no re-score
+        uint64_t best = TG_KEY_INF;
 #pragma unroll
-      for (int q = 0; q < LJ_TILE; ++q)
-        if (sv[q] < INF && !(sv[q] - ev[q] > tau)) mask |= 1u << q;
+        for (int q = 0; q < LJ_TILE; ++q)
+          if (sv[q] < INF) best = min(best, mk_key(sv[q], cb ? B[lo + q] : A[lo + q]));
+        const uint32_t dest = col ? B[r] : A[r];
+        if (key_d(best) < worst && !row_has_key(a.G, a.GD, K, dest, best))
+          pb_push(a.pc, dest, best);
+      } else {
+#pragma unroll
+        for (int q = 0; q < LJ_TILE; ++q)
+          if (sv[q] < INF && !(sv[q] - ev[q] > tau)) mask |= 1u << q;
+      }
     }
+    if (a.exact_screen) continue;  // uniform across the workgroup
     // ---- queue offsets (wave 0 scan)
     if (tid < QJ) {
       const int c = __popc(mask);
@@ -261,8 +283,9 @@ __global__ __launch_bounds__(256) void k_localjoin_screen(KnnArgs a) {
     const int nq = qoff[QJ];
     // ---- exact re-scores, four pairs in flight per 16-lane group
     for (int e0 = 0; e0 < nq; e0 += 64) {
-      const float* ra[4];
-      const float* rb[4];
+      const T* Xr = a.Xr16 ? reinterpret_cast<const T*>(a.Xr16) : reinterpret_cast<const T*>(a.X);
+      const T* ra[4];
+      const T* rb[4];
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
         const int e = e0 + gg + 16 * b;
@@ -274,8 +297,8 @@ __global__ __launch_bounds__(256) void k_localjoin_screen(KnnArgs a) {
           const int rr = jk == 0 ? J2 / tB : (jk == 1 ? (J2 - nj0) / tA : (J2 - nj0 - nj1) / tA);
           const uint32_t dest = jk == 1 ? B[rr] : A[rr];
           const uint32_t cid = jk == 0 ? B[jc] : A[jc];
-          ra[b] = a.X + size_t(dest) * D;
-          rb[b] = a.X + size_t(cid) * D;
+          ra[b] = Xr + size_t(dest) * D;
+          rb[b] = Xr + size_t(cid) * D;
         }
       }
       float dd[4];
@@ -312,12 +335,17 @@ __global__ __launch_bounds__(256) void k_localjoin_screen(KnnArgs a) {
 
 void launch_knn_localjoin_screen(const KnnArgs& a, int precision, hipStream_t s) {
   const int mc = (int(a.D) + 63) / 64;
-#define LJS(P, M) k_localjoin_screen<P, M><<<a.N, 256, 0, s>>>(a)
+#define LJS(P, M, T) k_localjoin_screen<P, M, T><<<a.N, 256, 0, s>>>(a)
+#define LJS_MC(P, T)                                                       \
+  do {                                                                     \
+    if (mc <= 2) LJS(P, 2, T); else if (mc <= 4) LJS(P, 4, T); else LJS(P, 16, T); \
+  } while (0)
   if (precision == KP_BF16) {
-    if (mc <= 2) LJS(2, 2); else if (mc <= 4) LJS(2, 4); else LJS(2, 16);
+    if (a.Xr16) LJS_MC(2, uint16_t); else LJS_MC(2, float);
   } else {
-    if (mc <= 2) LJS(1, 2); else if (mc <= 4) LJS(1, 4); else LJS(1, 16);
+    if (a.Xr16) LJS_MC(1, uint16_t); else LJS_MC(1, float);
   }
+#undef LJS_MC
 #undef LJS
 }
 

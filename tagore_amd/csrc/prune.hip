@@ -58,14 +58,14 @@ __device__ __forceinline__ bool hset_has(const uint32_t* tab, uint32_t id) {
 // out of the beam can never re-enter it (the beam's TOPM-th key only decreases), so the
 // two representations agree.  Returns the final sorted array (beam = first TOPM).
 // -------------------------------------------------------------------------------------
-template <int MC>
+template <int MC, class T>
 __device__ uint64_t* beam_search(const PruneArgs& a, const Row16<MC>& tgt, uint32_t start,
                                  int max_exp, uint64_t* M0, uint64_t* M1, uint64_t* S,
                                  uint32_t* tab, uint32_t* expl, uint16_t* cl, uint64_t* ring,
                                  uint32_t& ring_c, uint32_t& n_eval, uint32_t& n_adj) {
   const int lane = threadIdx.x;
   const int K = a.K, TOPM = a.TOPM, D = a.D, nK = pow2_at_least(K);
-  const float* X = a.X;
+  const T* X = reinterpret_cast<const T*>(a.rows);
   const uint64_t INFKEY = mk_key(__uint_as_float(0x7F800000u), 0u);
   // start list S = G[start] with distances (ring slots 0..K-1 in list order), sorted
   for (int j = lane; j < nK; j += TG_WAVE)
@@ -198,7 +198,7 @@ __device__ int next_valid(const uint64_t* P, int from, int L, uint32_t skip) {
 // Incremental Filter over the sorted pool P[0..L) (spec A.4 in the oracle).  Writes the
 // kept entries to sel[] and returns their number.
 // -------------------------------------------------------------------------------------
-template <int MC>
+template <int MC, class T>
 __device__ int filter_pool(const PruneArgs& a, uint64_t* P, int L, uint32_t u, int rule,
                            uint64_t* sel, uint16_t* cl, uint32_t& n_eval) {
   const int lane = threadIdx.x, t = lane & 15;
@@ -213,7 +213,7 @@ __device__ int filter_pool(const PruneArgs& a, uint64_t* P, int L, uint32_t u, i
   __syncthreads();
   while (nsel < FD) {
     Row16<MC> prow;
-    load_row16<MC>(prow, a.X + size_t(pid) * D, D, t);
+    load_row16<MC>(prow, reinterpret_cast<const T*>(a.rows) + size_t(pid) * D, D, t);
     // compact the live candidates after tix; entries repeating p's id are invalidated
     int cnt = 0;
     for (int base = tix; base < L; base += TG_WAVE) {
@@ -229,7 +229,7 @@ __device__ int filter_pool(const PruneArgs& a, uint64_t* P, int L, uint32_t u, i
       cnt += __popcll(m);
     }
     __syncthreads();
-    eval_batch<MC>(a.X, D, prow, cnt, [&](int q) { return key_id(P[cl[q]]); },
+    eval_batch<MC>(reinterpret_cast<const T*>(a.rows), D, prow, cnt, [&](int q) { return key_id(P[cl[q]]); },
                    [&](int q, float res) {
                      const int i = cl[q];
                      const uint64_t k = P[i];
@@ -259,7 +259,7 @@ __device__ __forceinline__ void count_add(unsigned long long* c, int idx, uint32
 // -------------------------------------------------------------------------------------
 // Forward pass (Collect + Filter + Store reverse pushes).  kind 0: path, 1: two-hop.
 // -------------------------------------------------------------------------------------
-template <int MC, int KIND>
+template <int MC, class T, int KIND>
 __global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
   __shared__ uint64_t P[POOL_MAX];
   __shared__ uint64_t M0[W_MAX], M1[W_MAX], S[128];
@@ -272,7 +272,7 @@ __global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
   const int lane = threadIdx.x, t = lane & 15;
   const int K = a.K, D = a.D;
   Row16<MC> xu;
-  load_row16<MC>(xu, a.X + size_t(u) * D, D, t);
+  load_row16<MC>(xu, reinterpret_cast<const T*>(a.rows) + size_t(u) * D, D, t);
   for (int i = lane; i < POOL_MAX; i += TG_WAVE)
     P[i] = (i < K + RING) ? mk_key(__uint_as_float(0x7F800000u), TG_INV) : TG_KEY_INF;
   __syncthreads();
@@ -281,7 +281,7 @@ __global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
   uint32_t n_col = 0, n_adj = 0, n_flt = 0;
   if (KIND == 0) {
     uint32_t ring_c = 0;
-    beam_search<MC>(a, xu, *a.ep, 10, M0, M1, S, tab, expl, cl, P + K, ring_c, n_col, n_adj);
+    beam_search<MC, T>(a, xu, *a.ep, 10, M0, M1, S, tab, expl, cl, P + K, ring_c, n_col, n_adj);
   } else {
     // two-hop ids y_i = G[G[u][i/K]][i%K], i < 480
     for (int i = lane; i < RING; i += TG_WAVE) {
@@ -289,7 +289,7 @@ __global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
       P[K + i] = mk_key(0.f, a.G[size_t(v) * K + (i % K)]);
     }
     __syncthreads();
-    eval_batch<MC>(a.X, D, xu, RING, [&](int q) { return key_id(P[K + q]); },
+    eval_batch<MC>(reinterpret_cast<const T*>(a.rows), D, xu, RING, [&](int q) { return key_id(P[K + q]); },
                    [&](int q, float d) { P[K + q] = mk_key(d, key_id(P[K + q])); });
     n_col = RING;
     n_adj = (RING + K - 1) / K;
@@ -297,7 +297,7 @@ __global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
   __syncthreads();
   const int L = K + RING;
   sort_dedup(P, pow2_at_least(L));
-  const int deg = filter_pool<MC>(a, P, L, u, a.fwd_rule, sel, cl, n_flt);
+  const int deg = filter_pool<MC, T>(a, P, L, u, a.fwd_rule, sel, cl, n_flt);
   __syncthreads();
   for (int k = lane; k < deg; k += TG_WAVE) {
     const uint64_t s = sel[k];
@@ -315,7 +315,7 @@ __global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
 // Reverse pass (filter_reverse, spec A.7): forward list plus the K best reverse pushes;
 // re-filtered with the reverse rule when more than FD, otherwise their union.
 // -------------------------------------------------------------------------------------
-template <int MC>
+template <int MC, class T>
 __global__ __launch_bounds__(64) void k_reverse(PruneArgs a) {
   __shared__ uint64_t buf[256];
   __shared__ uint64_t P[256];
@@ -340,7 +340,7 @@ __global__ __launch_bounds__(64) void k_reverse(PruneArgs a) {
     }
     __syncthreads();
     sort_dedup(P, n);
-    const int nd = filter_pool<MC>(a, P, L, u, a.rev_rule, sel, cl, n_flt);
+    const int nd = filter_pool<MC, T>(a, P, L, u, a.rev_rule, sel, cl, n_flt);
     __syncthreads();
     for (int k = lane; k < FD; k += TG_WAVE) row[k] = k < nd ? key_id(sel[k]) : TG_INV;
     if (lane == 0) a.out_deg[u] = nd;
@@ -380,7 +380,7 @@ __global__ __launch_bounds__(64) void k_reverse(PruneArgs a) {
 // -------------------------------------------------------------------------------------
 // One-hop rank (CAGRA: id membership; DPG: occlusion by distance), spec A.8 / A.9.
 // -------------------------------------------------------------------------------------
-template <int MC, int DPG>
+template <int MC, class T, int DPG>
 __global__ __launch_bounds__(64) void k_rank_forward(PruneArgs a) {
   __shared__ uint64_t srt[256];   // (id << 32 | pos) sorted, then (detour << 32 | id)
   __shared__ uint32_t det[128];
@@ -420,9 +420,10 @@ __global__ __launch_bounds__(64) void k_rank_forward(PruneArgs a) {
     // pairs r < j, row by row; the four groups take every fourth j of a row
     const int npairs = K * (K - 1) / 2;
     for (int r = 0; r + 1 < K; ++r) {
-      const float* xr = a.X + size_t(nei[r]) * D;
+      const T* xr = reinterpret_cast<const T*>(a.rows) + size_t(nei[r]) * D;
       for (int j = r + 1 + g; j < K; j += 4) {
-        const float dd = dist16_mm<MC>(xr, a.X + size_t(nei[j]) * D, D, t);
+        const float dd =
+            dist16_mm<MC>(xr, reinterpret_cast<const T*>(a.rows) + size_t(nei[j]) * D, D, t);
         if (t == 0 && dd < a.GD[size_t(u) * K + j]) atomicAdd(&det[j], 1u);
       }
     }
@@ -498,7 +499,7 @@ __global__ __launch_bounds__(64) void k_rank_reverse(PruneArgs a) {
 // -------------------------------------------------------------------------------------
 // Entry point: one workgroup, beam of TOPM from node 0's list toward the centroid.
 // -------------------------------------------------------------------------------------
-template <int MC>
+template <int MC, class T>
 __global__ __launch_bounds__(64) void k_cal_ep(PruneArgs a, const float* centroid, uint32_t* ep) {
   __shared__ uint64_t M0[W_MAX], M1[W_MAX], S[128];
   __shared__ uint32_t tab[HSET];
@@ -507,7 +508,7 @@ __global__ __launch_bounds__(64) void k_cal_ep(PruneArgs a, const float* centroi
   Row16<MC> c;
   load_row16<MC>(c, centroid, a.D, threadIdx.x & 15);
   uint32_t rc = 0, ne = 0, na = 0;
-  const uint64_t* B = beam_search<MC>(a, c, 0u, 30, M0, M1, S, tab, expl, cl, nullptr, rc, ne, na);
+  const uint64_t* B = beam_search<MC, T>(a, c, 0u, 30, M0, M1, S, tab, expl, cl, nullptr, rc, ne, na);
   if (threadIdx.x == 0) *ep = key_id(B[0]);
   count_add(a.counters, 4, ne);
 }
@@ -531,28 +532,35 @@ __global__ void k_centroid_final(const double* part, uint32_t nch, uint32_t N, u
 }
 
 // ------------------------------------------------------------------------- launchers
-template <int MC>
+template <int MC, class T>
 static void launch_prune_mc(const PruneArgs& a, int phase, const float* centroid, uint32_t* ep,
                             hipStream_t s) {
   const dim3 grid(a.N), block(64);
   switch (phase) {
-    case PH_EP: k_cal_ep<MC><<<1, 64, 0, s>>>(a, centroid, ep); break;
-    case PH_FWD_PATH: k_forward<MC, 0><<<grid, block, 0, s>>>(a); break;
-    case PH_FWD_2HOP: k_forward<MC, 1><<<grid, block, 0, s>>>(a); break;
-    case PH_REV: k_reverse<MC><<<grid, block, 0, s>>>(a); break;
-    case PH_CAGRA: k_rank_forward<MC, 0><<<grid, block, 0, s>>>(a); break;
-    case PH_DPG: k_rank_forward<MC, 1><<<grid, block, 0, s>>>(a); break;
+    case PH_EP: k_cal_ep<MC, T><<<1, 64, 0, s>>>(a, centroid, ep); break;
+    case PH_FWD_PATH: k_forward<MC, T, 0><<<grid, block, 0, s>>>(a); break;
+    case PH_FWD_2HOP: k_forward<MC, T, 1><<<grid, block, 0, s>>>(a); break;
+    case PH_REV: k_reverse<MC, T><<<grid, block, 0, s>>>(a); break;
+    case PH_CAGRA: k_rank_forward<MC, T, 0><<<grid, block, 0, s>>>(a); break;
+    case PH_DPG: k_rank_forward<MC, T, 1><<<grid, block, 0, s>>>(a); break;
     case PH_RANK_REV: k_rank_reverse<<<grid, block, 0, s>>>(a); break;
     default: break;
   }
 }
 
+template <class T>
+static void launch_prune_t(const PruneArgs& a, int phase, const float* centroid, uint32_t* ep,
+                           hipStream_t s) {
+  const int mc = (a.D + 63) / 64;
+  if (mc <= 2) launch_prune_mc<2, T>(a, phase, centroid, ep, s);
+  else if (mc <= 4) launch_prune_mc<4, T>(a, phase, centroid, ep, s);
+  else launch_prune_mc<16, T>(a, phase, centroid, ep, s);
+}
+
 void launch_prune(const PruneArgs& a, int phase, const float* centroid, uint32_t* ep,
                   hipStream_t s) {
-  const int mc = (a.D + 63) / 64;
-  if (mc <= 2) launch_prune_mc<2>(a, phase, centroid, ep, s);
-  else if (mc <= 4) launch_prune_mc<4>(a, phase, centroid, ep, s);
-  else launch_prune_mc<16>(a, phase, centroid, ep, s);
+  if (a.rows_bf16) launch_prune_t<uint16_t>(a, phase, centroid, ep, s);
+  else launch_prune_t<float>(a, phase, centroid, ep, s);
 }
 
 void launch_centroid(const float* X, uint32_t N, uint32_t D, double* part, float* c,

@@ -137,6 +137,7 @@ void Pruning(unsigned K, unsigned POINTS, unsigned DIM, unsigned FINAL_DEGREE, u
   auto h = holder_of(ptrs);
   check_shape(h->s, K, POINTS, DIM);
   tg_prune_params p{};
+  p.precision = TG_PREC_BF16;  // BF16 rows only when exact for this data set
   p.FD = FINAL_DEGREE;
   p.TOPM = TOPM;
   p.thre = thre;
@@ -174,6 +175,7 @@ void Pruning_with_CFS(unsigned K, unsigned POINTS, unsigned DIM, unsigned FINAL_
   auto h = holder_of(ptrs);
   check_shape(h->s, K, POINTS, DIM);
   tg_prune_params p{};
+  p.precision = TG_PREC_BF16;  // BF16 rows only when exact for this data set
   p.FD = FINAL_DEGREE;
   p.TOPM = TOPM;
   p.thre = thre;
@@ -201,7 +203,7 @@ PYBIND11_MODULE(Tagore, m) {
   m.doc() = "Tagore GPU Indexing";
   m.def("GNN_descent", &GNN_descent, "GNN_descent", py::arg("K"), py::arg("POINTS"),
         py::arg("DIM"), py::arg("iter"), py::arg("fname"), py::kw_only(),
-        py::arg("precision") = int(TG_PREC_EXACT), py::arg("seed") = uint64_t(0x7A600000ull),
+        py::arg("precision") = int(TG_PREC_BF16), py::arg("seed") = uint64_t(0x7A600000ull),
         py::arg("device") = 0);
   m.def("Pruning", &Pruning, "Pruning", py::arg("K"), py::arg("POINTS"), py::arg("DIM"),
         py::arg("FINAL_DEGREE"), py::arg("TOPM") = 64, py::arg("ptrs"), py::arg("index_type"),

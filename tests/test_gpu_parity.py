@@ -37,22 +37,23 @@ def unit_case():
     return X, G, GD
 
 
-def _gpu_prune(X, G, GD, kind, fr, rr, FD, TOPM, th):
+def _gpu_prune(X, G, GD, kind, fr, rr, FD, TOPM, th, precision="fp32"):
     with State(*X.shape) as s:
         s.set_data(X)
         s.set_knn(G, GD)
-        s.prune(kind, fr, rr, FD, TOPM, th)
+        s.prune(kind, fr, rr, FD, TOPM, th, precision)
         deg, ids, ep = s.get_pruned()
         fdeg, fids, fd = s.get_forward()
         st = s.stats()
     return deg, ids, ep, fdeg, fids, fd, st
 
 
-def _compare(X, G, GD, kind, fr, rr, FD, TOPM, th):
+def _compare(X, G, GD, kind, fr, rr, FD, TOPM, th, precision="fp32"):
     _say(f"prune parity kind={kind} fwd={fr} rev={rr} FD={FD} TOPM={TOPM} th={th} "
-         f"N={X.shape[0]} D={X.shape[1]} K={G.shape[1]}")
+         f"N={X.shape[0]} D={X.shape[1]} K={G.shape[1]} rows={precision}")
     ref = oracle.prune(X, G, GD, FD, TOPM, kind, fr, rr, th, stages=True)
-    deg, ids, ep, fdeg, fids, fd, st = _gpu_prune(X, G, GD, kind, fr, rr, FD, TOPM, th)
+    deg, ids, ep, fdeg, fids, fd, st = _gpu_prune(X, G, GD, kind, fr, rr, FD, TOPM, th,
+                                                  precision)
     N = X.shape[0]
     assert np.array_equal(fdeg, ref["fwd_deg"])
     for u in range(N):
@@ -99,6 +100,24 @@ def test_prune_parity_unit(unit_case, name, kind, fr, rr, th):
     _compare(X, G, GD, kind, fr, rr, 32, 64, th)
 
 
+@pytest.mark.parametrize("name,kind,fr,rr,th", [
+    ("NSG", 0, 0, 0, 1.0),
+    ("NSSG", 1, 1, 1, 0.5),
+    ("DPG", 3, 0, 2, 0.0),
+])
+def test_prune_parity_bf16_rows(int_case, name, kind, fr, rr, th):
+    """BF16 rows are read only for data sets that BF16 holds exactly: integer data in
+    [0,255] and BF16-rounded unit vectors take the BF16 path, plain unit vectors fall back
+    to FP32 rows; all three must equal the oracle bit for bit."""
+    X, G, GD = int_case
+    _compare(X, G, GD, kind, fr, rr, 32, 64, th, "bf16")
+    U = gmm_unit(2000, 96, 16, 303)
+    Ub = (U.view(np.uint32) & np.uint32(0xFFFF0000)).view(np.float32)  # exact in BF16
+    for Y in (U, Ub):
+        GY, GDY = oracle.knn_bruteforce(Y, 48)
+        _compare(Y, GY, GDY, kind, fr, rr, 32, 64, th, "bf16")
+
+
 def test_prune_parity_small_k_and_odd_sizes():
     # K < TOPM (beam padding), FD > K/2, a degree cap that is reached often
     X = gmm_int(1500, 64, 8, 7)
@@ -128,11 +147,23 @@ def test_index_files_are_byte_identical(int_case, tmp_path):
 
 @pytest.mark.parametrize("precision", ["exact", "fp32", "bf16"])
 @pytest.mark.parametrize("N,D,K,iters,kind", [(3000, 128, 32, 6, "int"), (2500, 96, 48, 8, "unit"),
-                                              (1200, 64, 64, 4, "int")])
+                                              (1200, 64, 64, 4, "int"), (2000, 96, 32, 6, "unitb"),
+                                              (1500, 128, 32, 6, "sint")])
 def test_knn_bit_identical(N, D, K, iters, kind, precision):
-    """Every precision gives the oracle's graph: the screens only decide what to re-score."""
+    """Every precision gives the oracle's graph: the screens only decide what to re-score.
+
+    int: non-negative integers (exact MFMA screen, no re-score); sint: signed integers
+    (exact screen under the tighter signed bound); unitb: unit vectors truncated to BF16
+    (re-scores read the BF16 copy); unit: general FP32 (re-scores read FP32 rows)."""
     _say(f"kNN {precision} N={N} D={D} K={K} iters={iters} data={kind}")
-    X = gmm_int(N, D, 16, 5) if kind == "int" else gmm_unit(N, D, 16, 5)
+    if kind == "int":
+        X = gmm_int(N, D, 16, 5)
+    elif kind == "sint":
+        X = gmm_int(N, D, 16, 5) - np.float32(128)
+    else:
+        X = gmm_unit(N, D, 16, 5)
+        if kind == "unitb":
+            X = (X.view(np.uint32) & np.uint32(0xFFFF0000)).view(np.float32)
     seed = 0x1234
     Gr, GDr, ev = oracle.gnn_descent(X, K, iters, seed)
     with State(N, D) as s:
