@@ -92,7 +92,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--seed", type=int, default=0x7A600002)
     ap.add_argument("--recall-queries", type=int, default=10000)
-    ap.add_argument("--cpu-sample", type=int, default=30000,
+    ap.add_argument("--cpu-sample", type=int, default=100000,
                     help="vectors in the CPU-oracle baseline sample (0 skips it)")
     args = ap.parse_args()
 
