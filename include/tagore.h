@@ -46,11 +46,20 @@ typedef struct tg_state tg_state;
 #define TG_RULE_ANGLE 1
 #define TG_RULE_RANK 2
 
+/* initial lists of GNN-Descent */
+#define TG_INIT_RANDOM 0  /* seeded random ids (initialize_graph, Tagore_src.cu:158-166) */
+#define TG_INIT_LEARNED 1 /* MLP codes -> code-space buckets -> seeded lists (D % 32 == 0) */
+#define TG_INIT_GIVEN 2   /* the ids of the current graph (tg_set_knn / a previous build) */
+
 typedef struct {
   uint32_t K;         /* neighbours per node */
-  uint32_t iters;     /* iterations; iters/2 local-join + iters/2 expansion (pybind.cu:101) */
-  uint64_t seed;      /* random initial lists */
+  uint32_t iters;     /* iterations; iters/2 local-join + iters/2 expansion (pybind.cu:101);
+                         0 returns the initial lists */
+  uint64_t seed;      /* random streams of the initial lists */
   int32_t precision;  /* TG_PREC_* */
+  int32_t init;       /* TG_INIT_* */
+  uint32_t sample;    /* learned init: sample size (0: min(N, 50000)) */
+  uint32_t epochs;    /* learned init: training epochs (0: 5) */
 } tg_knn_params;
 
 typedef struct {
@@ -119,10 +128,30 @@ typedef struct {
   uint64_t knn_evals;        /* distance evaluations of the last build */
   uint64_t knn_rescored;     /* canonical FP32 re-scores after the screen (phase 2 + join) */
   uint64_t prune_counters[8];/* collect, filter, reverse, adjacency rows, ep evaluations */
+  double learn_ms[5];        /* learned init: sample graph, training, encoding, buckets, seeding
+                                (filled when profiling is on) */
+  float learn_loss[2];       /* training loss at the end of the first and the last epoch */
+  uint32_t learn_buckets;    /* code-space buckets */
+  uint32_t learn_steps;      /* Adam steps */
 } tg_stats;
 int tg_get_stats(tg_state* s, tg_stats* out);
 /* 1: time each kernel with events (adds a little overhead), 0: only the windows. */
 int tg_set_profiling(tg_state* s, int on);
+
+/* Codes [N*32] of the last learned initialisation (FP32). */
+int tg_get_codes(tg_state* s, float* codes);
+
+/* Learned-init encoder checks (no counterpart in the reference, which uses random init).
+ * tg_mlp_eval: one forward/backward of the encoder D->256->32 on host rows X[M*D],
+ * M = B*(1+P) (rows [0,B) anchors, row B+b*P+j positive j of anchor b), normalisation
+ * (x-mu)*scale, flat parameters [256*(D+1)] (W1 | b1 per hidden unit) then [32*257]
+ * (W2 | b2 per code); returns the loss, its gradient (same layout) and the codes [M*32].
+ * tg_mlp_adam: one Adam step t (beta 0.9/0.999, eps 1e-8) of the flat block in place. */
+int tg_mlp_eval(int device, uint32_t D, uint32_t B, uint32_t P, const float* X, const float* mu,
+                float scale, const float* params, float margin, float* loss, float* grad,
+                float* codes);
+int tg_mlp_adam(int device, uint32_t D, uint32_t t, float lr, const float* grad, float* params,
+                float* m, float* v);
 
 #ifdef __cplusplus
 }

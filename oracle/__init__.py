@@ -75,6 +75,17 @@ def lib():
         L.or_gnn_descent.restype = ctypes.c_int
         L.or_gnn_descent.argtypes = [f32p, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, ctypes.c_uint64, u32p, f32p, vp]
+        L.or_gnn_descent_from.restype = ctypes.c_int
+        L.or_gnn_descent_from.argtypes = [f32p, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_uint64, u32p, f32p, u32p,
+                                          f32p, vp]
+        L.or_mlp_eval.restype = ctypes.c_int
+        L.or_mlp_eval.argtypes = [f32p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  f32p, ctypes.c_float, f64p, ctypes.c_double, ctypes.c_int,
+                                  f64p, f64p, f64p]
+        L.or_mlp_adam.restype = None
+        L.or_mlp_adam.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_double, f64p, f64p,
+                                  f64p, f64p]
         L.or_filter_pool.restype = ctypes.c_int
         L.or_filter_pool.argtypes = [f32p, ctypes.c_int, u32p, f32p, ctypes.c_int,
                                      ctypes.c_uint32, ctypes.c_int, ctypes.c_int,
@@ -215,13 +226,22 @@ def store(path: str, index_type: str, deg, ids, ep: int = 0, eps=None):
         raise IOError(f"or_store failed for {path}")
 
 
-def gnn_descent(X, K: int, iters: int, seed: int):
+def gnn_descent(X, K: int, iters: int, seed: int, init=None):
+    """GNN-Descent restatement; `init` = (G0, GD0) starts from given sorted lists instead of
+    the seeded random ones.  iters == 0 returns the initial lists."""
     X = _f32(X)
     N = X.shape[0]
     G = np.empty((N, K), np.uint32)
     GD = np.empty((N, K), np.float32)
     cnt = np.zeros(4, np.uint64)
-    if lib().or_gnn_descent(X, N, X.shape[1], K, iters, seed, G, GD, _ptr(cnt)) != 0:
+    if init is None:
+        rc = lib().or_gnn_descent(X, N, X.shape[1], K, iters, seed, G, GD, _ptr(cnt))
+    else:
+        G0, GD0 = _u32(init[0]), _f32(init[1])
+        assert G0.shape == (N, K) and GD0.shape == (N, K)
+        rc = lib().or_gnn_descent_from(X, N, X.shape[1], K, iters, seed, G0, GD0, G, GD,
+                                       _ptr(cnt))
+    if rc != 0:
         raise ValueError("invalid GNN-Descent arguments")
     return G, GD, int(cnt[0])
 
@@ -245,3 +265,32 @@ def collect_path(X, G, GD, TOPM: int, ep: int, u: int):
     pd = np.zeros(K + 480, np.float32)
     c = lib().or_collect_path(X, X.shape[1], G, GD, K, TOPM, ep, u, pid, pd)
     return pid, pd, int(c)
+
+
+MLP_HID, MLP_CODE = 256, 32
+
+
+def mlp_nparams(D: int) -> int:
+    return MLP_HID * (D + 1) + MLP_CODE * (MLP_HID + 1)
+
+
+def mlp_eval(X, B: int, P: int, mu, scale: float, params, margin: float = 1.0,
+             round_bf16: bool = True):
+    """Encoder forward/backward in FP64 (optionally with the HIP path's BF16 roundings).
+    Returns (loss, grad[nparams], codes[M, 32])."""
+    X = _f32(X)
+    M, D = X.shape
+    prm = np.ascontiguousarray(params, np.float64)
+    assert prm.shape == (mlp_nparams(D),)
+    loss = np.zeros(1, np.float64)
+    grad = np.zeros_like(prm)
+    codes = np.zeros((M, MLP_CODE), np.float64)
+    if lib().or_mlp_eval(X, M, D, B, P, _f32(mu), float(scale), prm, float(margin),
+                         1 if round_bf16 else 0, loss, grad, codes) != 0:
+        raise ValueError("bad encoder batch shape")
+    return float(loss[0]), grad, codes
+
+
+def mlp_adam(t: int, lr: float, g, p, m, v):
+    """One FP64 Adam step in place on p, m, v (float64 arrays)."""
+    lib().or_mlp_adam(p.shape[0], t, lr, np.ascontiguousarray(g, np.float64), p, m, v)

@@ -792,13 +792,30 @@ void or_nssg_eps(int64_t N, uint64_t seed, uint32_t* eps) {
 //     with d < worst, not listed, not u; merge.
 // Outputs G (N*K ids), GD (N*K dists).  Returns 0 on success.
 // ---------------------------------------------------------------------------------------
+// or_gnn_descent_from: the same iterations started from given lists G0/GD0 (rows sorted by
+// (d,id), no self, no repeats, canonical distances; all entries unvisited).  iters == 0
+// returns the initial lists unchanged (no reverse merge).
+int or_gnn_descent_from(const float* X, int64_t N, int D, int K, int iters, uint64_t seed,
+                        const uint32_t* G0, const float* GD0, uint32_t* G, float* GD,
+                        uint64_t* counters);
+
 int or_gnn_descent(const float* X, int64_t N, int D, int K, int iters, uint64_t seed,
                    uint32_t* G, float* GD, uint64_t* counters) {
+  return or_gnn_descent_from(X, N, D, K, iters, seed, nullptr, nullptr, G, GD, counters);
+}
+
+int or_gnn_descent_from(const float* X, int64_t N, int D, int K, int iters, uint64_t seed,
+                        const uint32_t* Ginit, const float* GDinit, uint32_t* G, float* GD,
+                        uint64_t* counters) {
   const int S = 32, TILE = 16, P2_LEADERS = 4, P2_TAKE = 64;
   if (K < 1 || N <= K) return 1;
   const int nthreads = omp_get_max_threads();
   std::vector<uint8_t> V(size_t(N) * K, 0);
   uint64_t evals = 0;
+  if (Ginit) {
+    std::copy(Ginit, Ginit + size_t(N) * K, G);
+    std::copy(GDinit, GDinit + size_t(N) * K, GD);
+  } else {
   // ---- init ----
 #pragma omp parallel
   {
@@ -828,6 +845,11 @@ int or_gnn_descent(const float* X, int64_t N, int D, int K, int iters, uint64_t 
     }
   }
   evals += uint64_t(N) * K;
+  }
+  if (iters == 0) {
+    if (counters) counters[0] = evals;
+    return 0;
+  }
 
   auto merge_into = [&](int64_t u, std::vector<Ent>& cand, uint32_t* g, float* gd,
                         uint8_t* vis) {
@@ -1174,6 +1196,126 @@ uint64_t or_collect_path(const float* X, int D, const uint32_t* G, const float* 
     pool_d[i] = P[i].d;
   }
   return c;
+}
+
+// ---------------------------------------------------------------------------------------
+// Learned-init encoder (no counterpart in the reference, whose lists start random:
+// initialize_graph, Tagore_src.cu:158-166).  FP64 restatement of the HIP forward/backward
+// with the same BF16 rounding points when round != 0:
+//   xn = bf16((x - mu) * s); h = bf16(relu(bf16(W1) xn + b1)); z = bf16(W2) h + b2;
+//   L = 1/(B P) sum_{b,j} |z_b - z_p(b,j)|^2 + 1/(B (B-1)) sum_{b != c} relu(m - |z_b - z_c|^2)
+//     (rows [0,B) anchors, row B + b P + j positive j of anchor b; a pair is active iff
+//      |z_b - z_c|^2 < m);
+//   dz -> bf16; [dW2 | db2] = dz^T [h | 1]; dh = bf16((dz bf16(W2)) * [h > 0]);
+//   [dW1 | db1] = dh^T [xn | 1].
+// With round == 0 every quantity is plain FP64, the form the finite-difference test pins.
+// Parameter layout: [256][D+1] (W1 row | b1) then [32][257] (W2 row | b2).
+// ---------------------------------------------------------------------------------------
+static double bf16_round(double x, int round) {
+  if (!round) return x;
+  float f = float(x);
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u = (u + 0x7FFFu + ((u >> 16) & 1u)) & 0xFFFF0000u;
+  std::memcpy(&f, &u, 4);
+  return double(f);
+}
+
+int or_mlp_eval(const float* X, int64_t M, int D, int B, int P, const float* mu, float scale,
+                const double* prm, double margin, int round, double* loss, double* grad,
+                double* codes) {
+  const int H = 256, C = 32;
+  if (M != int64_t(B) * (1 + P) || B < 2) return 1;
+  const size_t n1 = size_t(H) * (D + 1);
+  std::vector<double> W1(size_t(H) * D), W2(size_t(C) * H);
+  for (int j = 0; j < H; ++j)
+    for (int i = 0; i < D; ++i) W1[size_t(j) * D + i] = bf16_round(prm[size_t(j) * (D + 1) + i], round);
+  for (int k = 0; k < C; ++k)
+    for (int j = 0; j < H; ++j) W2[size_t(k) * H + j] = bf16_round(prm[n1 + size_t(k) * (H + 1) + j], round);
+  std::vector<double> xn(size_t(M) * D), h(size_t(M) * H), z(size_t(M) * C);
+  for (int64_t m = 0; m < M; ++m)
+    for (int i = 0; i < D; ++i)
+      xn[m * D + i] = bf16_round(round ? double(float((X[m * D + i] - mu[i]) * scale))
+                                       : (double(X[m * D + i]) - mu[i]) * double(scale),
+                                 round);
+#pragma omp parallel for schedule(static)
+  for (int64_t m = 0; m < M; ++m) {
+    for (int j = 0; j < H; ++j) {
+      double a = prm[size_t(j) * (D + 1) + D];
+      for (int i = 0; i < D; ++i) a += W1[size_t(j) * D + i] * xn[m * D + i];
+      h[m * H + j] = bf16_round(a > 0 ? a : 0.0, round);
+    }
+    for (int k = 0; k < C; ++k) {
+      double a = prm[n1 + size_t(k) * (H + 1) + H];
+      for (int j = 0; j < H; ++j) a += W2[size_t(k) * H + j] * h[m * H + j];
+      z[m * C + k] = a;
+    }
+  }
+  std::vector<double> dz(size_t(M) * C, 0.0);
+  const double cpos = 2.0 / (double(B) * P), cneg = 4.0 / (double(B) * (B - 1));
+  double lpos = 0, lneg = 0;
+  for (int b = 0; b < B; ++b) {
+    for (int j = 0; j < P; ++j) {
+      const int64_t r = B + int64_t(b) * P + j;
+      for (int k = 0; k < C; ++k) {
+        const double e = z[b * C + k] - z[r * C + k];
+        lpos += e * e;
+        dz[b * C + k] += cpos * e;
+        dz[r * C + k] += -cpos * e;
+      }
+    }
+    for (int c = 0; c < B; ++c) {
+      if (c == b) continue;
+      double d2 = 0;
+      for (int k = 0; k < C; ++k) {
+        const double e = z[b * C + k] - z[c * C + k];
+        d2 += e * e;
+      }
+      if (d2 < margin) {
+        lneg += margin - d2;
+        for (int k = 0; k < C; ++k) dz[b * C + k] -= cneg * (z[b * C + k] - z[c * C + k]);
+      }
+    }
+  }
+  *loss = lpos / (double(B) * P) + lneg / (double(B) * (B - 1));
+  for (auto& v : dz) v = bf16_round(v, round);
+  // [dW2 | db2]
+  for (int k = 0; k < C; ++k) {
+    for (int j = 0; j <= H; ++j) {
+      double a = 0;
+      for (int64_t m = 0; m < M; ++m) a += dz[m * C + k] * (j < H ? h[m * H + j] : 1.0);
+      grad[n1 + size_t(k) * (H + 1) + j] = a;
+    }
+  }
+  // dh, then [dW1 | db1]
+  std::vector<double> dh(size_t(M) * H);
+  for (int64_t m = 0; m < M; ++m)
+    for (int j = 0; j < H; ++j) {
+      double a = 0;
+      for (int k = 0; k < C; ++k) a += dz[m * C + k] * W2[size_t(k) * H + j];
+      dh[m * H + j] = bf16_round(h[m * H + j] > 0 ? a : 0.0, round);
+    }
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < H; ++j)
+    for (int i = 0; i <= D; ++i) {
+      double a = 0;
+      for (int64_t m = 0; m < M; ++m) a += dh[m * H + j] * (i < D ? xn[m * D + i] : 1.0);
+      grad[size_t(j) * (D + 1) + i] = a;
+    }
+  if (codes) std::copy(z.begin(), z.end(), codes);
+  return 0;
+}
+
+// Adam step t (Kingma & Ba 2015, Algorithm 1) in FP64: beta1 0.9, beta2 0.999, eps 1e-8,
+// each constant taken at its FP32 value (the values the HIP kernel holds).
+void or_mlp_adam(int64_t n, int t, double lr, const double* g, double* p, double* m, double* v) {
+  const double b1 = double(0.9f), b2 = double(0.999f), eps = double(1e-8f);
+  const double bc1 = 1.0 - std::pow(b1, t), bc2 = 1.0 - std::pow(b2, t);
+  for (int64_t i = 0; i < n; ++i) {
+    m[i] = b1 * m[i] + (1 - b1) * g[i];
+    v[i] = b2 * v[i] + (1 - b2) * g[i] * g[i];
+    p[i] -= lr * (m[i] / bc1) / (std::sqrt(v[i] / bc2) + eps);
+  }
 }
 
 int or_num_threads(void) { return omp_get_max_threads(); }

@@ -42,9 +42,14 @@ class TagoreError(RuntimeError):
         self.code = code
 
 
+INIT = {"random": 0, "learned": 1, "given": 2}
+MLP_HID, MLP_CODE = 256, 32
+
+
 class KnnParams(ctypes.Structure):
     _fields_ = [("K", ctypes.c_uint32), ("iters", ctypes.c_uint32), ("seed", ctypes.c_uint64),
-                ("precision", ctypes.c_int32)]
+                ("precision", ctypes.c_int32), ("init", ctypes.c_int32),
+                ("sample", ctypes.c_uint32), ("epochs", ctypes.c_uint32)]
 
 
 class PruneParams(ctypes.Structure):
@@ -58,7 +63,9 @@ class Stats(ctypes.Structure):
     _fields_ = [("knn_ms", ctypes.c_double), ("prune_ms", ctypes.c_double),
                 ("knn_kernel_ms", ctypes.c_double * 8), ("prune_kernel_ms", ctypes.c_double * 4),
                 ("knn_evals", ctypes.c_uint64), ("knn_rescored", ctypes.c_uint64),
-                ("prune_counters", ctypes.c_uint64 * 8)]
+                ("prune_counters", ctypes.c_uint64 * 8),
+                ("learn_ms", ctypes.c_double * 5), ("learn_loss", ctypes.c_float * 2),
+                ("learn_buckets", ctypes.c_uint32), ("learn_steps", ctypes.c_uint32)]
 
 
 KNN_SECTIONS = ["init", "sample", "rsample", "localjoin", "merge", "reverse", "phase2", "final"]
@@ -105,6 +112,10 @@ def lib():
     L.tg_buffer.argtypes = [vp, c.c_int32, P(vp), P(c.c_size_t)]
     L.tg_get_stats.argtypes = [vp, P(Stats)]
     L.tg_set_profiling.argtypes = [vp, c.c_int]
+    L.tg_get_codes.argtypes = [vp, vp]
+    L.tg_mlp_eval.argtypes = [c.c_int, c.c_uint32, c.c_uint32, c.c_uint32, vp, vp, c.c_float, vp,
+                              c.c_float, P(c.c_float), vp, vp]
+    L.tg_mlp_adam.argtypes = [c.c_int, c.c_uint32, c.c_uint32, c.c_float, vp, vp, vp, vp]
     for name in exported_symbols():
         f = getattr(L, name)
         if name not in ("tg_last_error", "tg_host_free"):
@@ -165,10 +176,16 @@ class State:
         return out
 
     # kNN
-    def knn_build(self, K: int, iters: int, seed: int = 0x7A600000, precision: str = "exact"):
-        p = KnnParams(K, iters, seed, PREC[precision])
+    def knn_build(self, K: int, iters: int, seed: int = 0x7A600000, precision: str = "exact",
+                  init: str = "random", sample: int = 0, epochs: int = 0):
+        p = KnnParams(K, iters, seed, PREC[precision], INIT[init], sample, epochs)
         _check(lib().tg_knn_build(self._h, ctypes.byref(p)))
         self.K = K
+
+    def codes(self):
+        out = np.empty((self.N, MLP_CODE), np.float32)
+        _check(lib().tg_get_codes(self._h, _ptr(out)))
+        return out
 
     def set_knn(self, ids, dists):
         ids = np.ascontiguousarray(ids, np.uint32)
@@ -222,7 +239,11 @@ class State:
                     prune_kernel_ms=dict(zip(PRUNE_SECTIONS, list(s.prune_kernel_ms))),
                     knn_evals=int(s.knn_evals), knn_rescored=int(s.knn_rescored),
                     prune_counters=dict(zip(["collect", "filter", "reverse", "adj_rows", "ep"],
-                                            list(s.prune_counters)[:5])))
+                                            list(s.prune_counters)[:5])),
+                    learn_ms=dict(zip(["sample_graph", "train", "encode", "buckets", "seed"],
+                                      list(s.learn_ms))),
+                    learn_loss=list(s.learn_loss), learn_buckets=int(s.learn_buckets),
+                    learn_steps=int(s.learn_steps))
 
     def set_profiling(self, on: bool):
         _check(lib().tg_set_profiling(self._h, 1 if on else 0))
@@ -231,6 +252,36 @@ class State:
         p, n = ctypes.c_void_p(), ctypes.c_size_t()
         _check(lib().tg_buffer(self._h, idx, ctypes.byref(p), ctypes.byref(n)))
         return p.value, n.value
+
+
+def mlp_nparams(D: int) -> int:
+    return MLP_HID * (D + 1) + MLP_CODE * (MLP_HID + 1)
+
+
+def mlp_eval(X, B: int, P: int, mu, scale: float, params, margin: float = 1.0, device: int = 0):
+    """Encoder forward/backward on the GPU: (loss, grad[nparams], codes[M, 32])."""
+    X = np.ascontiguousarray(X, np.float32)
+    M, D = X.shape
+    assert M == B * (1 + P)
+    prm = np.ascontiguousarray(params, np.float32)
+    assert prm.shape == (mlp_nparams(D),)
+    mu = np.ascontiguousarray(mu, np.float32)
+    grad = np.empty_like(prm)
+    codes = np.empty((M, MLP_CODE), np.float32)
+    loss = ctypes.c_float()
+    _check(lib().tg_mlp_eval(device, D, B, P, _ptr(X), _ptr(mu), scale, _ptr(prm), margin,
+                             ctypes.byref(loss), _ptr(grad), _ptr(codes)))
+    return loss.value, grad, codes
+
+
+def mlp_adam(D: int, t: int, lr: float, grad, params, m, v, device: int = 0):
+    """One Adam step on the GPU; returns the new (params, m, v) as float32 arrays."""
+    g = np.ascontiguousarray(grad, np.float32)
+    p = np.array(params, np.float32)
+    m = np.array(m, np.float32)
+    v = np.array(v, np.float32)
+    _check(lib().tg_mlp_adam(device, D, t, lr, _ptr(g), _ptr(p), _ptr(m), _ptr(v)))
+    return p, m, v
 
 
 def nssg_entry_points(N: int, seed: int):

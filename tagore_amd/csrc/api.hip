@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -16,6 +17,7 @@
 
 #include "../../include/tagore.h"
 #include "kernels.hpp"
+#include "learn.hpp"
 
 using namespace tg;
 
@@ -119,6 +121,12 @@ struct tg_state {
   bool has_prune = false;
   tg_prune_params last_prune{};
   uint32_t ep = 0;
+  // learned initial lists
+  Learner learn;
+  tg_state* sub = nullptr;     // the sample's own handle (its kNN graph trains the encoder)
+  uint32_t* samp = nullptr;    // sample ids
+  uint32_t samp_cap = 0;
+  bool has_codes = false;
   // stats
   bool profiling = false;
   tg_stats stats{};
@@ -350,6 +358,10 @@ int tg_free(tg_state* s) {
   if (!s) return TG_OK;
   (void)hipSetDevice(s->device);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
+  tg_free(s->sub);
+  s->sub = nullptr;
+  learner_release(s->learn);
+  dfree(s->samp);
   dfree(s->X);
   dfree(s->centres);
   dfree(s->misc);
@@ -479,6 +491,94 @@ int tg_set_profiling(tg_state* s, int on) {
   return TG_OK;
 }
 
+int tg_get_codes(tg_state* s, float* codes) {
+  if (!s || !codes) return fail(TG_ERR_ARG, "null argument");
+  if (!s->has_codes) return fail(TG_ERR_STATE, "no learned initialisation has run");
+  HIPX(hipSetDevice(s->device));
+  HIPX(hipMemcpy(codes, s->learn.codes, size_t(s->N) * MLP_CODE * 4, hipMemcpyDeviceToHost));
+  return TG_OK;
+}
+
+namespace {
+
+// device resources of the encoder checks, released on every exit path
+struct MlpScratch {
+  hipStream_t st = nullptr;
+  Learner L;
+  float* X = nullptr;
+  float* g = nullptr;
+  ~MlpScratch() {
+    if (st) (void)hipStreamSynchronize(st);
+    learner_release(L);
+    dfree(X);
+    dfree(g);
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+}  // namespace
+
+int tg_mlp_eval(int device, uint32_t D, uint32_t B, uint32_t P, const float* X, const float* mu,
+                float scale, const float* params, float margin, float* loss, float* grad,
+                float* codes) {
+  if (!X || !mu || !params || !loss || !grad || !codes) return fail(TG_ERR_ARG, "null argument");
+  if (D == 0 || D % 32 != 0 || D > 1024) return fail(TG_ERR_ARG, "D must be a multiple of 32 <= 1024");
+  if (B < 32 || B % 32 != 0 || B > 4096) return fail(TG_ERR_ARG, "B must be a multiple of 32 in [32, 4096]");
+  if (P < 1 || P > 16) return fail(TG_ERR_ARG, "P must be in [1, 16]");
+  HIPX(hipSetDevice(device));
+  const uint32_t M = B * (1 + P);
+  const size_t n = size_t(mlp_n1(D)) + mlp_n2();
+  MlpScratch w;
+  HIPX(hipStreamCreate(&w.st));
+  HIPX(learner_reserve(w.L, D, M, M, 1, 1));
+  if (dalloc(&w.X, size_t(M) * D) || dalloc(&w.g, n)) return TG_ERR_HIP;
+  HIPX(hipMemcpyAsync(w.X, X, size_t(M) * D * 4, hipMemcpyHostToDevice, w.st));
+  HIPX(hipMemcpyAsync(w.L.mu, mu, size_t(D) * 4, hipMemcpyHostToDevice, w.st));
+  HIPX(hipMemcpyAsync(w.L.scal, &scale, 4, hipMemcpyHostToDevice, w.st));
+  HIPX(mlp_set_params(w.L, params, w.st));
+  mlp_forward_backward(w.L, w.X, nullptr, B, P, margin, w.st);
+  LearnConfig c;
+  launch_mlp_adam(w.L, 1, 0, c, w.g, w.st);
+  if (int rc = check_launch("encoder check")) return rc;
+  std::vector<float> lp(B);
+  HIPX(hipMemcpyAsync(lp.data(), w.L.lossp, size_t(B) * 4, hipMemcpyDeviceToHost, w.st));
+  HIPX(hipMemcpyAsync(grad, w.g, n * 4, hipMemcpyDeviceToHost, w.st));
+  HIPX(hipMemcpyAsync(codes, w.L.Z, size_t(M) * MLP_CODE * 4, hipMemcpyDeviceToHost, w.st));
+  HIPX(hipStreamSynchronize(w.st));
+  double sum = 0.0;
+  for (float v : lp) sum += v;
+  *loss = float(sum);
+  return TG_OK;
+}
+
+int tg_mlp_adam(int device, uint32_t D, uint32_t t, float lr, const float* grad, float* params,
+                float* m, float* v) {
+  if (!grad || !params || !m || !v) return fail(TG_ERR_ARG, "null argument");
+  if (D == 0 || D % 32 != 0 || D > 1024) return fail(TG_ERR_ARG, "D must be a multiple of 32 <= 1024");
+  if (t < 1) return fail(TG_ERR_ARG, "t must be >= 1");
+  HIPX(hipSetDevice(device));
+  const size_t n1 = mlp_n1(D), n2 = mlp_n2(), n = n1 + n2;
+  MlpScratch w;
+  HIPX(hipStreamCreate(&w.st));
+  HIPX(learner_reserve(w.L, D, 32, 32, 1, 1));
+  fill_u32(reinterpret_cast<uint32_t*>(w.L.gpart), 0, n * w.L.splits, w.st);
+  HIPX(hipMemcpyAsync(w.L.prm, params, n * 4, hipMemcpyHostToDevice, w.st));
+  HIPX(hipMemcpyAsync(w.L.mom, m, n * 4, hipMemcpyHostToDevice, w.st));
+  HIPX(hipMemcpyAsync(w.L.vel, v, n * 4, hipMemcpyHostToDevice, w.st));
+  HIPX(hipMemcpyAsync(w.L.gpart, grad, n1 * 4, hipMemcpyHostToDevice, w.st));
+  HIPX(hipMemcpyAsync(w.L.gpart + n1 * w.L.splits, grad + n1, n2 * 4, hipMemcpyHostToDevice,
+                      w.st));
+  LearnConfig c;
+  c.lr = lr;
+  launch_mlp_adam(w.L, 2, t, c, nullptr, w.st);
+  if (int rc = check_launch("Adam check")) return rc;
+  HIPX(hipMemcpyAsync(params, w.L.prm, n * 4, hipMemcpyDeviceToHost, w.st));
+  HIPX(hipMemcpyAsync(m, w.L.mom, n * 4, hipMemcpyDeviceToHost, w.st));
+  HIPX(hipMemcpyAsync(v, w.L.vel, n * 4, hipMemcpyDeviceToHost, w.st));
+  HIPX(hipStreamSynchronize(w.st));
+  return TG_OK;
+}
+
 int tg_get_stats(tg_state* s, tg_stats* out) {
   if (!s || !out) return fail(TG_ERR_ARG, "null argument");
   *out = s->stats;
@@ -486,6 +586,63 @@ int tg_get_stats(tg_state* s, tg_stats* out) {
 }
 
 // ----------------------------------------------------------------------- kNN build
+namespace {
+
+// Learned initial ids in s->G: a sample's GNN-Descent graph trains the encoder, whose codes
+// bucket every vector (learn.hip).  Adds the sample build's distance evaluations to *evals.
+int learned_init(tg_state* s, const tg_knn_params* p, uint64_t* evals) {
+  using clk = std::chrono::steady_clock;
+  const uint32_t N = s->N, D = s->D, K = s->K;
+  const uint32_t S = std::min<uint32_t>(p->sample ? p->sample : 50000u, N);
+  const uint32_t Ks = std::min<uint32_t>(16u, S - 1);
+  hipStream_t st = s->stream;
+  const auto t0 = clk::now();
+  if (s->samp_cap < S) {
+    dfree(s->samp);
+    if (dalloc(&s->samp, S)) return TG_ERR_HIP;
+    s->samp_cap = S;
+  }
+  if (s->sub && s->sub->N != S) {
+    tg_free(s->sub);
+    s->sub = nullptr;
+  }
+  if (!s->sub)
+    if (int rc = tg_create(S, D, s->device, &s->sub)) return rc;
+  launch_sample_ids(s->samp, S, N, p->seed, st);
+  launch_gather_rows(s->X, D, s->samp, S, s->sub->X, st);
+  if (int rc = check_launch("learned init: sample")) return rc;
+  HIPX(hipStreamSynchronize(st));
+  s->sub->has_knn = s->sub->prep_valid = s->sub->has_prune = false;
+  tg_knn_params sp{};
+  sp.K = Ks;
+  sp.iters = 6;
+  sp.seed = p->seed ^ 0x5A3D1E5Bull;
+  sp.precision = TG_PREC_EXACT;
+  sp.init = TG_INIT_RANDOM;
+  if (int rc = tg_knn_build(s->sub, &sp)) return rc;
+  *evals += s->sub->stats.knn_evals;
+  s->stats.learn_ms[0] = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+  LearnConfig cfg;
+  if (p->epochs) cfg.epochs = p->epochs;
+  LearnStats ls;
+  const hipError_t e = learned_seed(s->learn, s->X, N, D, K, s->sub->X, s->samp, S, s->sub->G,
+                                    Ks, cfg, p->seed, s->G, st, &ls, s->profiling);
+  if (e != hipSuccess)
+    return fail(TG_ERR_HIP, std::string("learned init: ") + hipGetErrorString(e));
+  s->stats.learn_ms[1] = ls.ms_train;
+  s->stats.learn_ms[2] = ls.ms_encode;
+  s->stats.learn_ms[3] = ls.ms_cluster;
+  s->stats.learn_ms[4] = ls.ms_seed;
+  s->stats.learn_loss[0] = ls.loss_first;
+  s->stats.learn_loss[1] = ls.loss_last;
+  s->stats.learn_buckets = ls.buckets;
+  s->stats.learn_steps = ls.steps;
+  s->has_codes = true;
+  return TG_OK;
+}
+
+}  // namespace
+
 int tg_knn_build(tg_state* s, const tg_knn_params* p) {
   if (!s || !p) return fail(TG_ERR_ARG, "null argument");
   const uint32_t K = p->K;
@@ -495,6 +652,15 @@ int tg_knn_build(tg_state* s, const tg_knn_params* p) {
     return fail(TG_ERR_ARG, "unknown precision");
   if (p->precision == TG_PREC_BF16 && s->D % 8 != 0)
     return fail(TG_ERR_ARG, "BF16 screening needs D % 8 == 0");
+  if (p->init < TG_INIT_RANDOM || p->init > TG_INIT_GIVEN)
+    return fail(TG_ERR_ARG, "unknown init");
+  if (p->init == TG_INIT_GIVEN && (!s->has_knn || s->K != K))
+    return fail(TG_ERR_STATE, "init=given needs a current graph with the same K");
+  if (p->init == TG_INIT_LEARNED) {
+    if (s->D % 32 != 0) return fail(TG_ERR_ARG, "learned init needs D % 32 == 0");
+    if (std::min<uint32_t>(p->sample ? p->sample : 50000u, s->N) < 64)
+      return fail(TG_ERR_ARG, "learned init needs a sample of at least 64 vectors");
+  }
   HIPX(hipSetDevice(s->device));
   if (ensure_graph(s, K) || ensure_knn_scratch(s, K)) return TG_ERR_HIP;
   if (p->precision != TG_PREC_EXACT)
@@ -521,8 +687,18 @@ int tg_knn_build(tg_state* s, const tg_knn_params* p) {
   };
   KnnArgs a = args();
   const uint32_t N = s->N;
+  for (double& v : s->stats.learn_ms) v = 0.0;
+  s->stats.learn_loss[0] = s->stats.learn_loss[1] = 0.f;
+  s->stats.learn_buckets = s->stats.learn_steps = 0;
+  uint64_t extra_evals = 0;
   if (sec.start(0)) return TG_ERR_HIP;
-  launch_knn_init(a, st);
+  if (p->init == TG_INIT_RANDOM) {
+    launch_knn_init(a, st);
+  } else {
+    if (p->init == TG_INIT_LEARNED)
+      if (int rc = learned_init(s, p, &extra_evals)) return rc;
+    launch_knn_init_sort(a, st);
+  }
   if (sec.stop()) return TG_ERR_HIP;
   const uint32_t it1 = p->iters / 2, it2 = p->iters / 2;
   for (uint32_t it = 0; it < it1; ++it) {
@@ -547,29 +723,31 @@ int tg_knn_build(tg_state* s, const tg_knn_params* p) {
     if (sec.stop()) return TG_ERR_HIP;
     if (int rc = check_launch("GNN-Descent phase 1")) return rc;
   }
-  if (sec.start(5)) return TG_ERR_HIP;
-  pb_reset(a.pr, N, s->fill, st);
-  launch_knn_rev_build(a, st);
-  pb_group_overflow(a.pr, N, s->fill, s->scan_tmp, st);
-  launch_knn_rev_reduce(a, st);
-  if (sec.stop()) return TG_ERR_HIP;
-  for (uint32_t it = 0; it < it2; ++it) {
-    if (sec.start(6)) return TG_ERR_HIP;
+  if (p->iters > 0) {  // iters == 0 returns the initial lists
+    if (sec.start(5)) return TG_ERR_HIP;
     pb_reset(a.pr, N, s->fill, st);
-    launch_knn_phase2(a, p->precision, st);
+    launch_knn_rev_build(a, st);
     pb_group_overflow(a.pr, N, s->fill, s->scan_tmp, st);
-    launch_knn_rupdate(a, st);
+    launch_knn_rev_reduce(a, st);
     if (sec.stop()) return TG_ERR_HIP;
-    std::swap(s->G, s->G1);
-    std::swap(s->GD, s->GD1);
-    std::swap(s->R0, s->R1);
-    std::swap(s->Rn0, s->Rn1);
-    a = args();
-    if (int rc = check_launch("GNN-Descent phase 2")) return rc;
+    for (uint32_t it = 0; it < it2; ++it) {
+      if (sec.start(6)) return TG_ERR_HIP;
+      pb_reset(a.pr, N, s->fill, st);
+      launch_knn_phase2(a, p->precision, st);
+      pb_group_overflow(a.pr, N, s->fill, s->scan_tmp, st);
+      launch_knn_rupdate(a, st);
+      if (sec.stop()) return TG_ERR_HIP;
+      std::swap(s->G, s->G1);
+      std::swap(s->GD, s->GD1);
+      std::swap(s->R0, s->R1);
+      std::swap(s->Rn0, s->Rn1);
+      a = args();
+      if (int rc = check_launch("GNN-Descent phase 2")) return rc;
+    }
+    if (sec.start(7)) return TG_ERR_HIP;
+    launch_knn_final(a, st);
+    if (sec.stop()) return TG_ERR_HIP;
   }
-  if (sec.start(7)) return TG_ERR_HIP;
-  launch_knn_final(a, st);
-  if (sec.stop()) return TG_ERR_HIP;
   HIPX(hipEventRecord(w1, st));
   HIPX(hipStreamSynchronize(st));
   if (int rc = check_launch("GNN-Descent")) return rc;
@@ -582,7 +760,7 @@ int tg_knn_build(tg_state* s, const tg_knn_params* p) {
   if (sec.collect(s->stats.knn_kernel_ms)) return TG_ERR_HIP;
   unsigned long long cnt[8];
   HIPX(hipMemcpy(cnt, s->counters, sizeof(cnt), hipMemcpyDeviceToHost));
-  s->stats.knn_evals = cnt[0];
+  s->stats.knn_evals = cnt[0] + extra_evals;
   s->stats.knn_rescored = cnt[1];
   if (check_pb_err(s, s->pn, "reverse sampling") || check_pb_err(s, s->po, "reverse sampling") ||
       check_pb_err(s, s->pc, "local join") || check_pb_err(s, s->pr, "reverse lists"))

@@ -53,6 +53,32 @@ struct Row16 {
 // Four consecutive elements as FP32: rows are stored as FP32, or as BF16 bits when every
 // value of the data set is exactly representable in BF16 (then the widened values are the
 // FP32 values themselves and every distance below is bit-identical).
+// SplitMix64 (Steele, Lea, Flood, OOPSLA 2014): step, per-index stream start, and a
+// multiply-shift map of the high 32 bits onto [0, n)
+__device__ __forceinline__ uint64_t sm_next(uint64_t& s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ uint64_t sm_stream(uint64_t seed, uint64_t idx) {
+  uint64_t s = seed ^ (idx * 0xD6E8FEB86659FD93ull);
+  return sm_next(s);
+}
+__device__ __forceinline__ uint32_t sm_bounded(uint64_t x, uint32_t n) {
+  return uint32_t(((x >> 32) * uint64_t(n)) >> 32);
+}
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// round-to-nearest-even FP32 -> BF16 bits (finite inputs)
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return uint16_t((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }
+
 __device__ __forceinline__ float4 load4(const float* p) {
   return *reinterpret_cast<const float4*>(p);
 }

@@ -90,6 +90,8 @@ void launch_screen_prep(const float* X, uint32_t N, uint32_t D, uint16_t* Xb, fl
                         float* nq, float* ne, uint32_t* flags, hipStream_t s);
 
 void launch_knn_init(const KnnArgs& a, hipStream_t s);
+// canonical distances of the ids in G, rows sorted by (d, id), flags cleared
+void launch_knn_init_sort(const KnnArgs& a, hipStream_t s);
 void launch_knn_sample(const KnnArgs& a, hipStream_t s);
 void launch_knn_rsample(const KnnArgs& a, hipStream_t s);
 void launch_knn_localjoin(const KnnArgs& a, int precision, hipStream_t s);

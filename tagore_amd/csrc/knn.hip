@@ -23,20 +23,6 @@ constexpr int HROW = 128;    // sample-list row: new' [0,64), old' [64,128)
 constexpr int LEADERS = 4;   // phase-2 expansions per node
 constexpr int TAKE = 64;     // forward / reverse entries taken per leader
 
-__device__ __forceinline__ uint64_t sm_next(uint64_t& s) {
-  uint64_t z = (s += 0x9E3779B97F4A7C15ull);
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  return z ^ (z >> 31);
-}
-__device__ __forceinline__ uint64_t sm_stream(uint64_t seed, uint64_t idx) {
-  uint64_t s = seed ^ (idx * 0xD6E8FEB86659FD93ull);
-  return sm_next(s);
-}
-__device__ __forceinline__ uint32_t sm_bounded(uint64_t x, uint32_t n) {
-  return uint32_t(((x >> 32) * uint64_t(n)) >> 32);
-}
-
 // ------------------------------------------------------------------------------- init
 // The i-th draw of node u's SplitMix64 stream is a pure function of (stream start, i), so a
 // wave evaluates 64 draws at once and accepts them in stream order: a draw is taken unless it
@@ -576,6 +562,9 @@ __global__ __launch_bounds__(64) void k_final(KnnArgs a) {
     else KERNEL<16><<<GRID, BLOCK, 0, S>>> ARGS;                     \
   } while (0)
 
+void launch_knn_init_sort(const KnnArgs& a, hipStream_t s) {
+  TG_MC_DISPATCH(k_init_sort, a.N, 64, s, (a));
+}
 void launch_knn_init(const KnnArgs& a, hipStream_t s) {
   k_init_ids<<<a.N, 64, 0, s>>>(a);
   TG_MC_DISPATCH(k_init_sort, a.N, 64, s, (a));

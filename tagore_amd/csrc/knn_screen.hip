@@ -16,18 +16,10 @@
 
 namespace tg {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int LJ_TILE = 16;
 constexpr int HROW_S = 128;
 
-// round-to-nearest-even FP32 -> BF16 bits
-__device__ __forceinline__ uint16_t f2bf(float f) {
-  const uint32_t u = __float_as_uint(f);
-  return uint16_t((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
-}
-__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }
 
 // Per-vector screen statistics (one wave per row, FP64 sums) and data-set certificates:
 // flags[0] |= some value is not exactly representable in BF16,

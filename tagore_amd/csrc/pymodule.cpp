@@ -79,7 +79,10 @@ void check_shape(tg_state* s, unsigned K, unsigned POINTS, unsigned DIM) {
 
 // GNN_descent (pybind.cu:19-142)
 py::tuple GNN_descent(unsigned K, unsigned POINTS, unsigned DIM, unsigned iter, std::string fname,
-                      int precision, uint64_t seed, int device) {
+                      int precision, uint64_t seed, int device, std::string init) {
+  int init_kind = TG_INIT_RANDOM;
+  if (init == "learned") init_kind = TG_INIT_LEARNED;
+  else if (init != "random") throw py::value_error("init must be 'random' or 'learned'");
   float* rows = nullptr;
   uint32_t n = 0, d = 0;
   check(tg_read_fvecs(fname.c_str(), &rows, &n, &d));
@@ -95,6 +98,7 @@ py::tuple GNN_descent(unsigned K, unsigned POINTS, unsigned DIM, unsigned iter, 
   p.iters = iter;
   p.seed = seed;
   p.precision = precision;
+  p.init = init_kind;
   double ms = 0.0;
   {
     py::gil_scoped_release nogil;
@@ -204,7 +208,7 @@ PYBIND11_MODULE(Tagore, m) {
   m.def("GNN_descent", &GNN_descent, "GNN_descent", py::arg("K"), py::arg("POINTS"),
         py::arg("DIM"), py::arg("iter"), py::arg("fname"), py::kw_only(),
         py::arg("precision") = int(TG_PREC_BF16), py::arg("seed") = uint64_t(0x7A600000ull),
-        py::arg("device") = 0);
+        py::arg("device") = 0, py::arg("init") = "random");
   m.def("Pruning", &Pruning, "Pruning", py::arg("K"), py::arg("POINTS"), py::arg("DIM"),
         py::arg("FINAL_DEGREE"), py::arg("TOPM") = 64, py::arg("ptrs"), py::arg("index_type"),
         py::arg("thre") = 1.0f, py::arg("index_path") = "index.data");
