@@ -6,12 +6,16 @@ Gaussian mixture, generated on the device from a fixed seed; GNN-Descent K=96; C
 path/dist/1.0 pruning (NSG) with FINAL_DEGREE=32, m=64.  One step = kNN build + prune on
 data already resident in HBM.
 
-Two paths are timed in the same run:
+Three paths are timed in the same run:
   * plain FP32: FP32 MFMA screen + canonical FP32 re-score, random init, 10 iterations
     (the configuration the reference runs: pybind.cu:101 with iter=10);
-  * headline:   BF16 MFMA screen + canonical FP32 re-score (--precision, --iters, --init).
-`value` is the headline's vectors/s over the timed steps; the plain path's figure and the
-ratio are reported beside it.  Both graphs are judged by FP64 recall@10 on a seeded sample.
+  * headline:   learned initial lists + BF16 MFMA screen + canonical FP32 re-score,
+    2 iterations (--init, --precision, --iters); the encoder's sample graph, training,
+    encoding and bucketing are inside the timed step;
+  * ablation:   random init + BF16 screen at the fewest iterations that reach the recall
+    target (--ablation-iters), to separate the two ingredients.
+`value` is the headline's vectors/s over the timed steps; the other paths' figures and the
+ratio are reported beside it.  Every graph is judged by FP64 recall@10 on a seeded sample.
 The roofline entry is the pruning forward kernel (HBM-bound gather), timed live with HIP
 events on the library's stream; the CPU oracle runs the same pipeline on a bounded sample.
 
@@ -33,11 +37,11 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def measure(st, args, precision, iters, label):
+def measure(st, args, precision, iters, init, label):
     import tagore_amd
 
     def step():
-        st.knn_build(args.k, iters, args.seed, precision)
+        st.knn_build(args.k, iters, args.seed, precision, init=init)
         st.prune("path", "dist", "dist", args.fd, args.topm, 1.0,
                  args.prune_rows if precision == "bf16" else "fp32")
 
@@ -65,11 +69,30 @@ def measure(st, args, precision, iters, label):
     rec = st.recall(min(args.recall_queries, st.N), 10, 0x5A11)
     log(f"{label}: {ms:.1f} ms/step, recall@10 = {rec:.4f}")
     return dict(ms_per_step=ms, vectors_per_s=st.N / (ms / 1e3), recall_at_10=rec,
-                precision=precision, iters=iters, init="random",
+                precision=precision, iters=iters, init=init,
+                learn_ms=last["learn_ms"], learn_loss=last["learn_loss"],
                 knn_ms=sum(knn_ms) / len(knn_ms), prune_ms=sum(prune_ms) / len(prune_ms),
                 knn_sections=sec_knn, prune_sections=sec_prune,
                 knn_evals=last["knn_evals"], knn_rescored=last["knn_rescored"],
                 prune_counters=last["prune_counters"])
+
+
+def measured_traffic(N, D, K, FD, TOPM):
+    """HBM bytes per k_forward launch from the committed rocprofv3 counter runs of the same
+    workload (FETCH_SIZE and WRITE_SIZE, each scaled by its copy-kernel calibration), or
+    (None, None) when no run of this workload is recorded."""
+    path = os.path.join(ROOT, "profiles", "k_forward_traffic.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        rec = json.load(f)
+    w = rec.get("workload", {})
+    if (w.get("N"), w.get("dim"), w.get("K"), w.get("final_degree"), w.get("m")) != (
+            N, D, K, FD, TOPM):
+        return None, None
+    # the calibration copy moved `scale` times the bytes the counter reported
+    b = rec["fetch_kib"] * 1024 * rec["fetch_scale"] + rec["write_kib"] * 1024 * rec["write_scale"]
+    return b, "profiles/k_forward_traffic.json"
 
 
 def main():
@@ -80,8 +103,12 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--dim", type=int, default=128)
     ap.add_argument("--k", type=int, default=96)
-    ap.add_argument("--iters", type=int, default=10, help="headline iterations")
+    ap.add_argument("--iters", type=int, default=2, help="headline iterations")
+    ap.add_argument("--init", default="learned", choices=["learned", "random"],
+                    help="headline initial lists")
     ap.add_argument("--plain-iters", type=int, default=10)
+    ap.add_argument("--ablation-iters", type=int, default=6,
+                    help="random-init BF16 iterations of the ablation line (0 skips it)")
     ap.add_argument("--fd", type=int, default=32)
     ap.add_argument("--topm", type=int, default=64)
     ap.add_argument("--ncomp", type=int, default=1024)
@@ -111,10 +138,13 @@ def main():
     st.gen_data(0, args.ncomp, args.seed, 20.0)
     st.set_profiling(True)
 
-    plain = None
+    plain = ablation = None
     if not args.no_plain:
-        plain = measure(st, args, args.plain_precision, args.plain_iters, "plain")
-    head = measure(st, args, args.precision, args.iters, "headline")
+        plain = measure(st, args, args.plain_precision, args.plain_iters, "random", "plain")
+        if args.ablation_iters > 0:
+            ablation = measure(st, args, args.precision, args.ablation_iters, "random",
+                               "ablation")
+    head = measure(st, args, args.precision, args.iters, args.init, "headline")
 
     # roofline of the pruning forward kernel (Collect + Filter): algorithmic bytes = every
     # canonical distance evaluation reads one FP32 row, plus adjacency rows, own lists, output
@@ -123,9 +153,11 @@ def main():
     evals = pc["collect"] + pc["filter"]
     alg_bytes = evals * D * 4 + pc["adj_rows"] * K * 4 + N * K * 8 + N * args.fd * 8
     achieved = alg_bytes / (fwd_ms / 1e3) / 1e9 if fwd_ms > 0 else None
+    traffic, traffic_src = measured_traffic(N, D, K, args.fd, args.topm)
     roof = {"bound": "hbm", "kernel": "k_forward (NSG path collect + filter)",
             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+            "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
+            "traffic_source": traffic_src,
             "algorithmic_bytes": alg_bytes, "kernel_ms": fwd_ms,
             "evaluations": evals}
 
@@ -154,10 +186,13 @@ def main():
         "config": {"workload": f"GNN-Descent K={K} + NSG prune FD={args.fd} m={args.topm} over "
                                f"{N}x{D} integer GMM({args.ncomp}), sigma 20",
                    "N": N, "dim": D, "K": K, "iters": args.iters, "final_degree": args.fd,
-                   "m": args.topm, "precision": args.precision, "init": "random"},
+                   "m": args.topm, "precision": args.precision, "init": args.init},
         "recall_at_10": head["recall_at_10"],
+        "recall_target_met": bool(head["recall_at_10"] >= 0.95
+                                  and (plain is None or plain["recall_at_10"] >= 0.95)),
         "headline": head,
         "plain_fp32": plain,
+        "ablation_random_bf16": ablation,
         "speedup_vs_plain": (plain["ms_per_step"] / head["ms_per_step"]) if plain else None,
         "roofline": roof,
         "cpu_baseline": cpu,
