@@ -110,6 +110,50 @@ int tg_get_forward(tg_state* s, uint32_t* deg, uint32_t* ids, float* dists);
  * (StoreNSG..StoreCAGRA, common.cu:151-248). */
 int tg_store_index(tg_state* s, const char* path, const char* index_type, uint64_t nssg_seed);
 
+/* Segments: the N rows are nseg independent data sets (clusters) stored back to back,
+ * segment c = rows [starts[c], starts[c+1]), starts[0] = 0, starts[nseg] = N.  Builds and
+ * prunes then run every segment in the same launches (gpu_construct's per-cluster loop,
+ * large_index.cu:173-277); each segment's result equals that of a handle holding only its
+ * rows, built with seed ^ ((key_c + 1) * 0xA0761D6478BD642F), with ids shifted by
+ * starts[c]; key_c = keys[c] (keys == NULL: c).  Every segment must hold more than K rows.
+ * nseg = 0 restores one data set.  Clears the current graph.  Not available with learned
+ * init, tg_recall or tg_store_index. */
+int tg_set_segments(tg_state* s, uint32_t nseg, const uint32_t* starts, const uint32_t* keys);
+/* Entry points of the last path prune, one per segment (one entry without segments). */
+int tg_get_entry_points(tg_state* s, uint32_t* eps);
+
+/* largeIndex on one GPU (pybind.cu:297-391, large_index.cu:9-522): clusters from the
+ * two-centre file (u32 first, u32 second per point; equal centres are rejected), oversized
+ * clusters split round-robin, the Algorithm-4 dispatch order, per-cluster GNN-Descent +
+ * Vamana pruning (path/dist, alpha = thre) batched over segmented handles, the streaming
+ * union merge through buffer_size cluster graphs with spill files
+ * (local_index_path_prefix + cluster id: per point u32 degree + ids) and the merged index
+ * (u64 size, u32 max_degree, u32 entry point, u64 0, then per point u32 degree + ids).
+ * Where the reference reads undefined values the restatement takes the smallest unvisited
+ * cluster / slot 0 (reorder) and clamps buffer_size to the cluster count. */
+typedef struct {
+  const int32_t* devicelist;
+  uint32_t ndevices;               /* must be 1 */
+  uint32_t DIM, K, cluster_num, max_points;
+  const char* file_prefix;         /* cluster c: file_prefix + decimal c: u32 count, rows */
+  const char* local_index_path_prefix;
+  const char* index_store_path;
+  uint32_t max_degree, buffer_size, all_node_num;
+  const char* centers_path;
+  float thre;
+  uint32_t FINAL_DEGREE, TOPM;
+  uint32_t iters;                  /* 0: 16 = 8 local-join + 8 expansion (large_index.cu:225) */
+  uint64_t seed;                   /* cluster c is a segment keyed c */
+  int32_t precision;               /* TG_PREC_* of the kNN screen */
+  uint64_t batch_rows;             /* rows per batched build (0: from free device memory) */
+  int32_t verbose;
+} tg_large_params;
+typedef struct {
+  uint32_t clusters, batches;      /* after splitting; batched builds */
+  double build_ms, merge_ms, total_ms;
+} tg_large_stats;
+int tg_large_index(const tg_large_params* p, tg_large_stats* stats);
+
 /* FP64 recall@k of the kNN graph over nq seeded sample nodes (tie at the k-th distance
  * counts as a hit).  Queries: the first nq of a seeded permutation prefix. */
 int tg_recall(tg_state* s, uint32_t nq, uint64_t seed, uint32_t k, double* recall);

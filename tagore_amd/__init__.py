@@ -68,6 +68,25 @@ class Stats(ctypes.Structure):
                 ("learn_buckets", ctypes.c_uint32), ("learn_steps", ctypes.c_uint32)]
 
 
+class LargeParams(ctypes.Structure):
+    _fields_ = [("devicelist", ctypes.POINTER(ctypes.c_int32)), ("ndevices", ctypes.c_uint32),
+                ("DIM", ctypes.c_uint32), ("K", ctypes.c_uint32), ("cluster_num", ctypes.c_uint32),
+                ("max_points", ctypes.c_uint32), ("file_prefix", ctypes.c_char_p),
+                ("local_index_path_prefix", ctypes.c_char_p), ("index_store_path", ctypes.c_char_p),
+                ("max_degree", ctypes.c_uint32), ("buffer_size", ctypes.c_uint32),
+                ("all_node_num", ctypes.c_uint32), ("centers_path", ctypes.c_char_p),
+                ("thre", ctypes.c_float), ("FINAL_DEGREE", ctypes.c_uint32),
+                ("TOPM", ctypes.c_uint32), ("iters", ctypes.c_uint32), ("seed", ctypes.c_uint64),
+                ("precision", ctypes.c_int32), ("batch_rows", ctypes.c_uint64),
+                ("verbose", ctypes.c_int32)]
+
+
+class LargeStats(ctypes.Structure):
+    _fields_ = [("clusters", ctypes.c_uint32), ("batches", ctypes.c_uint32),
+                ("build_ms", ctypes.c_double), ("merge_ms", ctypes.c_double),
+                ("total_ms", ctypes.c_double)]
+
+
 KNN_SECTIONS = ["init", "sample", "rsample", "localjoin", "merge", "reverse", "phase2", "final"]
 PRUNE_SECTIONS = ["ep", "forward", "reverse", "group"]
 
@@ -113,6 +132,9 @@ def lib():
     L.tg_get_stats.argtypes = [vp, P(Stats)]
     L.tg_set_profiling.argtypes = [vp, c.c_int]
     L.tg_get_codes.argtypes = [vp, vp]
+    L.tg_set_segments.argtypes = [vp, c.c_uint32, vp, vp]
+    L.tg_large_index.argtypes = [P(LargeParams), P(LargeStats)]
+    L.tg_get_entry_points.argtypes = [vp, vp]
     L.tg_mlp_eval.argtypes = [c.c_int, c.c_uint32, c.c_uint32, c.c_uint32, vp, vp, c.c_float, vp,
                               c.c_float, P(c.c_float), vp, vp]
     L.tg_mlp_adam.argtypes = [c.c_int, c.c_uint32, c.c_uint32, c.c_float, vp, vp, vp, vp]
@@ -181,6 +203,23 @@ class State:
         p = KnnParams(K, iters, seed, PREC[precision], INIT[init], sample, epochs)
         _check(lib().tg_knn_build(self._h, ctypes.byref(p)))
         self.K = K
+
+    def set_segments(self, starts, keys=None):
+        """Rows [starts[c], starts[c+1]) become independent data sets seeded by keys[c]
+        (default c); None: one data set."""
+        if starts is None:
+            _check(lib().tg_set_segments(self._h, 0, None, None))
+            self.nseg = 0
+            return
+        st = np.ascontiguousarray(starts, np.uint32)
+        ky = None if keys is None else np.ascontiguousarray(keys, np.uint32)
+        _check(lib().tg_set_segments(self._h, st.shape[0] - 1, _ptr(st), _ptr(ky)))
+        self.nseg = st.shape[0] - 1
+
+    def entry_points(self):
+        out = np.empty(max(1, getattr(self, "nseg", 0)), np.uint32)
+        _check(lib().tg_get_entry_points(self._h, _ptr(out)))
+        return out
 
     def codes(self):
         out = np.empty((self.N, MLP_CODE), np.float32)
@@ -252,6 +291,27 @@ class State:
         p, n = ctypes.c_void_p(), ctypes.c_size_t()
         _check(lib().tg_buffer(self._h, idx, ctypes.byref(p), ctypes.byref(n)))
         return p.value, n.value
+
+
+def large_index(DIM, K, cluster_num, max_points, file_prefix, local_index_path_prefix,
+                index_store_path, max_degree, buffer_size, all_node_num, centers_path, thre,
+                FINAL_DEGREE, TOPM, device=0, seed=0x7A600000, precision="bf16", batch_rows=0,
+                iters=16, verbose=False):
+    """tg_large_index through ctypes; returns the stats as a dict."""
+    dev = (ctypes.c_int32 * 1)(device)
+    p = LargeParams(dev, 1, DIM, K, cluster_num, max_points, file_prefix.encode(),
+                    local_index_path_prefix.encode(), index_store_path.encode(), max_degree,
+                    buffer_size, all_node_num, centers_path.encode(), thre, FINAL_DEGREE, TOPM,
+                    iters, seed, PREC[precision], batch_rows, 1 if verbose else 0)
+    st = LargeStats()
+    _check(lib().tg_large_index(ctypes.byref(p), ctypes.byref(st)))
+    return dict(clusters=st.clusters, batches=st.batches, build_ms=st.build_ms,
+                merge_ms=st.merge_ms, total_ms=st.total_ms)
+
+
+def segment_seed(seed: int, c: int) -> int:
+    """Seed of segment c of a segmented handle."""
+    return (seed ^ ((c + 1) * 0xA0761D6478BD642F)) & 0xFFFFFFFFFFFFFFFF
 
 
 def mlp_nparams(D: int) -> int:

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/tagore.h"
+#include "errors.hpp"
 #include "kernels.hpp"
 #include "learn.hpp"
 
@@ -29,6 +30,12 @@ int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
 }
+
+}  // namespace
+
+int tg::set_error(int code, const std::string& msg) { return fail(code, msg); }
+
+namespace {
 
 #define HIPX(expr)                                                                   \
   do {                                                                               \
@@ -121,6 +128,16 @@ struct tg_state {
   bool has_prune = false;
   tg_prune_params last_prune{};
   uint32_t ep = 0;
+  // segments (clusters stored back to back; nseg == 0: one data set)
+  uint32_t nseg = 0;
+  std::vector<uint32_t> seg_lo_host;
+  uint32_t* seg_of = nullptr;   // [N]
+  uint32_t* seg_lo = nullptr;   // [nseg + 1]
+  uint32_t* seg_ep = nullptr;   // [nseg]
+  uint32_t* seg_key = nullptr;  // [nseg] seed keys
+  float* seg_cent = nullptr;    // [nseg * D]
+  uint32_t seg_cap = 0;
+  std::vector<uint32_t> seg_ep_host;
   // learned initial lists
   Learner learn;
   tg_state* sub = nullptr;     // the sample's own handle (its kNN graph trains the encoder)
@@ -362,6 +379,11 @@ int tg_free(tg_state* s) {
   s->sub = nullptr;
   learner_release(s->learn);
   dfree(s->samp);
+  dfree(s->seg_of);
+  dfree(s->seg_lo);
+  dfree(s->seg_ep);
+  dfree(s->seg_key);
+  dfree(s->seg_cent);
   dfree(s->X);
   dfree(s->centres);
   dfree(s->misc);
@@ -656,6 +678,13 @@ int tg_knn_build(tg_state* s, const tg_knn_params* p) {
     return fail(TG_ERR_ARG, "unknown init");
   if (p->init == TG_INIT_GIVEN && (!s->has_knn || s->K != K))
     return fail(TG_ERR_STATE, "init=given needs a current graph with the same K");
+  if (s->nseg) {
+    for (uint32_t c = 0; c < s->nseg; ++c)
+      if (s->seg_lo_host[c + 1] - s->seg_lo_host[c] <= K)
+        return fail(TG_ERR_ARG, "every segment must hold more than K rows");
+    if (p->init == TG_INIT_LEARNED)
+      return fail(TG_ERR_ARG, "learned init is not available on a segmented handle");
+  }
   if (p->init == TG_INIT_LEARNED) {
     if (s->D % 32 != 0) return fail(TG_ERR_ARG, "learned init needs D % 32 == 0");
     if (std::min<uint32_t>(p->sample ? p->sample : 50000u, s->N) < 64)
@@ -681,6 +710,11 @@ int tg_knn_build(tg_state* s, const tg_knn_params* p) {
   const uint16_t* rows16 = (p->precision == TG_PREC_BF16 && s->cert_bf16) ? s->Xb : nullptr;
   auto args = [&]() {
     KnnArgs k = knn_args(s, p->seed);
+    if (s->nseg) {
+      k.seg_of = s->seg_of;
+      k.seg_lo = s->seg_lo;
+      k.seg_key = s->seg_key;
+    }
     k.exact_screen = exact_screen ? 1 : 0;
     k.Xr16 = rows16;
     return k;
@@ -843,6 +877,11 @@ int tg_prune(tg_state* s, const tg_prune_params* p) {
   a.fwd_rule = p->fwd_rule;
   a.rev_rule = p->rev_rule;
   a.ep = s->ep_dev;
+  if (s->nseg) {
+    a.ep = s->seg_ep;
+    a.seg_of = s->seg_of;
+    a.seg_lo = s->seg_lo;
+  }
   a.fwd_deg = s->fwd_deg;
   a.fwd_ids = s->fwd_ids;
   a.fwd_d = s->fwd_d;
@@ -870,8 +909,17 @@ int tg_prune(tg_state* s, const tg_prune_params* p) {
   pb_reset(a.rev, N, s->fill, st);
   if (p->kind == TG_KIND_PATH) {
     if (sec.start(0)) return TG_ERR_HIP;
-    launch_centroid(s->X, N, s->D, s->cpart, s->centroid, st);
-    launch_prune(a, PH_EP, s->centroid, s->ep_dev, st);
+    if (s->nseg) {
+      for (uint32_t c = 0; c < s->nseg; ++c) {
+        const uint32_t lo = s->seg_lo_host[c], n = s->seg_lo_host[c + 1] - lo;
+        launch_centroid(s->X + size_t(lo) * s->D, n, s->D, s->cpart,
+                        s->seg_cent + size_t(c) * s->D, st);
+      }
+      launch_prune(a, PH_EP, s->seg_cent, s->seg_ep, st, s->nseg);
+    } else {
+      launch_centroid(s->X, N, s->D, s->cpart, s->centroid, st);
+      launch_prune(a, PH_EP, s->centroid, s->ep_dev, st);
+    }
     if (sec.stop()) return TG_ERR_HIP;
   }
   if (sec.start(1)) return TG_ERR_HIP;
@@ -901,7 +949,16 @@ int tg_prune(tg_state* s, const tg_prune_params* p) {
   unsigned long long cnt[8];
   HIPX(hipMemcpy(cnt, s->counters, sizeof(cnt), hipMemcpyDeviceToHost));
   for (int i = 0; i < 8; ++i) s->stats.prune_counters[i] = cnt[i];
-  HIPX(hipMemcpy(&s->ep, s->ep_dev, 4, hipMemcpyDeviceToHost));
+  if (s->nseg) {
+    s->seg_ep_host.resize(s->nseg);
+    if (p->kind == TG_KIND_PATH)
+      HIPX(hipMemcpy(s->seg_ep_host.data(), s->seg_ep, size_t(s->nseg) * 4, hipMemcpyDeviceToHost));
+    else
+      std::fill(s->seg_ep_host.begin(), s->seg_ep_host.end(), 0u);
+    s->ep = s->seg_ep_host[0];
+  } else {
+    HIPX(hipMemcpy(&s->ep, s->ep_dev, 4, hipMemcpyDeviceToHost));
+  }
   if (check_pb_err(s, s->rev, "reverse edges")) return TG_ERR_CAPACITY;
   s->has_prune = true;
   s->last_prune = *p;
@@ -931,7 +988,57 @@ int tg_get_forward(tg_state* s, uint32_t* deg, uint32_t* ids, float* dists) {
 }
 
 // ------------------------------------------------------------------------ writers
+int tg_set_segments(tg_state* s, uint32_t nseg, const uint32_t* starts, const uint32_t* keys) {
+  if (!s) return fail(TG_ERR_ARG, "null handle");
+  HIPX(hipSetDevice(s->device));
+  if (nseg == 0) {
+    s->nseg = 0;
+    s->seg_lo_host.clear();
+  } else {
+    if (!starts) return fail(TG_ERR_ARG, "null starts");
+    if (starts[0] != 0 || starts[nseg] != s->N)
+      return fail(TG_ERR_ARG, "segment starts must run from 0 to N");
+    for (uint32_t c = 0; c < nseg; ++c)
+      if (starts[c + 1] <= starts[c]) return fail(TG_ERR_ARG, "segments must be non-empty");
+    if (!s->seg_of && dalloc(&s->seg_of, s->N)) return TG_ERR_HIP;
+    if (s->seg_cap < nseg) {
+      dfree(s->seg_lo);
+      dfree(s->seg_ep);
+      dfree(s->seg_key);
+      dfree(s->seg_cent);
+      if (dalloc(&s->seg_lo, size_t(nseg) + 1) || dalloc(&s->seg_ep, nseg) ||
+          dalloc(&s->seg_key, nseg) || dalloc(&s->seg_cent, size_t(nseg) * s->D))
+        return TG_ERR_HIP;
+      s->seg_cap = nseg;
+    }
+    std::vector<uint32_t> key(nseg);
+    for (uint32_t c = 0; c < nseg; ++c) key[c] = keys ? keys[c] : c;
+    HIPX(hipMemcpy(s->seg_key, key.data(), size_t(nseg) * 4, hipMemcpyHostToDevice));
+    std::vector<uint32_t> of(s->N);
+    for (uint32_t c = 0; c < nseg; ++c)
+      std::fill(of.begin() + starts[c], of.begin() + starts[c + 1], c);
+    HIPX(hipMemcpy(s->seg_of, of.data(), size_t(s->N) * 4, hipMemcpyHostToDevice));
+    HIPX(hipMemcpy(s->seg_lo, starts, (size_t(nseg) + 1) * 4, hipMemcpyHostToDevice));
+    s->nseg = nseg;
+    s->seg_lo_host.assign(starts, starts + nseg + 1);
+  }
+  s->has_knn = false;
+  s->has_prune = false;
+  return TG_OK;
+}
+
+int tg_get_entry_points(tg_state* s, uint32_t* eps) {
+  if (!s || !eps) return fail(TG_ERR_ARG, "null argument");
+  if (!s->has_prune) return fail(TG_ERR_STATE, "no pruned graph");
+  if (s->nseg)
+    std::copy(s->seg_ep_host.begin(), s->seg_ep_host.end(), eps);
+  else
+    eps[0] = s->ep;
+  return TG_OK;
+}
+
 int tg_store_index(tg_state* s, const char* path, const char* index_type, uint64_t nssg_seed) {
+  if (s && s->nseg) return fail(TG_ERR_STATE, "index files hold one graph: clear the segments");
   if (!s || !path || !index_type) return fail(TG_ERR_ARG, "null argument");
   if (!s->has_prune) return fail(TG_ERR_STATE, "no pruned graph to store");
   const std::string t(index_type);
@@ -1015,6 +1122,7 @@ int tg_store_index(tg_state* s, const char* path, const char* index_type, uint64
 
 // ------------------------------------------------------------------------ recall
 int tg_recall(tg_state* s, uint32_t nq, uint64_t seed, uint32_t k, double* recall) {
+  if (s && s->nseg) return fail(TG_ERR_STATE, "recall is judged on a handle without segments");
   if (!s || !recall) return fail(TG_ERR_ARG, "null argument");
   if (!s->has_knn) return fail(TG_ERR_STATE, "no kNN graph");
   if (k < 1 || k > 16 || k > s->K) return fail(TG_ERR_ARG, "k must be in [1, min(16, K)]");

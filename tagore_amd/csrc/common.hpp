@@ -69,6 +69,11 @@ __device__ __forceinline__ uint32_t sm_bounded(uint64_t x, uint32_t n) {
   return uint32_t(((x >> 32) * uint64_t(n)) >> 32);
 }
 
+// seed of segment c of a segmented handle (host and device)
+__host__ __device__ inline uint64_t segment_seed(uint64_t seed, uint32_t c) {
+  return seed ^ ((uint64_t(c) + 1) * 0xA0761D6478BD642Full);
+}
+
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 

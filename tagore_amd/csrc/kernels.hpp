@@ -26,7 +26,7 @@ struct PruneArgs {
   uint32_t FD, TOPM;
   float th;
   int fwd_rule, rev_rule;  // 0 dist, 1 angle
-  const uint32_t* ep;      // device scalar
+  const uint32_t* ep;      // entry point of each segment (one entry without segments)
   uint32_t* fwd_deg;       // [N]
   uint32_t* fwd_ids;       // [N*FD]
   float* fwd_d;            // [N*FD]
@@ -38,10 +38,15 @@ struct PruneArgs {
   // data set is exactly representable in BF16 (identical FP32 values, half the bytes)
   const void* rows;
   int rows_bf16;
+  // segments: rows [seg_lo[c], seg_lo[c+1]) form independent data sets (null: one segment)
+  const uint32_t* seg_of;  // [N] segment of each row
+  const uint32_t* seg_lo;  // [nseg + 1]
 };
 
+// PH_EP runs one workgroup per segment: segment c starts its beam at its first row and
+// aims at centroid + c * D, writing ep[c]
 void launch_prune(const PruneArgs& a, int phase, const float* centroid, uint32_t* ep,
-                  hipStream_t s);
+                  hipStream_t s, uint32_t nseg = 1);
 void launch_centroid(const float* X, uint32_t N, uint32_t D, double* part, float* c,
                      hipStream_t s);
 
@@ -82,6 +87,11 @@ struct KnnArgs {
   int exact_screen;
   // rows used for canonical re-scores: BF16 copy when the data set is BF16-exact, else X
   const uint16_t* Xr16;
+  // segments (see PruneArgs): the initial lists draw ids inside the row's own segment, from a
+  // stream of (segment seed, row index within the segment); every later step follows edges
+  const uint32_t* seg_of;
+  const uint32_t* seg_lo;
+  const uint32_t* seg_key;  // [nseg] seed key of each segment
 };
 
 // screen-mode preparation: BF16 copy (if Xb), norms, quantisation-error norms and the

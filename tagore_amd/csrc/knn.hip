@@ -33,11 +33,21 @@ __global__ __launch_bounds__(64) void k_init_ids(KnnArgs a) {
   const uint32_t u = blockIdx.x;
   if (u >= a.N) return;
   const int lane = threadIdx.x, K = a.K;
-  const uint64_t st0 = sm_stream(a.seed, u);
+  // a segment draws inside [lo, lo + n) from the stream of its own seed and local row index,
+  // exactly as a handle holding only that segment would (ids shifted by lo)
+  uint32_t lo = 0, n = a.N;
+  uint64_t seed = a.seed;
+  if (a.seg_of) {
+    const uint32_t c = a.seg_of[u];
+    lo = a.seg_lo[c];
+    n = a.seg_lo[c + 1] - lo;
+    seed = segment_seed(a.seed, a.seg_key[c]);
+  }
+  const uint64_t st0 = sm_stream(seed, u - lo);
   int cnt = 0;
   for (uint64_t base = 0; cnt < K; base += TG_WAVE) {
     uint64_t s = st0 + (base + lane) * 0x9E3779B97F4A7C15ull;
-    const uint32_t c = sm_bounded(sm_next(s), a.N);
+    const uint32_t c = lo + sm_bounded(sm_next(s), n);
     bool ok = c != u;
     for (int j = 0; j < cnt && ok; ++j) ok = acc[j] != c;
     for (int j = 0; j < TG_WAVE; ++j) {

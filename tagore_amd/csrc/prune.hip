@@ -281,7 +281,8 @@ __global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
   uint32_t n_col = 0, n_adj = 0, n_flt = 0;
   if (KIND == 0) {
     uint32_t ring_c = 0;
-    beam_search<MC, T>(a, xu, *a.ep, 10, M0, M1, S, tab, expl, cl, P + K, ring_c, n_col, n_adj);
+    const uint32_t ep = a.ep[a.seg_of ? a.seg_of[u] : 0];
+    beam_search<MC, T>(a, xu, ep, 10, M0, M1, S, tab, expl, cl, P + K, ring_c, n_col, n_adj);
   } else {
     // two-hop ids y_i = G[G[u][i/K]][i%K], i < 480
     for (int i = lane; i < RING; i += TG_WAVE) {
@@ -497,7 +498,8 @@ __global__ __launch_bounds__(64) void k_rank_reverse(PruneArgs a) {
 }
 
 // -------------------------------------------------------------------------------------
-// Entry point: one workgroup, beam of TOPM from node 0's list toward the centroid.
+// Entry point (cal_ep_gpu, Tagore_src.cu:1889-2058): beam of TOPM from the first node's list
+// toward the centroid; one workgroup per segment.
 // -------------------------------------------------------------------------------------
 template <int MC, class T>
 __global__ __launch_bounds__(64) void k_cal_ep(PruneArgs a, const float* centroid, uint32_t* ep) {
@@ -505,11 +507,13 @@ __global__ __launch_bounds__(64) void k_cal_ep(PruneArgs a, const float* centroi
   __shared__ uint32_t tab[HSET];
   __shared__ uint32_t expl[EXPL_MAX];
   __shared__ uint16_t cl[128];
+  const uint32_t seg = blockIdx.x;
+  const uint32_t start = a.seg_lo ? a.seg_lo[seg] : 0u;
   Row16<MC> c;
-  load_row16<MC>(c, centroid, a.D, threadIdx.x & 15);
+  load_row16<MC>(c, centroid + size_t(seg) * a.D, a.D, threadIdx.x & 15);
   uint32_t rc = 0, ne = 0, na = 0;
-  const uint64_t* B = beam_search<MC, T>(a, c, 0u, 30, M0, M1, S, tab, expl, cl, nullptr, rc, ne, na);
-  if (threadIdx.x == 0) *ep = key_id(B[0]);
+  const uint64_t* B = beam_search<MC, T>(a, c, start, 30, M0, M1, S, tab, expl, cl, nullptr, rc, ne, na);
+  if (threadIdx.x == 0) ep[seg] = key_id(B[0]);
   count_add(a.counters, 4, ne);
 }
 
@@ -534,10 +538,10 @@ __global__ void k_centroid_final(const double* part, uint32_t nch, uint32_t N, u
 // ------------------------------------------------------------------------- launchers
 template <int MC, class T>
 static void launch_prune_mc(const PruneArgs& a, int phase, const float* centroid, uint32_t* ep,
-                            hipStream_t s) {
+                            hipStream_t s, uint32_t nseg) {
   const dim3 grid(a.N), block(64);
   switch (phase) {
-    case PH_EP: k_cal_ep<MC, T><<<1, 64, 0, s>>>(a, centroid, ep); break;
+    case PH_EP: k_cal_ep<MC, T><<<nseg, 64, 0, s>>>(a, centroid, ep); break;
     case PH_FWD_PATH: k_forward<MC, T, 0><<<grid, block, 0, s>>>(a); break;
     case PH_FWD_2HOP: k_forward<MC, T, 1><<<grid, block, 0, s>>>(a); break;
     case PH_REV: k_reverse<MC, T><<<grid, block, 0, s>>>(a); break;
@@ -550,17 +554,17 @@ static void launch_prune_mc(const PruneArgs& a, int phase, const float* centroid
 
 template <class T>
 static void launch_prune_t(const PruneArgs& a, int phase, const float* centroid, uint32_t* ep,
-                           hipStream_t s) {
+                           hipStream_t s, uint32_t nseg) {
   const int mc = (a.D + 63) / 64;
-  if (mc <= 2) launch_prune_mc<2, T>(a, phase, centroid, ep, s);
-  else if (mc <= 4) launch_prune_mc<4, T>(a, phase, centroid, ep, s);
-  else launch_prune_mc<16, T>(a, phase, centroid, ep, s);
+  if (mc <= 2) launch_prune_mc<2, T>(a, phase, centroid, ep, s, nseg);
+  else if (mc <= 4) launch_prune_mc<4, T>(a, phase, centroid, ep, s, nseg);
+  else launch_prune_mc<16, T>(a, phase, centroid, ep, s, nseg);
 }
 
 void launch_prune(const PruneArgs& a, int phase, const float* centroid, uint32_t* ep,
-                  hipStream_t s) {
-  if (a.rows_bf16) launch_prune_t<uint16_t>(a, phase, centroid, ep, s);
-  else launch_prune_t<float>(a, phase, centroid, ep, s);
+                  hipStream_t s, uint32_t nseg) {
+  if (a.rows_bf16) launch_prune_t<uint16_t>(a, phase, centroid, ep, s, nseg);
+  else launch_prune_t<float>(a, phase, centroid, ep, s, nseg);
 }
 
 void launch_centroid(const float* X, uint32_t N, uint32_t D, double* part, float* c,

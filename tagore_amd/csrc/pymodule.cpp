@@ -1,7 +1,7 @@
 // pymodule.cpp -- the drop-in Python module `Tagore` (pybind11) over the C ABI.
 //
 // Mirrors the reference module (pybind.cu:393-399): GNN_descent, Pruning, Pruning_with_CFS
-// and largeIndex with the same names, argument order, keyword names and defaults.
+// and largeIndex (one-GPU devicelist) with the same names, argument order and defaults.
 // Differences, all deliberate:
 //   * failures raise Python exceptions instead of exit() (Tagore_src.cuh:51-60);
 //   * unknown index types / modes / metrics and layout-mixing CFS combinations raise
@@ -201,6 +201,45 @@ void Pruning_with_CFS(unsigned K, unsigned POINTS, unsigned DIM, unsigned FINAL_
   run_prune(h->s, p, nullptr, "");
 }
 
+// largeIndex (pybind.cu:297-391) on a one-GPU devicelist
+void largeIndex(std::vector<int> devicelist, unsigned DIM, unsigned K, unsigned cluster_num,
+                unsigned max_points, std::string file_prefix, std::string local_index_path_prefix,
+                std::string index_store_path, unsigned max_degree, unsigned buffer_size,
+                unsigned all_node_num, std::string centers_path, float thre,
+                unsigned FINAL_DEGREE, unsigned TOPM, uint64_t seed, int precision,
+                uint64_t batch_rows, unsigned iters) {
+  std::vector<int32_t> dev(devicelist.begin(), devicelist.end());
+  tg_large_params p{};
+  p.devicelist = dev.data();
+  p.ndevices = uint32_t(dev.size());
+  p.DIM = DIM;
+  p.K = K;
+  p.cluster_num = cluster_num;
+  p.max_points = max_points;
+  p.file_prefix = file_prefix.c_str();
+  p.local_index_path_prefix = local_index_path_prefix.c_str();
+  p.index_store_path = index_store_path.c_str();
+  p.max_degree = max_degree;
+  p.buffer_size = buffer_size;
+  p.all_node_num = all_node_num;
+  p.centers_path = centers_path.c_str();
+  p.thre = thre;
+  p.FINAL_DEGREE = FINAL_DEGREE;
+  p.TOPM = TOPM;
+  p.iters = iters;
+  p.seed = seed;
+  p.precision = precision;
+  p.batch_rows = batch_rows;
+  p.verbose = 1;
+  tg_large_stats st{};
+  {
+    py::gil_scoped_release nogil;
+    check(tg_large_index(&p, &st));
+  }
+  std::cout << "Clusters: " << st.clusters << " in " << st.batches << " batched builds; build "
+            << st.build_ms / 1000.0 << "s, merge " << st.merge_ms / 1000.0 << "s" << std::endl;
+}
+
 }  // namespace
 
 PYBIND11_MODULE(Tagore, m) {
@@ -213,4 +252,11 @@ PYBIND11_MODULE(Tagore, m) {
         py::arg("FINAL_DEGREE"), py::arg("TOPM") = 64, py::arg("ptrs"), py::arg("index_type"),
         py::arg("thre") = 1.0f, py::arg("index_path") = "index.data");
   m.def("Pruning_with_CFS", &Pruning_with_CFS, "Pruning_with_CFS");
+  m.def("largeIndex", &largeIndex, "largeIndex", py::arg("devicelist"), py::arg("DIM"),
+        py::arg("K"), py::arg("cluster_num"), py::arg("max_points"), py::arg("file_prefix"),
+        py::arg("local_index_path_prefix"), py::arg("index_store_path"), py::arg("max_degree"),
+        py::arg("buffer_size"), py::arg("all_node_num"), py::arg("centers_path"),
+        py::arg("thre"), py::arg("FINAL_DEGREE"), py::arg("TOPM"), py::kw_only(),
+        py::arg("seed") = uint64_t(0x7A600000ull), py::arg("precision") = int(TG_PREC_BF16),
+        py::arg("batch_rows") = uint64_t(0), py::arg("iters") = 16u);
 }
