@@ -95,6 +95,72 @@ def measured_traffic(N, D, K, FD, TOPM):
     return b, "profiles/k_forward_traffic.json"
 
 
+def bench_clusters(args):
+    """One GPU's share of the clustered pipeline: uneven clusters (log-normal sizes) of unit
+    Gaussian-mixture vectors, each built (GNN-Descent 8+8 iterations, K=64) and pruned
+    (Vamana alpha 1.2, FD=32, m=64).  Batched: every cluster in the same launches
+    (segmented handle, autotuned kernel variant).  Per cluster: one handle and one launch
+    sequence per cluster, the reference's loop (large_index.cu:173-277).  Times are device
+    windows of the builds and prunes; the two produce identical graphs."""
+    import numpy as np
+
+    import tagore_amd
+
+    rng = np.random.default_rng(0x7A600004)
+    sizes = np.exp(rng.normal(np.log(args.cluster_median), 0.95, args.clusters))
+    sizes = np.clip(sizes, args.cluster_median / 8, args.cluster_median * 5).astype(np.int64)
+    starts = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint32)
+    N, D, K = int(starts[-1]), 96, 64
+    log(f"clusters: {args.clusters}, {N} vectors, sizes {sizes.min()}..{sizes.max()}")
+    st = tagore_amd.State(N, D, 0)
+    st.gen_data(1, 512, 0x7A600004, 1.0)
+    st.set_segments(starts)
+    t = time.perf_counter()
+    (prec, tile), cands = st.autotune(K, 0x51)
+    tune_s = time.perf_counter() - t
+    log(f"autotune ({tune_s:.2f} s): {cands} -> {prec} tile {tile}")
+
+    def batched():
+        st.knn_build(K, 16, 0x51, prec, tile=tile)
+        a = st.stats()["knn_ms"]
+        st.prune("path", "dist", "dist", 32, 64, 1.2)
+        return a + st.stats()["prune_ms"]
+
+    batched()
+    t_b = [batched() for _ in range(args.steps)]
+    X = None
+    t_c = []
+    for c in range(min(args.clusters, args.per_cluster_limit)):
+        lo, hi = int(starts[c]), int(starts[c + 1])
+        rows = st.get_data(lo, hi - lo)
+        with tagore_amd.State(hi - lo, D, 0) as sc:
+            sc.set_data(rows)
+            sc.set_segments(np.array([0, hi - lo], np.uint32), keys=[c])
+            sc.knn_build(K, 16, 0x51, prec, tile=tile)
+            ms = sc.stats()["knn_ms"]
+            sc.prune("path", "dist", "dist", 32, 64, 1.2)
+            t_c.append(ms + sc.stats()["prune_ms"])
+    st.close()
+    done = min(args.clusters, args.per_cluster_limit)
+    per_cluster_ms = sum(t_c) * (N / float(starts[done]))  # scaled to all clusters
+    batched_ms = sum(t_b) / len(t_b)
+    out = {
+        "metric": "clustered kNN+prune vectors/s, batched launches vs one launch per cluster",
+        "value": N / (batched_ms / 1e3), "unit": "vectors/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": 1, "ms_per_step": batched_ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": prec, "data": "synthetic",
+        "config": {"workload": f"{args.clusters} log-normal clusters (median "
+                               f"{args.cluster_median}), {N}x{D} unit GMM(512); GNN-Descent K=64 "
+                               "8+8 iterations + Vamana alpha 1.2 FD=32 m=64",
+                   "clusters": args.clusters, "N": N, "dim": D, "K": K},
+        "per_cluster_ms": per_cluster_ms,
+        "per_cluster_measured": done,
+        "speedup_vs_per_cluster": per_cluster_ms / batched_ms,
+        "autotune": {"choice": [prec, tile], "candidates": cands, "seconds": tune_s},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +187,12 @@ def main():
     ap.add_argument("--recall-queries", type=int, default=10000)
     ap.add_argument("--cpu-sample", type=int, default=100000,
                     help="vectors in the CPU-oracle baseline sample (0 skips it)")
+    ap.add_argument("--workload", default="headline", choices=["headline", "clusters"],
+                    help="clusters: the batched multi-cluster build instead of the headline")
+    ap.add_argument("--clusters", type=int, default=64)
+    ap.add_argument("--cluster-median", type=int, default=40000)
+    ap.add_argument("--per-cluster-limit", type=int, default=64,
+                    help="clusters timed one by one (the rest is scaled by row count)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -129,6 +201,10 @@ def main():
               "(the design notes describe how clusters would shard across GPUs).",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "clusters":
+        bench_clusters(args)
+        return
 
     import tagore_amd
 

@@ -35,6 +35,7 @@ typedef struct tg_state tg_state;
 #define TG_PREC_EXACT 0
 #define TG_PREC_FP32 1
 #define TG_PREC_BF16 2
+#define TG_PREC_FP8 3   /* e4m3 with a per-vector scale (D % 16 == 0) */
 
 /* pruning index kinds (Pruning: NSG/Vamana -> PATH, NSSG -> TWOHOP, CAGRA, DPG) */
 #define TG_KIND_PATH 0
@@ -60,6 +61,8 @@ typedef struct {
   int32_t init;       /* TG_INIT_* */
   uint32_t sample;    /* learned init: sample size (0: min(N, 50000)) */
   uint32_t epochs;    /* learned init: training epochs (0: 5) */
+  uint32_t tile;      /* local-join MFMA staging depth: 0 = 256-byte rows per LDS round,
+                         1 = 64-byte rows (results are identical; see tg_autotune_knn) */
 } tg_knn_params;
 
 typedef struct {
@@ -119,6 +122,20 @@ int tg_store_index(tg_state* s, const char* path, const char* index_type, uint64
  * nseg = 0 restores one data set.  Clears the current graph.  Not available with learned
  * init, tg_recall or tg_store_index. */
 int tg_set_segments(tg_state* s, uint32_t nseg, const uint32_t* starts, const uint32_t* keys);
+/* Autotuner of the kNN kernels: times a 2-iteration build (one local join, one expansion)
+ * of the current data for each (precision, tile) candidate -- exact, FP32-MFMA tiles 0/1,
+ * BF16-MFMA tiles 0/1 (BF16 when D % 8 == 0) -- and returns the fastest.  Every candidate
+ * gives the same graph, so the choice only affects time.  Leaves a 2-iteration graph. */
+typedef struct {
+  int32_t precision;          /* fastest candidate */
+  uint32_t tile;
+  uint32_t ncand;
+  int32_t cand_precision[8];
+  uint32_t cand_tile[8];
+  double cand_ms[8];          /* device window of each candidate's build */
+} tg_tune_result;
+int tg_autotune_knn(tg_state* s, uint32_t K, uint64_t seed, tg_tune_result* out);
+
 /* Entry points of the last path prune, one per segment (one entry without segments). */
 int tg_get_entry_points(tg_state* s, uint32_t* eps);
 
@@ -147,10 +164,14 @@ typedef struct {
   int32_t precision;               /* TG_PREC_* of the kNN screen */
   uint64_t batch_rows;             /* rows per batched build (0: from free device memory) */
   int32_t verbose;
+  int32_t tune;                    /* 1: tg_autotune_knn on the first batch picks precision
+                                      and tile for every batch (same graphs, less time) */
 } tg_large_params;
 typedef struct {
   uint32_t clusters, batches;      /* after splitting; batched builds */
   double build_ms, merge_ms, total_ms;
+  int32_t precision;               /* precision and tile the builds used */
+  uint32_t tile;
 } tg_large_stats;
 int tg_large_index(const tg_large_params* p, tg_large_stats* stats);
 

@@ -22,7 +22,7 @@ LIB_PATH = os.path.join(_HERE, "libtagore.so")
 INCLUDE = os.path.join(os.path.dirname(_HERE), "include", "tagore.h")
 
 TG_OK, TG_ERR_ARG, TG_ERR_IO, TG_ERR_HIP, TG_ERR_STATE, TG_ERR_CAPACITY = range(6)
-PREC = {"exact": 0, "fp32": 1, "bf16": 2}
+PREC = {"exact": 0, "fp32": 1, "bf16": 2, "fp8": 3}
 KIND = {"path": 0, "2-hop": 1, "cagra": 2, "dpg": 3}
 RULE = {"dist": 0, "angle": 1, "rank": 2}
 
@@ -49,7 +49,8 @@ MLP_HID, MLP_CODE = 256, 32
 class KnnParams(ctypes.Structure):
     _fields_ = [("K", ctypes.c_uint32), ("iters", ctypes.c_uint32), ("seed", ctypes.c_uint64),
                 ("precision", ctypes.c_int32), ("init", ctypes.c_int32),
-                ("sample", ctypes.c_uint32), ("epochs", ctypes.c_uint32)]
+                ("sample", ctypes.c_uint32), ("epochs", ctypes.c_uint32),
+                ("tile", ctypes.c_uint32)]
 
 
 class PruneParams(ctypes.Structure):
@@ -78,13 +79,20 @@ class LargeParams(ctypes.Structure):
                 ("thre", ctypes.c_float), ("FINAL_DEGREE", ctypes.c_uint32),
                 ("TOPM", ctypes.c_uint32), ("iters", ctypes.c_uint32), ("seed", ctypes.c_uint64),
                 ("precision", ctypes.c_int32), ("batch_rows", ctypes.c_uint64),
-                ("verbose", ctypes.c_int32)]
+                ("verbose", ctypes.c_int32), ("tune", ctypes.c_int32)]
 
 
 class LargeStats(ctypes.Structure):
     _fields_ = [("clusters", ctypes.c_uint32), ("batches", ctypes.c_uint32),
                 ("build_ms", ctypes.c_double), ("merge_ms", ctypes.c_double),
-                ("total_ms", ctypes.c_double)]
+                ("total_ms", ctypes.c_double), ("precision", ctypes.c_int32),
+                ("tile", ctypes.c_uint32)]
+
+
+class TuneResult(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int32), ("tile", ctypes.c_uint32),
+                ("ncand", ctypes.c_uint32), ("cand_precision", ctypes.c_int32 * 8),
+                ("cand_tile", ctypes.c_uint32 * 8), ("cand_ms", ctypes.c_double * 8)]
 
 
 KNN_SECTIONS = ["init", "sample", "rsample", "localjoin", "merge", "reverse", "phase2", "final"]
@@ -134,6 +142,7 @@ def lib():
     L.tg_get_codes.argtypes = [vp, vp]
     L.tg_set_segments.argtypes = [vp, c.c_uint32, vp, vp]
     L.tg_large_index.argtypes = [P(LargeParams), P(LargeStats)]
+    L.tg_autotune_knn.argtypes = [vp, c.c_uint32, c.c_uint64, P(TuneResult)]
     L.tg_get_entry_points.argtypes = [vp, vp]
     L.tg_mlp_eval.argtypes = [c.c_int, c.c_uint32, c.c_uint32, c.c_uint32, vp, vp, c.c_float, vp,
                               c.c_float, P(c.c_float), vp, vp]
@@ -199,8 +208,8 @@ class State:
 
     # kNN
     def knn_build(self, K: int, iters: int, seed: int = 0x7A600000, precision: str = "exact",
-                  init: str = "random", sample: int = 0, epochs: int = 0):
-        p = KnnParams(K, iters, seed, PREC[precision], INIT[init], sample, epochs)
+                  init: str = "random", sample: int = 0, epochs: int = 0, tile: int = 0):
+        p = KnnParams(K, iters, seed, PREC[precision], INIT[init], sample, epochs, tile)
         _check(lib().tg_knn_build(self._h, ctypes.byref(p)))
         self.K = K
 
@@ -220,6 +229,15 @@ class State:
         out = np.empty(max(1, getattr(self, "nseg", 0)), np.uint32)
         _check(lib().tg_get_entry_points(self._h, _ptr(out)))
         return out
+
+    def autotune(self, K: int, seed: int = 0x7A600000):
+        """Fastest (precision, tile) for a kNN build of this data, and every candidate's ms."""
+        r = TuneResult()
+        _check(lib().tg_autotune_knn(self._h, K, seed, ctypes.byref(r)))
+        inv = {v: k for k, v in PREC.items()}
+        cands = [(inv[r.cand_precision[i]], int(r.cand_tile[i]), float(r.cand_ms[i]))
+                 for i in range(r.ncand)]
+        return (inv[r.precision], int(r.tile)), cands
 
     def codes(self):
         out = np.empty((self.N, MLP_CODE), np.float32)
@@ -296,17 +314,20 @@ class State:
 def large_index(DIM, K, cluster_num, max_points, file_prefix, local_index_path_prefix,
                 index_store_path, max_degree, buffer_size, all_node_num, centers_path, thre,
                 FINAL_DEGREE, TOPM, device=0, seed=0x7A600000, precision="bf16", batch_rows=0,
-                iters=16, verbose=False):
+                iters=16, verbose=False, tune=False):
     """tg_large_index through ctypes; returns the stats as a dict."""
     dev = (ctypes.c_int32 * 1)(device)
     p = LargeParams(dev, 1, DIM, K, cluster_num, max_points, file_prefix.encode(),
                     local_index_path_prefix.encode(), index_store_path.encode(), max_degree,
                     buffer_size, all_node_num, centers_path.encode(), thre, FINAL_DEGREE, TOPM,
-                    iters, seed, PREC[precision], batch_rows, 1 if verbose else 0)
+                    iters, seed, PREC[precision], batch_rows, 1 if verbose else 0,
+                    1 if tune else 0)
     st = LargeStats()
     _check(lib().tg_large_index(ctypes.byref(p), ctypes.byref(st)))
+    inv = {v: k for k, v in PREC.items()}
     return dict(clusters=st.clusters, batches=st.batches, build_ms=st.build_ms,
-                merge_ms=st.merge_ms, total_ms=st.total_ms)
+                merge_ms=st.merge_ms, total_ms=st.total_ms, precision=inv[st.precision],
+                tile=st.tile)
 
 
 def segment_seed(seed: int, c: int) -> int:

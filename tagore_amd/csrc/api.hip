@@ -108,6 +108,9 @@ struct tg_state {
   uint64_t *R0 = nullptr, *R1 = nullptr;
   uint32_t *Rn0 = nullptr, *Rn1 = nullptr;
   PB pn, po, pc, pr;
+  uint8_t* Xq8 = nullptr;  // FP8 e4m3 copy of X with per-vector scales (screen)
+  float *sc8 = nullptr, *nq8 = nullptr, *ne8 = nullptr;
+  bool prep8_valid = false;
   uint16_t* Xb = nullptr;  // BF16 copy of X (screen)
   float *nrm2 = nullptr, *nq = nullptr, *ne = nullptr;
   uint32_t* cert_flags = nullptr;  // [4], see k_screen_prep
@@ -242,6 +245,20 @@ int ensure_prep(tg_state* s) {
   const double lim = 16777216.0;
   s->cert_int = f[1] == 0 && (f[2] == 0 ? 2.0 * qmax < lim : 4.0 * qmax < lim);
   s->prep_valid = true;
+  return TG_OK;
+}
+
+// FP8 copy with per-vector scales and its norms (once per data set)
+int ensure_prep8(tg_state* s) {
+  if (s->prep8_valid) return TG_OK;
+  const size_t N = s->N;
+  if (!s->Xq8) {
+    if (dalloc(&s->Xq8, N * s->D) || dalloc(&s->sc8, N) || dalloc(&s->nq8, N) ||
+        dalloc(&s->ne8, N))
+      return TG_ERR_HIP;
+  }
+  launch_screen_prep8(s->X, s->N, s->D, s->Xq8, s->sc8, s->nq8, s->ne8, s->stream);
+  s->prep8_valid = true;
   return TG_OK;
 }
 
@@ -407,6 +424,10 @@ int tg_free(tg_state* s) {
   s->pr.release();
   s->rev.release();
   dfree(s->Xb);
+  dfree(s->Xq8);
+  dfree(s->sc8);
+  dfree(s->nq8);
+  dfree(s->ne8);
   dfree(s->nrm2);
   dfree(s->nq);
   dfree(s->ne);
@@ -435,6 +456,7 @@ int tg_set_data(tg_state* s, const float* rows) {
   HIPX(hipStreamSynchronize(s->stream));
   s->has_knn = false;
   s->prep_valid = false;
+  s->prep8_valid = false;
   s->has_prune = false;
   return TG_OK;
 }
@@ -453,6 +475,7 @@ int tg_gen_data(tg_state* s, int32_t kind, uint32_t ncomp, uint64_t seed, float 
   HIPX(hipStreamSynchronize(s->stream));
   s->has_knn = false;
   s->prep_valid = false;
+  s->prep8_valid = false;
   s->has_prune = false;
   return TG_OK;
 }
@@ -670,12 +693,15 @@ int tg_knn_build(tg_state* s, const tg_knn_params* p) {
   const uint32_t K = p->K;
   if (K < 2 || K > 128) return fail(TG_ERR_ARG, "K must be in [2, 128]");
   if (K >= s->N) return fail(TG_ERR_ARG, "K must be smaller than N");
-  if (p->precision < TG_PREC_EXACT || p->precision > TG_PREC_BF16)
+  if (p->precision < TG_PREC_EXACT || p->precision > TG_PREC_FP8)
     return fail(TG_ERR_ARG, "unknown precision");
   if (p->precision == TG_PREC_BF16 && s->D % 8 != 0)
     return fail(TG_ERR_ARG, "BF16 screening needs D % 8 == 0");
+  if (p->precision == TG_PREC_FP8 && s->D % 16 != 0)
+    return fail(TG_ERR_ARG, "FP8 screening needs D % 16 == 0");
   if (p->init < TG_INIT_RANDOM || p->init > TG_INIT_GIVEN)
     return fail(TG_ERR_ARG, "unknown init");
+  if (p->tile > 1) return fail(TG_ERR_ARG, "tile must be 0 or 1");
   if (p->init == TG_INIT_GIVEN && (!s->has_knn || s->K != K))
     return fail(TG_ERR_STATE, "init=given needs a current graph with the same K");
   if (s->nseg) {
@@ -694,6 +720,8 @@ int tg_knn_build(tg_state* s, const tg_knn_params* p) {
   if (ensure_graph(s, K) || ensure_knn_scratch(s, K)) return TG_ERR_HIP;
   if (p->precision != TG_PREC_EXACT)
     if (int rc = ensure_prep(s)) return rc;
+  if (p->precision == TG_PREC_FP8)
+    if (int rc = ensure_prep8(s)) return rc;
   hipStream_t st = s->stream;
   if (ensure_events(s, 2)) return TG_ERR_HIP;
   // window events (separate from the per-section pool)
@@ -706,8 +734,12 @@ int tg_knn_build(tg_state* s, const tg_knn_params* p) {
   clear_errors(s, {&s->pn, &s->po, &s->pc, &s->pr});
   HIPX(hipEventRecord(w0, st));
   // precision-dependent parts of the argument block (re-applied after every swap)
-  const bool exact_screen = p->precision != TG_PREC_EXACT && s->cert_int;
-  const uint16_t* rows16 = (p->precision == TG_PREC_BF16 && s->cert_bf16) ? s->Xb : nullptr;
+  // integer data makes the FP32 / BF16 screens exact; FP8 rounds integers above 16
+  const bool exact_screen =
+      (p->precision == TG_PREC_FP32 || p->precision == TG_PREC_BF16) && s->cert_int;
+  const uint16_t* rows16 =
+      ((p->precision == TG_PREC_BF16 || p->precision == TG_PREC_FP8) && s->cert_bf16) ? s->Xb
+                                                                                      : nullptr;
   auto args = [&]() {
     KnnArgs k = knn_args(s, p->seed);
     if (s->nseg) {
@@ -717,6 +749,12 @@ int tg_knn_build(tg_state* s, const tg_knn_params* p) {
     }
     k.exact_screen = exact_screen ? 1 : 0;
     k.Xr16 = rows16;
+    if (p->precision == TG_PREC_FP8) {
+      k.Xq8 = s->Xq8;
+      k.sc8 = s->sc8;
+      k.nq = s->nq8;
+      k.ne = s->ne8;
+    }
     return k;
   };
   KnnArgs a = args();
@@ -749,7 +787,7 @@ int tg_knn_build(tg_state* s, const tg_knn_params* p) {
     if (sec.stop()) return TG_ERR_HIP;
     if (sec.start(3)) return TG_ERR_HIP;
     pb_reset(a.pc, N, s->fill, st);
-    launch_knn_localjoin(a, p->precision, st);
+    launch_knn_localjoin(a, p->precision, int(p->tile), st);
     if (sec.stop()) return TG_ERR_HIP;
     if (sec.start(4)) return TG_ERR_HIP;
     pb_group_overflow(a.pc, N, s->fill, s->scan_tmp, st);
@@ -1024,6 +1062,48 @@ int tg_set_segments(tg_state* s, uint32_t nseg, const uint32_t* starts, const ui
   }
   s->has_knn = false;
   s->has_prune = false;
+  return TG_OK;
+}
+
+int tg_autotune_knn(tg_state* s, uint32_t K, uint64_t seed, tg_tune_result* out) {
+  if (!s || !out) return fail(TG_ERR_ARG, "null argument");
+  *out = tg_tune_result{};
+  const bool bf16 = s->D % 8 == 0;
+  const int32_t precs[5] = {TG_PREC_EXACT, TG_PREC_FP32, TG_PREC_FP32, TG_PREC_BF16, TG_PREC_BF16};
+  const uint32_t tiles[5] = {0, 0, 1, 0, 1};
+  const bool prof = s->profiling;
+  s->profiling = false;
+  tg_knn_params kp{};
+  kp.K = K;
+  kp.iters = 2;
+  kp.seed = seed;
+  kp.init = TG_INIT_RANDOM;
+  // warm-up (first-touch allocations, code objects)
+  kp.precision = TG_PREC_EXACT;
+  if (int rc = tg_knn_build(s, &kp)) {
+    s->profiling = prof;
+    return rc;
+  }
+  double best = 1e300;
+  for (int c = 0; c < 5; ++c) {
+    if (precs[c] == TG_PREC_BF16 && !bf16) continue;
+    kp.precision = precs[c];
+    kp.tile = tiles[c];
+    if (int rc = tg_knn_build(s, &kp)) {
+      s->profiling = prof;
+      return rc;
+    }
+    const uint32_t i = out->ncand++;
+    out->cand_precision[i] = precs[c];
+    out->cand_tile[i] = tiles[c];
+    out->cand_ms[i] = s->stats.knn_ms;
+    if (s->stats.knn_ms < best) {
+      best = s->stats.knn_ms;
+      out->precision = precs[c];
+      out->tile = tiles[c];
+    }
+  }
+  s->profiling = prof;
   return TG_OK;
 }
 

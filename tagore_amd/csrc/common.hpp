@@ -138,69 +138,92 @@ __device__ __forceinline__ float tree16(float p0, float p1, float p2, float p3) 
   return p0 + p1;
 }
 
+// Row chunks held in flight per pair / candidate: at most 4 x 64 elements, so wide rows
+// (D up to 1024) stream through the same registers in order (each partial still folds its
+// elements in ascending index order).
+template <int MC>
+struct RowChunk {
+  static constexpr int N = MC < 4 ? MC : 4;
+};
+
 // NB distances from row a (registers) to rows b[0..NB) (b[i] may be null: result unused);
-// all loads are issued before the arithmetic.  Results valid in lane t == 0.
+// the loads of a chunk are issued before its arithmetic.  Results valid in lane t == 0.
 template <int MC, int NB, class T>
 __device__ __forceinline__ void dist16_batch(const Row16<MC>& a, const T* const (&b)[NB], int D,
                                              int t, float (&out)[NB]) {
-  float4 y[NB][MC];
+  constexpr int CH = RowChunk<MC>::N;
+  float p[NB][4];
 #pragma unroll
-  for (int k = 0; k < NB; ++k) {
+  for (int k = 0; k < NB; ++k) p[k][0] = p[k][1] = p[k][2] = p[k][3] = 0.f;
 #pragma unroll
-    for (int m = 0; m < MC; ++m) {
-      const int i = m * 64 + 4 * t;
-      y[k][m] = (b[k] && i < D) ? load4(b[k] + i) : make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-  }
+  for (int m0 = 0; m0 < MC; m0 += CH) {
+    float4 y[NB][CH];
 #pragma unroll
-  for (int k = 0; k < NB; ++k) {
-    float p0 = 0.f, p1 = 0.f, p2 = 0.f, p3 = 0.f;
+    for (int k = 0; k < NB; ++k) {
 #pragma unroll
-    for (int m = 0; m < MC; ++m) {
-      if (m * 64 + 4 * t < D) {
-        const float e0 = a.v[m].x - y[k][m].x, e1 = a.v[m].y - y[k][m].y,
-                    e2 = a.v[m].z - y[k][m].z, e3 = a.v[m].w - y[k][m].w;
-        p0 = __builtin_fmaf(e0, e0, p0);
-        p1 = __builtin_fmaf(e1, e1, p1);
-        p2 = __builtin_fmaf(e2, e2, p2);
-        p3 = __builtin_fmaf(e3, e3, p3);
+      for (int c = 0; c < CH; ++c) {
+        const int i = (m0 + c) * 64 + 4 * t;
+        y[k][c] = (b[k] && i < D) ? load4(b[k] + i) : make_float4(0.f, 0.f, 0.f, 0.f);
       }
     }
-    out[k] = tree16(p0, p1, p2, p3);
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const int m = m0 + c;
+        if (m * 64 + 4 * t < D) {
+          const float e0 = a.v[m].x - y[k][c].x, e1 = a.v[m].y - y[k][c].y,
+                      e2 = a.v[m].z - y[k][c].z, e3 = a.v[m].w - y[k][c].w;
+          p[k][0] = __builtin_fmaf(e0, e0, p[k][0]);
+          p[k][1] = __builtin_fmaf(e1, e1, p[k][1]);
+          p[k][2] = __builtin_fmaf(e2, e2, p[k][2]);
+          p[k][3] = __builtin_fmaf(e3, e3, p[k][3]);
+        }
+      }
+    }
   }
+#pragma unroll
+  for (int k = 0; k < NB; ++k) out[k] = tree16(p[k][0], p[k][1], p[k][2], p[k][3]);
 }
 
 // NB distances between the row pairs (a[k], b[k]), both read from memory (null: unused).
 template <int MC, int NB, class T>
 __device__ __forceinline__ void dist16_pairs(const T* const (&a)[NB], const T* const (&b)[NB],
                                              int D, int t, float (&out)[NB]) {
-  float4 x[NB][MC], y[NB][MC];
+  constexpr int CH = RowChunk<MC>::N;
+  float p[NB][4];
 #pragma unroll
-  for (int k = 0; k < NB; ++k) {
+  for (int k = 0; k < NB; ++k) p[k][0] = p[k][1] = p[k][2] = p[k][3] = 0.f;
 #pragma unroll
-    for (int m = 0; m < MC; ++m) {
-      const int i = m * 64 + 4 * t;
-      const bool ok = a[k] && i < D;
-      x[k][m] = ok ? load4(a[k] + i) : make_float4(0.f, 0.f, 0.f, 0.f);
-      y[k][m] = ok ? load4(b[k] + i) : make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-  }
+  for (int m0 = 0; m0 < MC; m0 += CH) {
+    float4 x[NB][CH], y[NB][CH];
 #pragma unroll
-  for (int k = 0; k < NB; ++k) {
-    float p0 = 0.f, p1 = 0.f, p2 = 0.f, p3 = 0.f;
+    for (int k = 0; k < NB; ++k) {
 #pragma unroll
-    for (int m = 0; m < MC; ++m) {
-      if (m * 64 + 4 * t < D) {
-        const float e0 = x[k][m].x - y[k][m].x, e1 = x[k][m].y - y[k][m].y,
-                    e2 = x[k][m].z - y[k][m].z, e3 = x[k][m].w - y[k][m].w;
-        p0 = __builtin_fmaf(e0, e0, p0);
-        p1 = __builtin_fmaf(e1, e1, p1);
-        p2 = __builtin_fmaf(e2, e2, p2);
-        p3 = __builtin_fmaf(e3, e3, p3);
+      for (int c = 0; c < CH; ++c) {
+        const int i = (m0 + c) * 64 + 4 * t;
+        const bool ok = a[k] && i < D;
+        x[k][c] = ok ? load4(a[k] + i) : make_float4(0.f, 0.f, 0.f, 0.f);
+        y[k][c] = ok ? load4(b[k] + i) : make_float4(0.f, 0.f, 0.f, 0.f);
       }
     }
-    out[k] = tree16(p0, p1, p2, p3);
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        if ((m0 + c) * 64 + 4 * t < D) {
+          const float e0 = x[k][c].x - y[k][c].x, e1 = x[k][c].y - y[k][c].y,
+                      e2 = x[k][c].z - y[k][c].z, e3 = x[k][c].w - y[k][c].w;
+          p[k][0] = __builtin_fmaf(e0, e0, p[k][0]);
+          p[k][1] = __builtin_fmaf(e1, e1, p[k][1]);
+          p[k][2] = __builtin_fmaf(e2, e2, p[k][2]);
+          p[k][3] = __builtin_fmaf(e3, e3, p[k][3]);
+        }
+      }
+    }
   }
+#pragma unroll
+  for (int k = 0; k < NB; ++k) out[k] = tree16(p[k][0], p[k][1], p[k][2], p[k][3]);
 }
 
 template <int MC, class T>

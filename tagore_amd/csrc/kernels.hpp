@@ -51,7 +51,7 @@ void launch_centroid(const float* X, uint32_t N, uint32_t D, double* part, float
                      hipStream_t s);
 
 // ---------------------------------------------------------------------- kNN construction
-enum KnnPrecision { KP_EXACT = 0, KP_FP32 = 1, KP_BF16 = 2 };
+enum KnnPrecision { KP_EXACT = 0, KP_FP32 = 1, KP_BF16 = 2, KP_FP8 = 3 };
 
 struct KnnArgs {
   const float* X;
@@ -87,6 +87,9 @@ struct KnnArgs {
   int exact_screen;
   // rows used for canonical re-scores: BF16 copy when the data set is BF16-exact, else X
   const uint16_t* Xr16;
+  // FP8 e4m3 copy with a per-vector scale (FP8 screen: nq/ne then hold its norms)
+  const uint8_t* Xq8;
+  const float* sc8;
   // segments (see PruneArgs): the initial lists draw ids inside the row's own segment, from a
   // stream of (segment seed, row index within the segment); every later step follows edges
   const uint32_t* seg_of;
@@ -96,6 +99,9 @@ struct KnnArgs {
 
 // screen-mode preparation: BF16 copy (if Xb), norms, quantisation-error norms and the
 // data-set certificates in flags[4] (zeroed by the caller)
+// FP8 copy: q = e4m3(x / s), s = max|x| / 440, with |s q| and |x - s q| (rounded up)
+void launch_screen_prep8(const float* X, uint32_t N, uint32_t D, uint8_t* Xq, float* sc,
+                         float* nq, float* ne, hipStream_t s);
 void launch_screen_prep(const float* X, uint32_t N, uint32_t D, uint16_t* Xb, float* nrm2,
                         float* nq, float* ne, uint32_t* flags, hipStream_t s);
 
@@ -104,7 +110,8 @@ void launch_knn_init(const KnnArgs& a, hipStream_t s);
 void launch_knn_init_sort(const KnnArgs& a, hipStream_t s);
 void launch_knn_sample(const KnnArgs& a, hipStream_t s);
 void launch_knn_rsample(const KnnArgs& a, hipStream_t s);
-void launch_knn_localjoin(const KnnArgs& a, int precision, hipStream_t s);
+// tile: staging depth of the MFMA screen (0: 256-byte rows per round, 1: 64-byte rows)
+void launch_knn_localjoin(const KnnArgs& a, int precision, int tile, hipStream_t s);
 void launch_knn_merge(const KnnArgs& a, hipStream_t s);
 void launch_knn_rev_build(const KnnArgs& a, hipStream_t s);
 void launch_knn_rev_reduce(const KnnArgs& a, hipStream_t s);

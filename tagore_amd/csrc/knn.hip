@@ -389,6 +389,59 @@ __device__ __forceinline__ float dot_bf16(const RowBf<MB>& a, const uint16_t* y,
   return s;
 }
 
+// FP8 screen of one candidate: lane t of the 16-lane group holds u's dequantised elements
+// 256m + 16t..+15 (D % 16 == 0); the scales are applied by the caller.
+template <int MF>
+struct RowF8 {
+  float v[MF][16];
+};
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+template <int MF>
+__device__ __forceinline__ void load_rowf8(RowF8<MF>& r, const uint8_t* x, int D, int t) {
+#pragma unroll
+  for (int m = 0; m < MF; ++m) {
+    const int i = 256 * m + 16 * t;
+    uint4 q = make_uint4(0, 0, 0, 0);
+    if (i < D) q = *reinterpret_cast<const uint4*>(x + i);
+    const int w[4] = {int(q.x), int(q.y), int(q.z), int(q.w)};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const f32x2 lo = __builtin_amdgcn_cvt_pk_f32_fp8(w[k], false);
+      const f32x2 hi = __builtin_amdgcn_cvt_pk_f32_fp8(w[k], true);
+      r.v[m][4 * k] = lo.x;
+      r.v[m][4 * k + 1] = lo.y;
+      r.v[m][4 * k + 2] = hi.x;
+      r.v[m][4 * k + 3] = hi.y;
+    }
+  }
+}
+
+template <int MF>
+__device__ __forceinline__ float dot_f8(const RowF8<MF>& a, const uint8_t* y, int D, int t) {
+  float s = 0.f;
+#pragma unroll
+  for (int m = 0; m < MF; ++m) {
+    const int i = 256 * m + 16 * t;
+    if (i < D) {
+      const uint4 q = *reinterpret_cast<const uint4*>(y + i);
+      const int w[4] = {int(q.x), int(q.y), int(q.z), int(q.w)};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const f32x2 lo = __builtin_amdgcn_cvt_pk_f32_fp8(w[k], false);
+        const f32x2 hi = __builtin_amdgcn_cvt_pk_f32_fp8(w[k], true);
+        s = __builtin_fmaf(a.v[m][4 * k], lo.x, s);
+        s = __builtin_fmaf(a.v[m][4 * k + 1], lo.y, s);
+        s = __builtin_fmaf(a.v[m][4 * k + 2], hi.x, s);
+        s = __builtin_fmaf(a.v[m][4 * k + 3], hi.y, s);
+      }
+    }
+  }
+#pragma unroll
+  for (int o = 8; o >= 1; o >>= 1) s += __shfl_xor(s, o, 16);
+  return s;
+}
+
 __device__ __forceinline__ float screen_bound2(float na, float qa, float ea, float nb, float qb,
                                                float eb, float eta) {
   return 2.0f * (ea * qb + qa * eb + ea * eb) + eta * (na + nb + 2.0f * qa * qb);
@@ -453,10 +506,17 @@ __global__ __launch_bounds__(64) void k_phase2(KnnArgs a) {
   const float worst = key_d(row.gk[K - 1]);
   const int nk = pow2_at_least(max(cnt, 1));
   constexpr int MB = (MC + 1) / 2;
+  constexpr int MF = (MC + 3) / 4;
+  constexpr bool SCREEN = PREC == KP_BF16 || PREC == KP_FP8;
   RowBf<PREC == KP_BF16 ? MB : 1> ub;
-  float nu = 0.f, qu = 0.f, eu = 0.f;
-  if (PREC == KP_BF16) {
-    load_rowbf(ub, a.Xb + size_t(u) * D, D, t);
+  RowF8<PREC == KP_FP8 ? MF : 1> uf;
+  float nu = 0.f, qu = 0.f, eu = 0.f, su = 1.f;
+  if (PREC == KP_BF16) load_rowbf(ub, a.Xb + size_t(u) * D, D, t);
+  if (PREC == KP_FP8) {
+    load_rowf8(uf, a.Xq8 + size_t(u) * D, D, t);
+    su = a.sc8[u];
+  }
+  if (SCREEN) {
     nu = a.nrm2[u];
     qu = a.nq[u];
     eu = a.ne[u];
@@ -465,11 +525,14 @@ __global__ __launch_bounds__(64) void k_phase2(KnnArgs a) {
   // BF16 screen: keep (in cl order) only the candidates whose bound interval reaches below
   // the worst distance; everything kept is re-scored exactly
   int nre = cnt;
-  if (PREC == KP_BF16) {
+  if (SCREEN) {
     __shared__ uint16_t keep[512];
     for (int q = g; q < cnt; q += 4) {
       const uint32_t y = ids[cl[q]];
-      const float s = (nu + a.nrm2[y]) - 2.0f * dot_bf16(ub, a.Xb + size_t(y) * D, D, t);
+      const float dot = PREC == KP_FP8
+                            ? (su * a.sc8[y]) * dot_f8(uf, a.Xq8 + size_t(y) * D, D, t)
+                            : dot_bf16(ub, a.Xb + size_t(y) * D, D, t);
+      const float s = (nu + a.nrm2[y]) - 2.0f * dot;
       if (a.exact_screen) {
         // integer data: s is the canonical distance itself
         if (t == 0 && s < worst) kv[q] = mk_key(s, y);
@@ -581,13 +644,13 @@ void launch_knn_init(const KnnArgs& a, hipStream_t s) {
 }
 void launch_knn_sample(const KnnArgs& a, hipStream_t s) { k_sample<<<a.N, 64, 0, s>>>(a); }
 void launch_knn_rsample(const KnnArgs& a, hipStream_t s) { k_rsample<<<a.N, 64, 0, s>>>(a); }
-void launch_knn_localjoin_screen(const KnnArgs& a, int precision, hipStream_t s);
+void launch_knn_localjoin_screen(const KnnArgs& a, int precision, int tile, hipStream_t s);
 
-void launch_knn_localjoin(const KnnArgs& a, int precision, hipStream_t s) {
+void launch_knn_localjoin(const KnnArgs& a, int precision, int tile, hipStream_t s) {
   if (precision == KP_EXACT)
     TG_MC_DISPATCH(k_localjoin_exact, a.N, 256, s, (a));
   else
-    launch_knn_localjoin_screen(a, precision, s);
+    launch_knn_localjoin_screen(a, precision, tile, s);
 }
 void launch_knn_merge(const KnnArgs& a, hipStream_t s) { k_merge<<<a.N, 64, 0, s>>>(a); }
 void launch_knn_rev_build(const KnnArgs& a, hipStream_t s) { k_rev_build<<<a.N, 64, 0, s>>>(a); }
@@ -599,7 +662,9 @@ void launch_knn_phase2(const KnnArgs& a, int precision, hipStream_t s) {
   do {                                                               \
     if (mc <= 2) P2(2, P, T); else if (mc <= 4) P2(4, P, T); else P2(16, P, T); \
   } while (0)
-  if (precision == KP_BF16) {
+  if (precision == KP_FP8) {
+    if (a.Xr16) P2_MC(KP_FP8, uint16_t); else P2_MC(KP_FP8, float);
+  } else if (precision == KP_BF16) {
     if (a.Xr16) P2_MC(KP_BF16, uint16_t); else P2_MC(KP_BF16, float);
   } else {
     if (a.Xr16) P2_MC(KP_EXACT, uint16_t); else P2_MC(KP_EXACT, float);

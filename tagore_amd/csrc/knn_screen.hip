@@ -62,6 +62,50 @@ __global__ void k_screen_prep(const float* X, uint32_t N, uint32_t D, uint16_t* 
   }
 }
 
+// FP8 e4m3 copy with a per-vector scale: q = e4m3(x / s), s = max|x| / 440 (below the
+// format's 448 maximum), and the norms the screen bound needs: |s q| and |x - s q| in FP64,
+// rounded up.  One wave per row, four elements per lane per step.
+__global__ void k_screen_prep8(const float* X, uint32_t N, uint32_t D, uint8_t* Xq, float* sc,
+                               float* nq, float* ne) {
+  const uint32_t row = blockIdx.x;
+  if (row >= N) return;
+  const float* x = X + size_t(row) * D;
+  float mx = 0.f;
+  for (uint32_t j = threadIdx.x; j < D; j += TG_WAVE) mx = fmaxf(mx, fabsf(x[j]));
+  for (int o = 32; o >= 1; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, TG_WAVE));
+  const float s = mx > 0.f ? mx / 440.f : 1.f;
+  double q2 = 0.0, e2 = 0.0;
+  for (uint32_t j = 4 * threadIdx.x; j < D; j += 4 * TG_WAVE) {
+    const float v0 = x[j], v1 = x[j + 1], v2 = x[j + 2], v3 = x[j + 3];
+    int p = __builtin_amdgcn_cvt_pk_fp8_f32(v0 / s, v1 / s, 0, false);
+    p = __builtin_amdgcn_cvt_pk_fp8_f32(v2 / s, v3 / s, p, true);
+    *reinterpret_cast<int*>(Xq + size_t(row) * D + j) = p;
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    const f2 lo = __builtin_amdgcn_cvt_pk_f32_fp8(p, false), hi = __builtin_amdgcn_cvt_pk_f32_fp8(p, true);
+    const float v[4] = {v0, v1, v2, v3}, dq[4] = {lo.x, lo.y, hi.x, hi.y};
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const double w = double(s) * double(dq[b]);
+      q2 += w * w;
+      e2 += (double(v[b]) - w) * (double(v[b]) - w);
+    }
+  }
+  for (int o = 32; o >= 1; o >>= 1) {
+    q2 += __shfl_xor(q2, o, TG_WAVE);
+    e2 += __shfl_xor(e2, o, TG_WAVE);
+  }
+  if (threadIdx.x == 0) {
+    sc[row] = s;
+    nq[row] = float(sqrt(q2) * (1.0 + 1e-6));
+    ne[row] = float(sqrt(e2) * (1.0 + 1e-6));
+  }
+}
+
+void launch_screen_prep8(const float* X, uint32_t N, uint32_t D, uint8_t* Xq, float* sc,
+                         float* nq, float* ne, hipStream_t s) {
+  k_screen_prep8<<<N, TG_WAVE, 0, s>>>(X, N, D, Xq, sc, nq, ne);
+}
+
 void launch_screen_prep(const float* X, uint32_t N, uint32_t D, uint16_t* Xb, float* nrm2,
                         float* nq, float* ne, uint32_t* flags, hipStream_t s) {
   k_screen_prep<<<N, TG_WAVE, 0, s>>>(X, N, D, Xb, nrm2, nq, ne, flags);
@@ -73,22 +117,27 @@ __device__ __forceinline__ float screen_bound(float na, float qa, float ea, floa
 }
 
 // -------------------------------------------------------------------------------------
-// Screened local join.  PREC 1: FP32 MFMA (16x16x4), PREC 2: BF16 MFMA (16x16x32).
+// Screened local join.  PREC 1: FP32 MFMA (16x16x4), PREC 2: BF16 MFMA (16x16x32),
+// PREC 3: FP8 e4m3 MFMA (16x16x32) with per-vector scales applied to the product.
 // 256 threads: wave w owns row block w (16 rows of A) against the 4 column blocks of B and
 // of A.  Staging chunk: DC elements per row (128 BF16 or 64 FP32 = 256 B).
 // -------------------------------------------------------------------------------------
-template <int PREC, int MC, class T>
+template <int PREC, int MC, class T, int DC>
 __global__ __launch_bounds__(256) void k_localjoin_screen(KnnArgs a) {
-  constexpr int DC = PREC == 2 ? 128 : 64;          // elements per staged chunk
-  constexpr int ESZ = PREC == 2 ? 2 : 4;            // element bytes
+  // DC: elements per staged chunk (the tile's k depth per LDS round)
+  constexpr int ESZ = PREC == 3 ? 1 : (PREC == 2 ? 2 : 4);  // element bytes
+  constexpr int CPR = DC * ESZ / 16;                // 16-byte loads per staged row
   constexpr int RS = DC * ESZ + 16;                 // padded row stride in bytes
   constexpr int STAGE = 64 * RS;                    // one operand
   constexpr int MROW = 65;                          // padded distance-matrix row (floats)
-  static_assert(2 * STAGE >= 2 * 64 * MROW * 4, "distance matrices alias the staging area");
-  __shared__ __attribute__((aligned(16))) unsigned char smem[2 * STAGE];
+  constexpr int SMEM = 2 * STAGE > 2 * 64 * MROW * 4 ? 2 * STAGE : 2 * 64 * MROW * 4;
+  static_assert(DC % (PREC >= 2 ? 32 : 4) == 0 && CPR >= 1, "tile depth");
+  // the distance matrices alias the staging area once the products are done
+  __shared__ __attribute__((aligned(16))) unsigned char smem[SMEM];
   __shared__ uint32_t A[64], B[64];
   __shared__ float naA[64], qA[64], eA[64], wA[64];
   __shared__ float naB[64], qB[64], eB[64], wB[64];
+  __shared__ float scA[64], scB[64];
   unsigned char* sA = smem;
   unsigned char* sB = smem + STAGE;
   float* MAB = reinterpret_cast<float*>(smem);
@@ -107,6 +156,7 @@ __global__ __launch_bounds__(256) void k_localjoin_screen(KnnArgs a) {
       qA[tid] = a.nq[v];
       eA[tid] = a.ne[v];
       wA[tid] = a.GD[size_t(v) * K + K - 1];
+      if (PREC == 3) scA[tid] = a.sc8[v];
     }
   } else if (tid < 128) {
     const int j = tid - 64;
@@ -117,6 +167,7 @@ __global__ __launch_bounds__(256) void k_localjoin_screen(KnnArgs a) {
       qB[j] = a.nq[v];
       eB[j] = a.ne[v];
       wB[j] = a.GD[size_t(v) * K + K - 1];
+      if (PREC == 3) scB[j] = a.sc8[v];
     }
   }
   __syncthreads();
@@ -127,12 +178,13 @@ __global__ __launch_bounds__(256) void k_localjoin_screen(KnnArgs a) {
     accB[c] = f32x4{0.f, 0.f, 0.f, 0.f};
     accA[c] = f32x4{0.f, 0.f, 0.f, 0.f};
   }
-  const unsigned char* src = PREC == 2 ? reinterpret_cast<const unsigned char*>(a.Xb)
-                                       : reinterpret_cast<const unsigned char*>(a.X);
+  const unsigned char* src = PREC == 3   ? reinterpret_cast<const unsigned char*>(a.Xq8)
+                             : PREC == 2 ? reinterpret_cast<const unsigned char*>(a.Xb)
+                                         : reinterpret_cast<const unsigned char*>(a.X);
   for (int d0 = 0; d0 < D; d0 += DC) {
-    // stage rows [d0, d0+DC) of A and B: 16 B per thread-load, 16 loads per 256-B row
-    for (int e = tid; e < 2 * 64 * 16; e += 256) {
-      const int op = e >> 10, r = (e >> 4) & 63, c16 = e & 15;
+    // stage rows [d0, d0+DC) of A and B: 16 B per thread-load, CPR loads per row
+    for (int e = tid; e < 2 * 64 * CPR; e += 256) {
+      const int op = e / (64 * CPR), r = (e / CPR) % 64, c16 = e % CPR;
       const int n = op == 0 ? nA : nB;
       uint4 v = make_uint4(0, 0, 0, 0);
       const int col = d0 + c16 * (16 / ESZ);
@@ -143,7 +195,20 @@ __global__ __launch_bounds__(256) void k_localjoin_screen(KnnArgs a) {
       *reinterpret_cast<uint4*>((op == 0 ? sA : sB) + r * RS + c16 * 16) = v;
     }
     __syncthreads();
-    if (PREC == 2) {
+    if (PREC == 3) {
+#pragma unroll
+      for (int k0 = 0; k0 < DC; k0 += 32) {
+        const int kk = k0 + 8 * (l >> 4);
+        const long fa = *reinterpret_cast<const long*>(sA + (16 * w + (l & 15)) * RS + kk);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const long fb = *reinterpret_cast<const long*>(sB + (16 * c + (l & 15)) * RS + kk);
+          const long fc = *reinterpret_cast<const long*>(sA + (16 * c + (l & 15)) * RS + kk);
+          accB[c] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(fa, fb, accB[c], 0, 0, 0);
+          accA[c] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(fa, fc, accA[c], 0, 0, 0);
+        }
+      }
+    } else if (PREC == 2) {
 #pragma unroll
       for (int k0 = 0; k0 < DC; k0 += 32) {
         const int kk = k0 + 8 * (l >> 4);
@@ -180,8 +245,14 @@ __global__ __launch_bounds__(256) void k_localjoin_screen(KnnArgs a) {
     for (int r = 0; r < 4; ++r) {
       const int i = 16 * w + 4 * (l >> 4) + r, j = 16 * c + (l & 15);
       float sb = INF, sa = INF;
-      if (i < nA && j < nB && A[i] != B[j]) sb = (naA[i] + naB[j]) - 2.0f * accB[c][r];
-      if (i < nA && j < nA && i != j) sa = (naA[i] + naA[j]) - 2.0f * accA[c][r];
+      float pb = accB[c][r], pa = accA[c][r];
+      if (PREC == 3) {
+        // scale back: (s_i s_j) * sum q_i q_j
+        pb *= (i < nA && j < nB) ? scA[i] * scB[j] : 0.f;
+        pa *= (i < nA && j < nA) ? scA[i] * scA[j] : 0.f;
+      }
+      if (i < nA && j < nB && A[i] != B[j]) sb = (naA[i] + naB[j]) - 2.0f * pb;
+      if (i < nA && j < nA && i != j) sa = (naA[i] + naA[j]) - 2.0f * pa;
       MAB[i * MROW + j] = sb;
       MAA[i * MROW + j] = sa;
     }
@@ -325,19 +396,30 @@ __global__ __launch_bounds__(256) void k_localjoin_screen(KnnArgs a) {
   }
 }
 
-void launch_knn_localjoin_screen(const KnnArgs& a, int precision, hipStream_t s) {
+// tile 0: 256-byte staged rows (256 FP8 / 128 BF16 / 64 FP32 elements per round); tile 1:
+// 64-byte rows (64 / 32 / 16 elements per round): less zero padding at D not a multiple of
+// the depth, more LDS rounds
+void launch_knn_localjoin_screen(const KnnArgs& a, int precision, int tile, hipStream_t s) {
   const int mc = (int(a.D) + 63) / 64;
-#define LJS(P, M, T) k_localjoin_screen<P, M, T><<<a.N, 256, 0, s>>>(a)
+#define LJS(P, M, T, DC) k_localjoin_screen<P, M, T, DC><<<a.N, 256, 0, s>>>(a)
+#define LJS_DC(P, M, T)                                                    \
+  do {                                                                     \
+    if (tile == 1) LJS(P, M, T, (P == 3 ? 64 : P == 2 ? 32 : 16));        \
+    else LJS(P, M, T, (P == 3 ? 256 : P == 2 ? 128 : 64));                 \
+  } while (0)
 #define LJS_MC(P, T)                                                       \
   do {                                                                     \
-    if (mc <= 2) LJS(P, 2, T); else if (mc <= 4) LJS(P, 4, T); else LJS(P, 16, T); \
+    if (mc <= 2) LJS_DC(P, 2, T); else if (mc <= 4) LJS_DC(P, 4, T); else LJS_DC(P, 16, T); \
   } while (0)
-  if (precision == KP_BF16) {
+  if (precision == KP_FP8) {
+    if (a.Xr16) LJS_MC(3, uint16_t); else LJS_MC(3, float);
+  } else if (precision == KP_BF16) {
     if (a.Xr16) LJS_MC(2, uint16_t); else LJS_MC(2, float);
   } else {
     if (a.Xr16) LJS_MC(1, uint16_t); else LJS_MC(1, float);
   }
 #undef LJS_MC
+#undef LJS_DC
 #undef LJS
 }
 

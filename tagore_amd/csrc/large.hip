@@ -204,6 +204,9 @@ extern "C" int tg_large_index(const tg_large_params* p, tg_large_stats* out_stat
   }
   const auto t_build = clk::now();
   uint32_t nbatches = 0;
+  int32_t precision = p->precision;
+  uint32_t tile = 0;
+  bool tuned = !p->tune;
   for (uint32_t b0 = 0; b0 < C;) {
     uint32_t b1 = b0;
     size_t rows = 0;
@@ -237,11 +240,23 @@ extern "C" int tg_large_index(const tg_large_params* p, tg_large_stats* out_stat
     };
     if ((rc = tg_set_data(s, X.data())) != TG_OK) return bail(rc);
     if ((rc = tg_set_segments(s, b1 - b0, starts.data(), keys.data())) != TG_OK) return bail(rc);
+    if (!tuned) {
+      tg_tune_result tr{};
+      if ((rc = tg_autotune_knn(s, K, p->seed, &tr)) != TG_OK) return bail(rc);
+      precision = tr.precision;
+      tile = tr.tile;
+      tuned = true;
+      if (p->verbose)
+        for (uint32_t c = 0; c < tr.ncand; ++c)
+          printf("autotune: precision %d tile %u: %.2f ms\n", tr.cand_precision[c],
+                 tr.cand_tile[c], tr.cand_ms[c]);
+    }
     tg_knn_params kp{};
     kp.K = K;
     kp.iters = p->iters ? p->iters : 16;
     kp.seed = p->seed;
-    kp.precision = p->precision;
+    kp.precision = precision;
+    kp.tile = tile;
     kp.init = TG_INIT_RANDOM;
     if ((rc = tg_knn_build(s, &kp)) != TG_OK) return bail(rc);
     tg_prune_params pp{};
@@ -427,6 +442,8 @@ extern "C" int tg_large_index(const tg_large_params* p, tg_large_stats* out_stat
     out_stats->build_ms = std::chrono::duration<double, std::milli>(t_merge - t_build).count();
     out_stats->merge_ms = std::chrono::duration<double, std::milli>(t_end - t_merge).count();
     out_stats->total_ms = std::chrono::duration<double, std::milli>(t_end - t_start).count();
+    out_stats->precision = precision;
+    out_stats->tile = tile;
   }
   return TG_OK;
 }

@@ -207,7 +207,7 @@ void largeIndex(std::vector<int> devicelist, unsigned DIM, unsigned K, unsigned 
                 std::string index_store_path, unsigned max_degree, unsigned buffer_size,
                 unsigned all_node_num, std::string centers_path, float thre,
                 unsigned FINAL_DEGREE, unsigned TOPM, uint64_t seed, int precision,
-                uint64_t batch_rows, unsigned iters) {
+                uint64_t batch_rows, unsigned iters, bool tune) {
   std::vector<int32_t> dev(devicelist.begin(), devicelist.end());
   tg_large_params p{};
   p.devicelist = dev.data();
@@ -231,6 +231,7 @@ void largeIndex(std::vector<int> devicelist, unsigned DIM, unsigned K, unsigned 
   p.precision = precision;
   p.batch_rows = batch_rows;
   p.verbose = 1;
+  p.tune = tune ? 1 : 0;
   tg_large_stats st{};
   {
     py::gil_scoped_release nogil;
@@ -258,5 +259,5 @@ PYBIND11_MODULE(Tagore, m) {
         py::arg("buffer_size"), py::arg("all_node_num"), py::arg("centers_path"),
         py::arg("thre"), py::arg("FINAL_DEGREE"), py::arg("TOPM"), py::kw_only(),
         py::arg("seed") = uint64_t(0x7A600000ull), py::arg("precision") = int(TG_PREC_BF16),
-        py::arg("batch_rows") = uint64_t(0), py::arg("iters") = 16u);
+        py::arg("batch_rows") = uint64_t(0), py::arg("iters") = 16u, py::arg("tune") = true);
 }

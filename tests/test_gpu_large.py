@@ -53,7 +53,7 @@ def test_large_index_matches_oracle(tmp_path, precision):
         index = os.path.join(tmp, f"index{br}.bin")
         st = tagore_amd.large_index(D, K, 6, maxp, prefix, local, index, md, B, Npts, cpath,
                                     thre, FD, TOPM, seed=seed, precision=precision,
-                                    batch_rows=br)
+                                    batch_rows=br, tune=(br == 0))
         _say(f"  batch_rows={br}: {st}")
         assert st["clusters"] == C
         assert st["batches"] == (C if br == 1 else 1)
@@ -95,3 +95,21 @@ def test_large_index_rejects(tmp_path):
         tagore_amd.large_index(*args[:10], bad, *args[11:])
     tagore_amd.large_index(*args)  # the valid call works
     assert os.path.getsize(os.path.join(tmp, "i.bin")) > 24
+
+
+def test_dropin_large_index(tmp_path):
+    """The drop-in module's largeIndex (reference argument order) writes the same index."""
+    tmp = str(tmp_path)
+    X, cen, cen2, cpath = _make_case(tmp, Npts=1500, ncl=4, seed=9)
+    clusters, *_ = L.clusters_from_centres(cen, cen2, 4, 10 ** 6)
+    prefix = _write_clusters(tmp, X, clusters)
+    a = os.path.join(tmp, "a.bin")
+    b = os.path.join(tmp, "b.bin")
+    tagore_amd.Tagore.largeIndex([0], 32, 24, 4, 10 ** 6, prefix, os.path.join(tmp, "la_"), a,
+                                 20, 2, 1500, cpath, 1.2, 16, 32)
+    tagore_amd.large_index(32, 24, 4, 10 ** 6, prefix, os.path.join(tmp, "lb_"), b, 20, 2,
+                           1500, cpath, 1.2, 16, 32, seed=0x7A600000, precision="bf16")
+    assert open(a, "rb").read() == open(b, "rb").read()
+    with pytest.raises(ValueError):
+        tagore_amd.Tagore.largeIndex([0, 1], 32, 24, 4, 10 ** 6, prefix, os.path.join(tmp, "l_"),
+                                     a, 20, 2, 1500, cpath, 1.2, 16, 32)

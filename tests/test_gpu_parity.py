@@ -145,7 +145,7 @@ def test_index_files_are_byte_identical(int_case, tmp_path):
             assert open(p_gpu, "rb").read() == open(p_ref, "rb").read(), name
 
 
-@pytest.mark.parametrize("precision", ["exact", "fp32", "bf16"])
+@pytest.mark.parametrize("precision", ["exact", "fp32", "bf16", "fp8"])
 @pytest.mark.parametrize("N,D,K,iters,kind", [(3000, 128, 32, 6, "int"), (2500, 96, 48, 8, "unit"),
                                               (1200, 64, 64, 4, "int"), (2000, 96, 32, 6, "unitb"),
                                               (1500, 128, 32, 6, "sint")])
@@ -242,3 +242,40 @@ def test_dropin_module_end_to_end(tmp_path):
     p_ref = str(tmp_path / "ref.index")
     oracle.store(p_ref, "NSG", ref["deg"], ref["ids"], ref["ep"])
     assert open(idx, "rb").read() == open(p_ref, "rb").read()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_knn_tiles_and_autotune(precision):
+    """Both MFMA staging depths give the oracle's graph; the autotuner times every variant."""
+    N, D, K, iters, seed = 2500, 96, 32, 6, 0x99
+    X = gmm_unit(N, D, 16, 6)
+    Gr, GDr, _ = oracle.gnn_descent(X, K, iters, seed)
+    with State(N, D) as s:
+        s.set_data(X)
+        s.knn_build(K, iters, seed, precision, tile=1)
+        G, GD = s.get_knn()
+        assert np.array_equal(G, Gr) and np.array_equal(GD, GDr)
+        (bp, bt), cands = s.autotune(K, seed)
+        _say(f"autotune: best {bp} tile {bt}; {cands}")
+        assert len(cands) == 5 and (bp, bt) in [(c[0], c[1]) for c in cands]
+        assert min(c[2] for c in cands) == [c[2] for c in cands if (c[0], c[1]) == (bp, bt)][0]
+
+
+@pytest.mark.parametrize("precision", ["exact", "fp8", "bf16"])
+def test_knn_bit_identical_d960(precision):
+    """The C5 shape (dimension 960, non-negative data scaled to [0,1]) in every screen."""
+    N, D, K, iters, seed = 1500, 960, 32, 4, 0x960
+    rng = np.random.default_rng(960)
+    C = rng.standard_normal((8, D))
+    X = np.abs(C[rng.integers(0, 8, N)] + rng.standard_normal((N, D)))
+    X = (X / X.max()).astype(np.float32)
+    _say(f"kNN D=960 {precision}")
+    Gr, GDr, ev = oracle.gnn_descent(X, K, iters, seed)
+    with State(N, D) as s:
+        s.set_data(X)
+        s.knn_build(K, iters, seed, precision)
+        G, GD = s.get_knn()
+        st = s.stats()
+    _say(f"  re-scored {st['knn_rescored']} of {st['knn_evals']} evaluations")
+    assert np.array_equal(G, Gr) and np.array_equal(GD, GDr)
+    assert st["knn_evals"] == ev
