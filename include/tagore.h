@@ -74,7 +74,9 @@ typedef struct {
   float thre;        /* threshold: alpha for dist, cos(angle) for angle */
   int32_t precision; /* TG_PREC_BF16: read rows from the BF16 copy when every value of the
                         data set is exactly representable in BF16 (bit-identical result);
-                        otherwise FP32 rows */
+                        otherwise FP32 rows.  DPG with K <= 64 under TG_PREC_BF16 / FP8:
+                        pair distances screened on an MFMA Gram, exact re-score near the
+                        threshold (bit-identical result) */
 } tg_prune_params;
 
 /* Last error message of the calling thread (never NULL). */
@@ -192,7 +194,8 @@ typedef struct {
   double prune_kernel_ms[4]; /* ep, forward, reverse, other */
   uint64_t knn_evals;        /* distance evaluations of the last build */
   uint64_t knn_rescored;     /* canonical FP32 re-scores after the screen (phase 2 + join) */
-  uint64_t prune_counters[8];/* collect, filter, reverse, adjacency rows, ep evaluations */
+  uint64_t prune_counters[8];/* collect, filter, reverse, adjacency rows, ep evaluations,
+                                exact re-scores of screened DPG pairs */
   double learn_ms[5];        /* learned init: sample graph, training, encoding, buckets, seeding
                                 (filled when profiling is on) */
   float learn_loss[2];       /* training loss at the end of the first and the last epoch */

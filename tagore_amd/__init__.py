@@ -295,8 +295,8 @@ class State:
                     knn_kernel_ms=dict(zip(KNN_SECTIONS, list(s.knn_kernel_ms))),
                     prune_kernel_ms=dict(zip(PRUNE_SECTIONS, list(s.prune_kernel_ms))),
                     knn_evals=int(s.knn_evals), knn_rescored=int(s.knn_rescored),
-                    prune_counters=dict(zip(["collect", "filter", "reverse", "adj_rows", "ep"],
-                                            list(s.prune_counters)[:5])),
+                    prune_counters=dict(zip(["collect", "filter", "reverse", "adj_rows", "ep",
+                                             "rescored"], list(s.prune_counters)[:6])),
                     learn_ms=dict(zip(["sample_graph", "train", "encode", "buckets", "seed"],
                                       list(s.learn_ms))),
                     learn_loss=list(s.learn_loss), learn_buckets=int(s.learn_buckets),

@@ -929,13 +929,19 @@ int tg_prune(tg_state* s, const tg_prune_params* p) {
   a.counters = s->counters;
   a.rows = s->X;
   a.rows_bf16 = 0;
-  if (p->precision == TG_PREC_BF16) {
+  if (p->precision == TG_PREC_BF16 || p->precision == TG_PREC_FP8) {
     if (int rc = ensure_prep(s)) return rc;
-    if (s->cert_bf16) {
+    if (p->precision == TG_PREC_BF16 && s->cert_bf16) {
       a.rows = s->Xb;
       a.rows_bf16 = 1;
     }
   }
+  // DPG's pair distances on an MFMA Gram (K <= 64) with exact re-scores near the threshold
+  const bool dpg_screen = p->kind == TG_KIND_DPG && K <= 64 &&
+                          ((p->precision == TG_PREC_BF16 && s->D % 8 == 0) ||
+                           (p->precision == TG_PREC_FP8 && s->D % 16 == 0));
+  if (dpg_screen && p->precision == TG_PREC_FP8)
+    if (int rc = ensure_prep8(s)) return rc;
   hipEvent_t w0, w1;
   HIPX(hipEventCreate(&w0));
   HIPX(hipEventCreate(&w1));
@@ -965,7 +971,21 @@ int tg_prune(tg_state* s, const tg_prune_params* p) {
     case TG_KIND_PATH: launch_prune(a, PH_FWD_PATH, nullptr, nullptr, st); break;
     case TG_KIND_TWOHOP: launch_prune(a, PH_FWD_2HOP, nullptr, nullptr, st); break;
     case TG_KIND_CAGRA: launch_prune(a, PH_CAGRA, nullptr, nullptr, st); break;
-    default: launch_prune(a, PH_DPG, nullptr, nullptr, st); break;
+    default:
+      if (dpg_screen) {
+        DpgScreen q{};
+        q.nrm2 = s->nrm2;
+        q.nq = p->precision == TG_PREC_FP8 ? s->nq8 : s->nq;
+        q.ne = p->precision == TG_PREC_FP8 ? s->ne8 : s->ne;
+        q.sc8 = s->sc8;
+        q.Xb = s->Xb;
+        q.Xq8 = s->Xq8;
+        q.eta = float((8.0 * s->D + 128.0) * 5.9604644775390625e-8);
+        launch_dpg_screen(a, q, p->precision == TG_PREC_FP8 ? KP_FP8 : KP_BF16, st);
+      } else {
+        launch_prune(a, PH_DPG, nullptr, nullptr, st);
+      }
+      break;
   }
   if (sec.stop()) return TG_ERR_HIP;
   if (sec.start(3)) return TG_ERR_HIP;

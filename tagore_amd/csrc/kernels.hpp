@@ -43,6 +43,20 @@ struct PruneArgs {
   const uint32_t* seg_lo;  // [nseg + 1]
 };
 
+// MFMA screen of DPG's pair distances (see prune_screen.hip): the norms of the screen bound,
+// the BF16 or FP8 copy, and the bound's relative slack
+struct DpgScreen {
+  const float* nrm2;
+  const float* nq;
+  const float* ne;
+  const float* sc8;
+  const uint16_t* Xb;
+  const uint8_t* Xq8;
+  float eta;
+};
+// precision: KP_BF16 or KP_FP8; K <= 64
+void launch_dpg_screen(const PruneArgs& a, const DpgScreen& q, int precision, hipStream_t s);
+
 // PH_EP runs one workgroup per segment: segment c starts its beam at its first row and
 // aims at centroid + c * D, writing ep[c]
 void launch_prune(const PruneArgs& a, int phase, const float* centroid, uint32_t* ep,

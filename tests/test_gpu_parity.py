@@ -100,22 +100,36 @@ def test_prune_parity_unit(unit_case, name, kind, fr, rr, th):
     _compare(X, G, GD, kind, fr, rr, 32, 64, th)
 
 
+@pytest.mark.parametrize("prec", ["bf16", "fp8"])
 @pytest.mark.parametrize("name,kind,fr,rr,th", [
     ("NSG", 0, 0, 0, 1.0),
     ("NSSG", 1, 1, 1, 0.5),
     ("DPG", 3, 0, 2, 0.0),
 ])
-def test_prune_parity_bf16_rows(int_case, name, kind, fr, rr, th):
+def test_prune_parity_bf16_rows(int_case, name, kind, fr, rr, th, prec):
     """BF16 rows are read only for data sets that BF16 holds exactly: integer data in
     [0,255] and BF16-rounded unit vectors take the BF16 path, plain unit vectors fall back
-    to FP32 rows; all three must equal the oracle bit for bit."""
+    to FP32 rows.  DPG screens its pair distances on an MFMA Gram (BF16 or FP8).  All must
+    equal the oracle bit for bit."""
     X, G, GD = int_case
-    _compare(X, G, GD, kind, fr, rr, 32, 64, th, "bf16")
+    _compare(X, G, GD, kind, fr, rr, 32, 64, th, prec)
     U = gmm_unit(2000, 96, 16, 303)
     Ub = (U.view(np.uint32) & np.uint32(0xFFFF0000)).view(np.float32)  # exact in BF16
     for Y in (U, Ub):
         GY, GDY = oracle.knn_bruteforce(Y, 48)
-        _compare(Y, GY, GDY, kind, fr, rr, 32, 64, th, "bf16")
+        _compare(Y, GY, GDY, kind, fr, rr, 32, 64, th, prec)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp8"])
+def test_dpg_parity_d960(prec):
+    """DPG and CAGRA (1-hop/rank) at dimension 960, the C5 shape, in every precision."""
+    rng = np.random.default_rng(961)
+    C = rng.standard_normal((8, 960))
+    X = np.abs(C[rng.integers(0, 8, 1200)] + rng.standard_normal((1200, 960)))
+    X = (X / X.max()).astype(np.float32)
+    G, GD = oracle.knn_bruteforce(X, 64)
+    _compare(X, G, GD, 3, 0, 2, 32, 64, 0.0, prec)
+    _compare(X, G, GD, 2, 0, 2, 32, 64, 0.0, prec)
 
 
 def test_prune_parity_small_k_and_odd_sizes():
