@@ -258,46 +258,95 @@ __device__ __forceinline__ void count_add(unsigned long long* c, int idx, uint32
 
 // -------------------------------------------------------------------------------------
 // Forward pass (Collect + Filter + Store reverse pushes).  kind 0: path, 1: two-hop.
+//
+// LDS (dynamic, sized from K, TOPM and FD so that more workgroups fit a CU):
+//   O | R : own list (pow2(K) keys) and the 480-slot ring padded to 512;
+//   B     : beam scratch M0 | M1 | S | tab | expl; once the beam is done, the merged pool
+//           (K + 480 keys) and the selected list;
+//   cl    : compaction indices (K + 480).
+// The pool is the merge of the sorted own list and the sorted ring, the same sequence a
+// sort of their union gives.
 // -------------------------------------------------------------------------------------
+constexpr int RING_P = 512;
+
+struct FwdLayout {
+  int nK, W, offB, offCl, bytes;
+};
+__host__ __device__ inline FwdLayout fwd_layout(int K, int TOPM, int FD) {
+  FwdLayout l;
+  l.nK = pow2_at_least(K);
+  l.W = TOPM + K;
+  l.offB = (l.nK + RING_P) * 8;
+  const int beam = (2 * l.W + l.nK) * 8 + HSET * 4 + EXPL_MAX * 4;
+  const int pool = (K + RING + FD + 1) * 8;
+  const int szB = ((beam > pool ? beam : pool) + 15) / 16 * 16;
+  l.offCl = l.offB + szB;
+  l.bytes = l.offCl + ((K + RING) * 2 + 15) / 16 * 16;
+  return l;
+}
+
 template <int MC, class T, int KIND>
 __global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
-  __shared__ uint64_t P[POOL_MAX];
-  __shared__ uint64_t M0[W_MAX], M1[W_MAX], S[128];
-  __shared__ uint32_t tab[HSET];
-  __shared__ uint32_t expl[EXPL_MAX];
-  __shared__ uint16_t cl[POOL_MAX];
-  __shared__ uint64_t sel[129];
+  extern __shared__ __attribute__((aligned(16))) unsigned char dsm[];
   const uint32_t u = blockIdx.x;
   if (u >= a.N) return;
   const int lane = threadIdx.x, t = lane & 15;
   const int K = a.K, D = a.D;
+  const FwdLayout ly = fwd_layout(K, int(a.TOPM), int(a.FD));
+  uint64_t* O = reinterpret_cast<uint64_t*>(dsm);
+  uint64_t* R = O + ly.nK;
+  uint64_t* M0 = reinterpret_cast<uint64_t*>(dsm + ly.offB);
+  uint64_t* M1 = M0 + ly.W;
+  uint64_t* S = M1 + ly.W;
+  uint32_t* tab = reinterpret_cast<uint32_t*>(S + ly.nK);
+  uint32_t* expl = tab + HSET;
+  uint64_t* P = reinterpret_cast<uint64_t*>(dsm + ly.offB);  // after the beam
+  const int L = K + RING;
+  uint64_t* sel = P + L;
+  uint16_t* cl = reinterpret_cast<uint16_t*>(dsm + ly.offCl);
   Row16<MC> xu;
   load_row16<MC>(xu, reinterpret_cast<const T*>(a.rows) + size_t(u) * D, D, t);
-  for (int i = lane; i < POOL_MAX; i += TG_WAVE)
-    P[i] = (i < K + RING) ? mk_key(__uint_as_float(0x7F800000u), TG_INV) : TG_KEY_INF;
+  for (int j = lane; j < ly.nK; j += TG_WAVE)
+    O[j] = j < K ? mk_key(a.GD[size_t(u) * K + j], a.G[size_t(u) * K + j]) : TG_KEY_INF;
+  for (int i = lane; i < RING_P; i += TG_WAVE)
+    R[i] = i < RING ? mk_key(__uint_as_float(0x7F800000u), TG_INV) : TG_KEY_INF;
   __syncthreads();
-  for (int j = lane; j < K; j += TG_WAVE)
-    P[j] = mk_key(a.GD[size_t(u) * K + j], a.G[size_t(u) * K + j]);
   uint32_t n_col = 0, n_adj = 0, n_flt = 0;
   if (KIND == 0) {
     uint32_t ring_c = 0;
     const uint32_t ep = a.ep[a.seg_of ? a.seg_of[u] : 0];
-    beam_search<MC, T>(a, xu, ep, 10, M0, M1, S, tab, expl, cl, P + K, ring_c, n_col, n_adj);
+    beam_search<MC, T>(a, xu, ep, 10, M0, M1, S, tab, expl, cl, R, ring_c, n_col, n_adj);
   } else {
     // two-hop ids y_i = G[G[u][i/K]][i%K], i < 480
     for (int i = lane; i < RING; i += TG_WAVE) {
       const uint32_t v = a.G[size_t(u) * K + i / K];
-      P[K + i] = mk_key(0.f, a.G[size_t(v) * K + (i % K)]);
+      R[i] = mk_key(0.f, a.G[size_t(v) * K + (i % K)]);
     }
     __syncthreads();
-    eval_batch<MC>(reinterpret_cast<const T*>(a.rows), D, xu, RING, [&](int q) { return key_id(P[K + q]); },
-                   [&](int q, float d) { P[K + q] = mk_key(d, key_id(P[K + q])); });
+    eval_batch<MC>(reinterpret_cast<const T*>(a.rows), D, xu, RING, [&](int q) { return key_id(R[q]); },
+                   [&](int q, float d) { R[q] = mk_key(d, key_id(R[q])); });
     n_col = RING;
     n_adj = (RING + K - 1) / K;
   }
   __syncthreads();
-  const int L = K + RING;
-  sort_dedup(P, pow2_at_least(L));
+  // pool = merge(sorted own list, sorted ring); entries repeating their predecessor's id
+  // get id INV
+  wave_bitonic(O, ly.nK);
+  wave_bitonic(R, RING_P);
+  merge_lds(O, K, R, RING, P);
+  __syncthreads();
+  constexpr int PQ = (128 + RING + TG_WAVE - 1) / TG_WAVE;
+  bool dup[PQ];
+#pragma unroll
+  for (int q = 0; q < PQ; ++q) {
+    const int i = q * TG_WAVE + lane;
+    dup[q] = (i < L && i > 0) ? (key_id(P[i]) == key_id(P[i - 1])) : false;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PQ; ++q)
+    if (dup[q]) P[q * TG_WAVE + lane] = key_with_id(P[q * TG_WAVE + lane], TG_INV);
+  __syncthreads();
   const int deg = filter_pool<MC, T>(a, P, L, u, a.fwd_rule, sel, cl, n_flt);
   __syncthreads();
   for (int k = lane; k < deg; k += TG_WAVE) {
@@ -542,8 +591,12 @@ static void launch_prune_mc(const PruneArgs& a, int phase, const float* centroid
   const dim3 grid(a.N), block(64);
   switch (phase) {
     case PH_EP: k_cal_ep<MC, T><<<nseg, 64, 0, s>>>(a, centroid, ep); break;
-    case PH_FWD_PATH: k_forward<MC, T, 0><<<grid, block, 0, s>>>(a); break;
-    case PH_FWD_2HOP: k_forward<MC, T, 1><<<grid, block, 0, s>>>(a); break;
+    case PH_FWD_PATH:
+      k_forward<MC, T, 0><<<grid, block, fwd_layout(a.K, a.TOPM, a.FD).bytes, s>>>(a);
+      break;
+    case PH_FWD_2HOP:
+      k_forward<MC, T, 1><<<grid, block, fwd_layout(a.K, a.TOPM, a.FD).bytes, s>>>(a);
+      break;
     case PH_REV: k_reverse<MC, T><<<grid, block, 0, s>>>(a); break;
     case PH_CAGRA: k_rank_forward<MC, T, 0><<<grid, block, 0, s>>>(a); break;
     case PH_DPG: k_rank_forward<MC, T, 1><<<grid, block, 0, s>>>(a); break;
