@@ -161,6 +161,106 @@ def bench_clusters(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_iterations(args, N, D, K, kind, ncomp, seed, precision, prunes, sweep):
+    """Iterations to recall@10 >= 0.95, learned vs random initial lists, then the prunes.
+    Each build's device window is timed; recall is the FP64 judge on a seeded sample.
+    precision "auto": the autotuner's fastest kNN variant (all give the same graph)."""
+    import tagore_amd
+
+    log(f"workload N={N} D={D} K={K} data kind {kind} precision {precision}")
+    st = tagore_amd.State(N, D, 0)
+    st.gen_data(kind, ncomp, seed, 20.0)
+    tile = 0
+    if precision == "auto":
+        (precision, tile), cands = st.autotune(K, seed)
+        log(f"autotune: {cands} -> {precision} tile {tile}")
+    st.set_profiling(True)
+    rows = {}
+    st.knn_build(K, 2, seed, precision, init="learned", tile=tile)  # warm-up
+    nq = min(args.recall_queries, N, 2000 if D > 256 else 10000)
+    for init, iters in sweep:
+        st.knn_build(K, iters, seed, precision, init=init, tile=tile)
+        s = st.stats()
+        best = s["knn_ms"]
+        rec = st.recall(nq, 10, 0x5A11)
+        rows[f"{init}_{iters}"] = {"knn_ms": best, "recall_at_10": rec,
+                                   "sections": s["knn_kernel_ms"]}
+        log(f"{init} init, {iters} iterations: {best:.1f} ms, recall@10 {rec:.4f}")
+    first = {}
+    for init in ("learned", "random"):
+        ok = [(int(k.split("_")[1]), v) for k, v in rows.items()
+              if k.startswith(init) and v["recall_at_10"] >= 0.95]
+        first[init] = min(ok, key=lambda x: x[0]) if ok else None
+    # prunes on the learned graph at its first iteration count reaching the target
+    it = first["learned"][0] if first["learned"] else max(i for n, i in sweep if n == "learned")
+    st.knn_build(K, it, seed, precision, init="learned", tile=tile)
+    knn_ms = st.stats()["knn_ms"]
+    pr = {}
+    for name, (kind_, fr, rr, th, pprecs) in prunes.items():
+        for pp in pprecs:
+            st.prune(kind_, fr, rr, 32, 64, th, pp)
+            s = st.stats()
+            pr[f"{name}_{pp}"] = {"prune_ms": s["prune_ms"], "sections": s["prune_kernel_ms"],
+                                  "counters": s["prune_counters"]}
+            log(f"prune {name} ({pp}): {s['prune_ms']:.1f} ms")
+    st.close()
+    # each prune at its fastest row precision (the results are identical)
+    best = {}
+    for k, v in pr.items():
+        n = k.rsplit("_", 1)[0]
+        best[n] = min(best.get(n, 1e30), v["prune_ms"])
+    step_ms = knn_ms + sum(best.values())
+    return rows, first, knn_ms, pr, step_ms, (precision, tile), it
+
+
+def bench_unit(args):
+    """1M x 96 unit-norm Gaussian mixture (512 components), K=64: learned vs random
+    initial lists (iterations to recall@10 >= 0.95), then NSSG (2-hop / angle 60 degrees)."""
+    N = args.n
+    rows, first, knn_ms, pr, step_ms, var, it = bench_iterations(
+        args, N, 96, 64, 1, 512, 0x7A600003, args.precision,
+        {"NSSG": ("2-hop", "angle", "angle", 0.5, ["fp32"])},
+        [("learned", 2), ("learned", 4), ("random", 2), ("random", 4), ("random", 6),
+         ("random", 8)])
+    out = {
+        "metric": "kNN+NSSG vectors/s at recall@10>=0.95, learned/BF16, 1 GPU",
+        "value": N / (step_ms / 1e3), "unit": "vectors/s", "n_gpus": 1, "steps": 1,
+        "warmup": 1, "ms_per_step": step_ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": args.precision, "data": "synthetic",
+        "config": {"workload": f"GNN-Descent K=64 + NSSG FD=32 angle 60 over {N}x96 unit "
+                               "GMM(512)", "N": N, "dim": 96, "K": 64},
+        "iterations_to_target": {k: (v[0] if v else None) for k, v in first.items()},
+        "timed_iterations": it, "knn_variant": list(var),
+        "builds": rows, "prunes": pr,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_wide(args):
+    """5M x 960 non-negative vectors (|Gaussian mixture| of 256 components scaled to [0,1]),
+    K=64, FP8 e4m3 screen with FP32 re-score, learned vs random initial lists, then DPG and
+    CFS 1-hop/rank (CAGRA) with FD=32."""
+    N = args.n if args.n != 1_000_000 else 5_000_000
+    rows, first, knn_ms, pr, step_ms, var, it = bench_iterations(
+        args, N, 960, 64, 2, 256, 0x7A600005, "auto",
+        {"DPG": ("dpg", "dist", "rank", 0.0, ["fp32", "bf16", "fp8"]),
+         "CAGRA": ("cagra", "dist", "rank", 0.0, ["fp32"])},
+        [("learned", 2), ("learned", 4), ("learned", 8), ("learned", 12), ("learned", 16),
+         ("random", 4), ("random", 8), ("random", 16)])
+    out = {
+        "metric": "kNN+DPG+CAGRA vectors/s, learned init, autotuned screen, 1 GPU",
+        "value": N / (step_ms / 1e3), "unit": "vectors/s", "n_gpus": 1, "steps": 1,
+        "warmup": 1, "ms_per_step": step_ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": var[0], "data": "synthetic",
+        "config": {"workload": f"GNN-Descent K=64 (autotuned screen) + DPG + CAGRA FD=32 over "
+                               f"{N}x960 |GMM(256)| in [0,1]", "N": N, "dim": 960, "K": 64},
+        "iterations_to_target": {k: (v[0] if v else None) for k, v in first.items()},
+        "timed_iterations": it, "knn_variant": list(var),
+        "builds": rows, "prunes": pr,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,8 +287,10 @@ def main():
     ap.add_argument("--recall-queries", type=int, default=10000)
     ap.add_argument("--cpu-sample", type=int, default=100000,
                     help="vectors in the CPU-oracle baseline sample (0 skips it)")
-    ap.add_argument("--workload", default="headline", choices=["headline", "clusters"],
-                    help="clusters: the batched multi-cluster build instead of the headline")
+    ap.add_argument("--workload", default="headline",
+                    choices=["headline", "clusters", "unit", "wide"],
+                    help="clusters: batched multi-cluster build (64 log-normal clusters); "
+                         "unit: 1M x 96 unit vectors + NSSG; wide: 5M x 960 FP8 + DPG/CAGRA")
     ap.add_argument("--clusters", type=int, default=64)
     ap.add_argument("--cluster-median", type=int, default=40000)
     ap.add_argument("--per-cluster-limit", type=int, default=64,
@@ -204,6 +306,12 @@ def main():
 
     if args.workload == "clusters":
         bench_clusters(args)
+        return
+    if args.workload == "unit":
+        bench_unit(args)
+        return
+    if args.workload == "wide":
+        bench_wide(args)
         return
 
     import tagore_amd

@@ -126,7 +126,8 @@ int tg_store_index(tg_state* s, const char* path, const char* index_type, uint64
 int tg_set_segments(tg_state* s, uint32_t nseg, const uint32_t* starts, const uint32_t* keys);
 /* Autotuner of the kNN kernels: times a 2-iteration build (one local join, one expansion)
  * of the current data for each (precision, tile) candidate -- exact, FP32-MFMA tiles 0/1,
- * BF16-MFMA tiles 0/1 (BF16 when D % 8 == 0) -- and returns the fastest.  Every candidate
+ * BF16-MFMA tiles 0/1 (D % 8 == 0), FP8-MFMA tiles 0/1 (D % 16 == 0) -- and returns the
+ * fastest.  Every candidate
  * gives the same graph, so the choice only affects time.  Leaves a 2-iteration graph. */
 typedef struct {
   int32_t precision;          /* fastest candidate */

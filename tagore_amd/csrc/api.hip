@@ -147,6 +147,7 @@ struct tg_state {
   uint32_t* samp = nullptr;    // sample ids
   uint32_t samp_cap = 0;
   bool has_codes = false;
+  bool order_valid = false;    // learn.order (bucket order) belongs to the current graph
   // stats
   bool profiling = false;
   tg_stats stats{};
@@ -458,6 +459,7 @@ int tg_set_data(tg_state* s, const float* rows) {
   s->prep_valid = false;
   s->prep8_valid = false;
   s->has_prune = false;
+  s->order_valid = false;
   return TG_OK;
 }
 
@@ -477,6 +479,7 @@ int tg_gen_data(tg_state* s, int32_t kind, uint32_t ncomp, uint64_t seed, float 
   s->prep_valid = false;
   s->prep8_valid = false;
   s->has_prune = false;
+  s->order_valid = false;
   return TG_OK;
 }
 
@@ -740,8 +743,11 @@ int tg_knn_build(tg_state* s, const tg_knn_params* p) {
   const uint16_t* rows16 =
       ((p->precision == TG_PREC_BF16 || p->precision == TG_PREC_FP8) && s->cert_bf16) ? s->Xb
                                                                                       : nullptr;
+  bool use_order = false;  // after a learned init: process rows in bucket order
+  s->order_valid = false;
   auto args = [&]() {
     KnnArgs k = knn_args(s, p->seed);
+    if (use_order) k.perm = s->learn.order;
     if (s->nseg) {
       k.seg_of = s->seg_of;
       k.seg_lo = s->seg_lo;
@@ -767,8 +773,11 @@ int tg_knn_build(tg_state* s, const tg_knn_params* p) {
   if (p->init == TG_INIT_RANDOM) {
     launch_knn_init(a, st);
   } else {
-    if (p->init == TG_INIT_LEARNED)
+    if (p->init == TG_INIT_LEARNED) {
       if (int rc = learned_init(s, p, &extra_evals)) return rc;
+      use_order = true;
+      a = args();
+    }
     launch_knn_init_sort(a, st);
   }
   if (sec.stop()) return TG_ERR_HIP;
@@ -839,6 +848,7 @@ int tg_knn_build(tg_state* s, const tg_knn_params* p) {
     return TG_ERR_CAPACITY;
   s->has_knn = true;
   s->has_prune = false;
+  s->order_valid = use_order;
   return TG_OK;
 }
 
@@ -851,6 +861,7 @@ int tg_set_knn(tg_state* s, uint32_t K, const uint32_t* ids, const float* dists)
   if (ensure_graph(s, K)) return TG_ERR_HIP;
   HIPX(hipMemcpy(s->G, ids, size_t(s->N) * K * 4, hipMemcpyHostToDevice));
   HIPX(hipMemcpy(s->GD, dists, size_t(s->N) * K * 4, hipMemcpyHostToDevice));
+  s->order_valid = false;
   s->has_knn = true;
   s->has_prune = false;
   return TG_OK;
@@ -915,6 +926,7 @@ int tg_prune(tg_state* s, const tg_prune_params* p) {
   a.fwd_rule = p->fwd_rule;
   a.rev_rule = p->rev_rule;
   a.ep = s->ep_dev;
+  if (s->order_valid && !s->nseg) a.perm = s->learn.order;
   if (s->nseg) {
     a.ep = s->seg_ep;
     a.seg_of = s->seg_of;
@@ -1082,15 +1094,17 @@ int tg_set_segments(tg_state* s, uint32_t nseg, const uint32_t* starts, const ui
   }
   s->has_knn = false;
   s->has_prune = false;
+  s->order_valid = false;
   return TG_OK;
 }
 
 int tg_autotune_knn(tg_state* s, uint32_t K, uint64_t seed, tg_tune_result* out) {
   if (!s || !out) return fail(TG_ERR_ARG, "null argument");
   *out = tg_tune_result{};
-  const bool bf16 = s->D % 8 == 0;
-  const int32_t precs[5] = {TG_PREC_EXACT, TG_PREC_FP32, TG_PREC_FP32, TG_PREC_BF16, TG_PREC_BF16};
-  const uint32_t tiles[5] = {0, 0, 1, 0, 1};
+  const bool bf16 = s->D % 8 == 0, fp8 = s->D % 16 == 0;
+  const int32_t precs[7] = {TG_PREC_EXACT, TG_PREC_FP32, TG_PREC_FP32, TG_PREC_BF16,
+                            TG_PREC_BF16, TG_PREC_FP8, TG_PREC_FP8};
+  const uint32_t tiles[7] = {0, 0, 1, 0, 1, 0, 1};
   const bool prof = s->profiling;
   s->profiling = false;
   tg_knn_params kp{};
@@ -1105,8 +1119,9 @@ int tg_autotune_knn(tg_state* s, uint32_t K, uint64_t seed, tg_tune_result* out)
     return rc;
   }
   double best = 1e300;
-  for (int c = 0; c < 5; ++c) {
+  for (int c = 0; c < 7; ++c) {
     if (precs[c] == TG_PREC_BF16 && !bf16) continue;
+    if (precs[c] == TG_PREC_FP8 && !fp8) continue;
     kp.precision = precs[c];
     kp.tile = tiles[c];
     if (int rc = tg_knn_build(s, &kp)) {

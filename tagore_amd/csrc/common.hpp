@@ -69,6 +69,21 @@ __device__ __forceinline__ uint32_t sm_bounded(uint64_t x, uint32_t n) {
   return uint32_t(((x >> 32) * uint64_t(n)) >> 32);
 }
 
+// Row handled by this workgroup.  With an order (perm: a permutation of the rows that puts
+// neighbouring rows together), workgroup b takes position (b % 8) * ceil(N/8) + b / 8 of
+// it: workgroups are dealt to the 8 XCDs round-robin, so each XCD walks one contiguous run
+// of the order and its L2 keeps the rows its neighbours gather.  Rows past N: 0xFFFFFFFF.
+__device__ __forceinline__ uint32_t row_of_block(const uint32_t* perm, uint32_t N) {
+  const uint32_t b = blockIdx.x;
+  if (!perm) return b;
+  const uint32_t chunk = (N + 7) / 8;
+  const uint32_t pos = (b & 7) * chunk + (b >> 3);
+  return pos < N ? perm[pos] : 0xFFFFFFFFu;
+}
+__host__ __device__ inline uint32_t grid_rows(const uint32_t* perm, uint32_t N) {
+  return perm ? 8 * ((N + 7) / 8) : N;
+}
+
 // seed of segment c of a segmented handle (host and device)
 __host__ __device__ inline uint64_t segment_seed(uint64_t seed, uint32_t c) {
   return seed ^ ((uint64_t(c) + 1) * 0xA0761D6478BD642Full);

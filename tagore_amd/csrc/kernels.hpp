@@ -41,6 +41,7 @@ struct PruneArgs {
   // segments: rows [seg_lo[c], seg_lo[c+1]) form independent data sets (null: one segment)
   const uint32_t* seg_of;  // [N] segment of each row
   const uint32_t* seg_lo;  // [nseg + 1]
+  const uint32_t* perm;    // processing order of the rows (null: 0..N-1)
 };
 
 // MFMA screen of DPG's pair distances (see prune_screen.hip): the norms of the screen bound,
@@ -109,6 +110,7 @@ struct KnnArgs {
   const uint32_t* seg_of;
   const uint32_t* seg_lo;
   const uint32_t* seg_key;  // [nseg] seed key of each segment
+  const uint32_t* perm;     // processing order of the gather-heavy kernels (null: 0..N-1)
 };
 
 // screen-mode preparation: BF16 copy (if Xb), norms, quantisation-error norms and the

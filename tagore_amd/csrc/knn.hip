@@ -67,7 +67,7 @@ __global__ __launch_bounds__(64) void k_init_ids(KnnArgs a) {
 template <int MC>
 __global__ __launch_bounds__(64) void k_init_sort(KnnArgs a) {
   __shared__ uint64_t kk[128];
-  const uint32_t u = blockIdx.x;
+  const uint32_t u = row_of_block(a.perm, a.N);
   if (u >= a.N) return;
   const int lane = threadIdx.x, g = lane >> 4, t = lane & 15;
   const int K = a.K, D = a.D, n = pow2_at_least(K);
@@ -161,7 +161,7 @@ __global__ __launch_bounds__(256) void k_localjoin_exact(KnnArgs a) {
   __shared__ uint32_t A[64], B[64];
   __shared__ float wA[64], wB[64];
   __shared__ float MAB[64 * 64], MAA[64 * 64];
-  const uint32_t u = blockIdx.x;
+  const uint32_t u = row_of_block(a.perm, a.N);
   if (u >= a.N) return;
   const int tid = threadIdx.x, gg = tid >> 4, t = tid & 15;
   const int K = a.K, D = a.D;
@@ -454,7 +454,7 @@ __global__ __launch_bounds__(64) void k_phase2(KnnArgs a) {
   __shared__ uint16_t cl[512];
   __shared__ uint64_t kv[512];
   __shared__ uint32_t lead[LEADERS];
-  const uint32_t u = blockIdx.x;
+  const uint32_t u = row_of_block(a.perm, a.N);
   if (u >= a.N) return;
   const int lane = threadIdx.x, g = lane >> 4, t = lane & 15;
   const int K = a.K, D = a.D;
@@ -636,11 +636,11 @@ __global__ __launch_bounds__(64) void k_final(KnnArgs a) {
   } while (0)
 
 void launch_knn_init_sort(const KnnArgs& a, hipStream_t s) {
-  TG_MC_DISPATCH(k_init_sort, a.N, 64, s, (a));
+  TG_MC_DISPATCH(k_init_sort, grid_rows(a.perm, a.N), 64, s, (a));
 }
 void launch_knn_init(const KnnArgs& a, hipStream_t s) {
   k_init_ids<<<a.N, 64, 0, s>>>(a);
-  TG_MC_DISPATCH(k_init_sort, a.N, 64, s, (a));
+  TG_MC_DISPATCH(k_init_sort, grid_rows(a.perm, a.N), 64, s, (a));
 }
 void launch_knn_sample(const KnnArgs& a, hipStream_t s) { k_sample<<<a.N, 64, 0, s>>>(a); }
 void launch_knn_rsample(const KnnArgs& a, hipStream_t s) { k_rsample<<<a.N, 64, 0, s>>>(a); }
@@ -648,7 +648,7 @@ void launch_knn_localjoin_screen(const KnnArgs& a, int precision, int tile, hipS
 
 void launch_knn_localjoin(const KnnArgs& a, int precision, int tile, hipStream_t s) {
   if (precision == KP_EXACT)
-    TG_MC_DISPATCH(k_localjoin_exact, a.N, 256, s, (a));
+    TG_MC_DISPATCH(k_localjoin_exact, grid_rows(a.perm, a.N), 256, s, (a));
   else
     launch_knn_localjoin_screen(a, precision, tile, s);
 }
@@ -657,7 +657,7 @@ void launch_knn_rev_build(const KnnArgs& a, hipStream_t s) { k_rev_build<<<a.N, 
 void launch_knn_rev_reduce(const KnnArgs& a, hipStream_t s) { k_rev_reduce<<<a.N, 64, 0, s>>>(a); }
 void launch_knn_phase2(const KnnArgs& a, int precision, hipStream_t s) {
   const int mc = (int(a.D) + 63) / 64;
-#define P2(M, P, T) k_phase2<M, P, T><<<a.N, 64, 0, s>>>(a)
+#define P2(M, P, T) k_phase2<M, P, T><<<grid_rows(a.perm, a.N), 64, 0, s>>>(a)
 #define P2_MC(P, T)                                                  \
   do {                                                               \
     if (mc <= 2) P2(2, P, T); else if (mc <= 4) P2(4, P, T); else P2(16, P, T); \

@@ -143,7 +143,7 @@ __global__ __launch_bounds__(256) void k_localjoin_screen(KnnArgs a) {
   float* MAB = reinterpret_cast<float*>(smem);
   float* MAA = MAB + 64 * MROW;
 
-  const uint32_t u = blockIdx.x;
+  const uint32_t u = row_of_block(a.perm, a.N);
   if (u >= a.N) return;
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
   const int K = a.K, D = a.D;
@@ -401,7 +401,8 @@ __global__ __launch_bounds__(256) void k_localjoin_screen(KnnArgs a) {
 // the depth, more LDS rounds
 void launch_knn_localjoin_screen(const KnnArgs& a, int precision, int tile, hipStream_t s) {
   const int mc = (int(a.D) + 63) / 64;
-#define LJS(P, M, T, DC) k_localjoin_screen<P, M, T, DC><<<a.N, 256, 0, s>>>(a)
+#define LJS(P, M, T, DC) \
+  k_localjoin_screen<P, M, T, DC><<<grid_rows(a.perm, a.N), 256, 0, s>>>(a)
 #define LJS_DC(P, M, T)                                                    \
   do {                                                                     \
     if (tile == 1) LJS(P, M, T, (P == 3 ? 64 : P == 2 ? 32 : 16));        \

@@ -288,7 +288,7 @@ __host__ __device__ inline FwdLayout fwd_layout(int K, int TOPM, int FD) {
 template <int MC, class T, int KIND>
 __global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
   extern __shared__ __attribute__((aligned(16))) unsigned char dsm[];
-  const uint32_t u = blockIdx.x;
+  const uint32_t u = row_of_block(a.perm, a.N);
   if (u >= a.N) return;
   const int lane = threadIdx.x, t = lane & 15;
   const int K = a.K, D = a.D;
@@ -371,7 +371,7 @@ __global__ __launch_bounds__(64) void k_reverse(PruneArgs a) {
   __shared__ uint64_t P[256];
   __shared__ uint16_t cl[256];
   __shared__ uint64_t sel[129];
-  const uint32_t u = blockIdx.x;
+  const uint32_t u = row_of_block(a.perm, a.N);
   if (u >= a.N) return;
   const int lane = threadIdx.x;
   const int K = a.K, FD = a.FD;
@@ -435,7 +435,7 @@ __global__ __launch_bounds__(64) void k_rank_forward(PruneArgs a) {
   __shared__ uint64_t srt[256];   // (id << 32 | pos) sorted, then (detour << 32 | id)
   __shared__ uint32_t det[128];
   __shared__ uint32_t nei[128];
-  const uint32_t u = blockIdx.x;
+  const uint32_t u = row_of_block(a.perm, a.N);
   if (u >= a.N) return;
   const int lane = threadIdx.x, g = lane >> 4, t = lane & 15;
   const int K = a.K, D = a.D, d = a.FD;
@@ -588,7 +588,7 @@ __global__ void k_centroid_final(const double* part, uint32_t nch, uint32_t N, u
 template <int MC, class T>
 static void launch_prune_mc(const PruneArgs& a, int phase, const float* centroid, uint32_t* ep,
                             hipStream_t s, uint32_t nseg) {
-  const dim3 grid(a.N), block(64);
+  const dim3 grid(grid_rows(a.perm, a.N)), block(64);
   switch (phase) {
     case PH_EP: k_cal_ep<MC, T><<<nseg, 64, 0, s>>>(a, centroid, ep); break;
     case PH_FWD_PATH:

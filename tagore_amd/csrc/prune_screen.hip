@@ -41,7 +41,7 @@ __global__ __launch_bounds__(256) void k_dpg_screen(PruneArgs a, DpgScreen q) {
   __shared__ uint64_t srt[64];
   float* M = reinterpret_cast<float*>(smem);
 
-  const uint32_t u = blockIdx.x;
+  const uint32_t u = row_of_block(a.perm, a.N);
   if (u >= a.N) return;
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
   const int K = a.K, D = a.D, d = a.FD;
@@ -175,7 +175,7 @@ __global__ __launch_bounds__(256) void k_dpg_screen(PruneArgs a, DpgScreen q) {
 
 void launch_dpg_screen(const PruneArgs& a, const DpgScreen& q, int precision, hipStream_t s) {
   const int mc = (int(a.D) + 63) / 64;
-#define DS(P, M, T) k_dpg_screen<P, M, T><<<a.N, 256, 0, s>>>(a, q)
+#define DS(P, M, T) k_dpg_screen<P, M, T><<<grid_rows(a.perm, a.N), 256, 0, s>>>(a, q)
 #define DS_MC(P, T)                                                    \
   do {                                                                 \
     if (mc <= 2) DS(P, 2, T); else if (mc <= 4) DS(P, 4, T); else DS(P, 16, T); \

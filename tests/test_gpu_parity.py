@@ -271,7 +271,7 @@ def test_knn_tiles_and_autotune(precision):
         assert np.array_equal(G, Gr) and np.array_equal(GD, GDr)
         (bp, bt), cands = s.autotune(K, seed)
         _say(f"autotune: best {bp} tile {bt}; {cands}")
-        assert len(cands) == 5 and (bp, bt) in [(c[0], c[1]) for c in cands]
+        assert len(cands) == 7 and (bp, bt) in [(c[0], c[1]) for c in cands]
         assert min(c[2] for c in cands) == [c[2] for c in cands if (c[0], c[1]) == (bp, bt)][0]
 
 
