@@ -324,11 +324,12 @@ __global__ __launch_bounds__(64) void k_merge(KnnArgs a) {
 }
 
 // ---------------------------------------------------------------- reverse build
-__global__ __launch_bounds__(64) void k_rev_build(KnnArgs a) {
-  const uint32_t u = blockIdx.x;
-  if (u >= a.N) return;
-  for (int j = threadIdx.x; j < int(a.K); j += TG_WAVE)
-    pb_push(a.pr, a.G[size_t(u) * a.K + j], mk_key(a.GD[size_t(u) * a.K + j], u));
+// one thread per list entry (u, j): push (d, u) to G[u][j]; a grid-stride loop over all
+// N*K entries keeps many pushes in flight per wave
+__global__ __launch_bounds__(256) void k_rev_build(KnnArgs a) {
+  const size_t n = size_t(a.N) * a.K;
+  for (size_t e = size_t(blockIdx.x) * 256 + threadIdx.x; e < n; e += size_t(gridDim.x) * 256)
+    pb_push(a.pr, a.G[e], mk_key(a.GD[e], uint32_t(e / a.K)));
 }
 
 __global__ __launch_bounds__(64) void k_rev_reduce(KnnArgs a) {
@@ -653,7 +654,11 @@ void launch_knn_localjoin(const KnnArgs& a, int precision, int tile, hipStream_t
     launch_knn_localjoin_screen(a, precision, tile, s);
 }
 void launch_knn_merge(const KnnArgs& a, hipStream_t s) { k_merge<<<a.N, 64, 0, s>>>(a); }
-void launch_knn_rev_build(const KnnArgs& a, hipStream_t s) { k_rev_build<<<a.N, 64, 0, s>>>(a); }
+void launch_knn_rev_build(const KnnArgs& a, hipStream_t s) {
+  const size_t n = size_t(a.N) * a.K;
+  const uint32_t grid = uint32_t(std::min<size_t>((n + 255) / 256, 256 * 64));
+  k_rev_build<<<grid, 256, 0, s>>>(a);
+}
 void launch_knn_rev_reduce(const KnnArgs& a, hipStream_t s) { k_rev_reduce<<<a.N, 64, 0, s>>>(a); }
 void launch_knn_phase2(const KnnArgs& a, int precision, hipStream_t s) {
   const int mc = (int(a.D) + 63) / 64;
