@@ -63,7 +63,13 @@ typedef struct {
   uint32_t epochs;    /* learned init: training epochs (0: 5) */
   uint32_t tile;      /* local-join MFMA staging depth: 0 = 256-byte rows per LDS round,
                          1 = 64-byte rows (results are identical; see tg_autotune_knn) */
+  uint32_t flags;     /* TG_FLAG_*: 0 for the defaults */
 } tg_knn_params;
+/* learned init: capture the encoder's training loop (about 2,200 small kernels) as a HIP
+ * graph and replay it in later builds.  Same result as the default individual launches;
+ * measured slower on MI355X with ROCm 7.2 (26.7 ms vs 21.7 ms per 1M-vector build), so
+ * off by default. */
+#define TG_FLAG_GRAPH 1u
 
 typedef struct {
   int32_t kind;      /* TG_KIND_* */
@@ -196,7 +202,9 @@ typedef struct {
   uint64_t knn_evals;        /* distance evaluations of the last build */
   uint64_t knn_rescored;     /* canonical FP32 re-scores after the screen (phase 2 + join) */
   uint64_t prune_counters[8];/* collect, filter, reverse, adjacency rows, ep evaluations,
-                                exact re-scores of screened DPG pairs */
+                                exact re-scores of screened DPG pairs; with profiling on,
+                                summed shader-clock cycles of the path / two-hop forward
+                                kernel's collect (6) and pool + filter (7) phases */
   double learn_ms[5];        /* learned init: sample graph, training, encoding, buckets, seeding
                                 (filled when profiling is on) */
   float learn_loss[2];       /* training loss at the end of the first and the last epoch */

@@ -50,7 +50,7 @@ class KnnParams(ctypes.Structure):
     _fields_ = [("K", ctypes.c_uint32), ("iters", ctypes.c_uint32), ("seed", ctypes.c_uint64),
                 ("precision", ctypes.c_int32), ("init", ctypes.c_int32),
                 ("sample", ctypes.c_uint32), ("epochs", ctypes.c_uint32),
-                ("tile", ctypes.c_uint32)]
+                ("tile", ctypes.c_uint32), ("flags", ctypes.c_uint32)]
 
 
 class PruneParams(ctypes.Structure):
@@ -86,7 +86,7 @@ class LargeStats(ctypes.Structure):
     _fields_ = [("clusters", ctypes.c_uint32), ("batches", ctypes.c_uint32),
                 ("build_ms", ctypes.c_double), ("merge_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double), ("precision", ctypes.c_int32),
-                ("tile", ctypes.c_uint32)]
+                ("tile", ctypes.c_uint32), ("flags", ctypes.c_uint32)]
 
 
 class TuneResult(ctypes.Structure):
@@ -208,8 +208,10 @@ class State:
 
     # kNN
     def knn_build(self, K: int, iters: int, seed: int = 0x7A600000, precision: str = "exact",
-                  init: str = "random", sample: int = 0, epochs: int = 0, tile: int = 0):
-        p = KnnParams(K, iters, seed, PREC[precision], INIT[init], sample, epochs, tile)
+                  init: str = "random", sample: int = 0, epochs: int = 0, tile: int = 0,
+                  graph: bool = False):
+        p = KnnParams(K, iters, seed, PREC[precision], INIT[init], sample, epochs, tile,
+                      1 if graph else 0)
         _check(lib().tg_knn_build(self._h, ctypes.byref(p)))
         self.K = K
 
@@ -296,7 +298,8 @@ class State:
                     prune_kernel_ms=dict(zip(PRUNE_SECTIONS, list(s.prune_kernel_ms))),
                     knn_evals=int(s.knn_evals), knn_rescored=int(s.knn_rescored),
                     prune_counters=dict(zip(["collect", "filter", "reverse", "adj_rows", "ep",
-                                             "rescored"], list(s.prune_counters)[:6])),
+                                             "rescored", "collect_cycles", "filter_cycles"],
+                                            list(s.prune_counters))),
                     learn_ms=dict(zip(["sample_graph", "train", "encode", "buckets", "seed"],
                                       list(s.learn_ms))),
                     learn_loss=list(s.learn_loss), learn_buckets=int(s.learn_buckets),

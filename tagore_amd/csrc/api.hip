@@ -674,7 +674,8 @@ int learned_init(tg_state* s, const tg_knn_params* p, uint64_t* evals) {
   if (p->epochs) cfg.epochs = p->epochs;
   LearnStats ls;
   const hipError_t e = learned_seed(s->learn, s->X, N, D, K, s->sub->X, s->samp, S, s->sub->G,
-                                    Ks, cfg, p->seed, s->G, st, &ls, s->profiling);
+                                    Ks, cfg, p->seed, s->G, st, &ls, s->profiling,
+                                    (p->flags & TG_FLAG_GRAPH) != 0);
   if (e != hipSuccess)
     return fail(TG_ERR_HIP, std::string("learned init: ") + hipGetErrorString(e));
   s->stats.learn_ms[1] = ls.ms_train;
@@ -939,6 +940,7 @@ int tg_prune(tg_state* s, const tg_prune_params* p) {
   a.out_ids = s->out_ids;
   a.rev = s->rev.d;
   a.counters = s->counters;
+  a.phase_clock = s->profiling ? 1 : 0;
   a.rows = s->X;
   a.rows_bf16 = 0;
   if (p->precision == TG_PREC_BF16 || p->precision == TG_PREC_FP8) {

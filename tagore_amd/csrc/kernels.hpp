@@ -42,6 +42,7 @@ struct PruneArgs {
   const uint32_t* seg_of;  // [N] segment of each row
   const uint32_t* seg_lo;  // [nseg + 1]
   const uint32_t* perm;    // processing order of the rows (null: 0..N-1)
+  int phase_clock;         // 1: k_forward adds its collect / filter cycles to counters[6], [7]
 };
 
 // MFMA screen of DPG's pair distances (see prune_screen.hip): the norms of the screen bound,

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "kernels.hpp"
@@ -156,13 +157,16 @@ __device__ __forceinline__ float wave_sum(float v) {
   return v;
 }
 
-// Contrastive loss and its gradient with respect to the codes (one wave per anchor b).
-__global__ __launch_bounds__(64) void k_mlp_loss(const float* __restrict__ Z, uint32_t B,
-                                                 uint32_t P, float margin, uint32_t M,
-                                                 uint16_t* dZb, uint16_t* dZT, float* lossp) {
+// Contrastive loss and its gradient with respect to the codes: one workgroup of four waves
+// per anchor b; each wave takes every fourth negative, the wave partials are added in wave
+// order (deterministic).
+__global__ __launch_bounds__(256) void k_mlp_loss(const float* __restrict__ Z, uint32_t B,
+                                                  uint32_t P, float margin, uint32_t M,
+                                                  uint16_t* dZb, uint16_t* dZT, float* lossp) {
+  __shared__ float red[4][MLP_CODE + 2];
   const uint32_t b = blockIdx.x;
   if (b >= B) return;
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   float za[MLP_CODE], acc[MLP_CODE];
 #pragma unroll
   for (int k = 0; k < int(MLP_CODE); ++k) {
@@ -170,7 +174,7 @@ __global__ __launch_bounds__(64) void k_mlp_loss(const float* __restrict__ Z, ui
     acc[k] = 0.f;
   }
   float nact = 0.f, lneg = 0.f;
-  for (uint32_t c = lane; c < B; c += TG_WAVE) {
+  for (uint32_t c = tid; c < B; c += 256) {
     if (c == b) continue;
     const float* zc = Z + size_t(c) * MLP_CODE;
     float d2 = 0.f;
@@ -188,14 +192,22 @@ __global__ __launch_bounds__(64) void k_mlp_loss(const float* __restrict__ Z, ui
   }
   nact = wave_sum(nact);
   lneg = wave_sum(lneg);
-  float accs = 0.f;
 #pragma unroll
   for (int k = 0; k < int(MLP_CODE); ++k) {
     const float v = wave_sum(acc[k]);
-    if (lane == k) accs = v;
+    if (lane == k) red[wave][2 + k] = v;
   }
-  const float cpos = 2.f / float(B * P), cneg = 4.f / (float(B) * float(B - 1));
+  if (lane == 0) {
+    red[wave][0] = nact;
+    red[wave][1] = lneg;
+  }
+  __syncthreads();
+  if (wave != 0) return;
   const int k = lane & 31;
+  nact = ((red[0][0] + red[1][0]) + red[2][0]) + red[3][0];
+  lneg = ((red[0][1] + red[1][1]) + red[2][1]) + red[3][1];
+  const float accs = ((red[0][2 + k] + red[1][2 + k]) + red[2][2 + k]) + red[3][2 + k];
+  const float cpos = 2.f / float(B * P), cneg = 4.f / (float(B) * float(B - 1));
   float zak = 0.f;
 #pragma unroll
   for (int q = 0; q < int(MLP_CODE); ++q)
@@ -550,6 +562,14 @@ __global__ __launch_bounds__(64) void k_seed_lists(const uint32_t* __restrict__ 
   for (int j = lane; j < int(K); j += TG_WAVE) G[size_t(u) * K + j] = acc[j];
 }
 
+// loss of the batch: fixed-order sum of the per-anchor partials (one thread)
+__global__ void k_loss_sum(const float* lossp, uint32_t B, float* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s = 0.0;
+  for (uint32_t b = 0; b < B; ++b) s += lossp[b];
+  *out = float(s);
+}
+
 __global__ void k_fill_u16(uint16_t* p, uint16_t v, size_t n) {
   const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -651,6 +671,7 @@ hipError_t learner_reserve(Learner& L, uint32_t D, uint32_t M, uint32_t N, uint3
     LX(rmalloc(&L.Z, size_t(Mb) * MLP_CODE));
     LX(rmalloc(&L.lossp, Mb));
     L.Mcap = Mb;
+    L.ones_M = 0;
   }
   if (L.Ncap < N) {
     LX(rmalloc(&L.codes, size_t(N) * MLP_CODE));
@@ -677,6 +698,8 @@ hipError_t learner_reserve(Learner& L, uint32_t D, uint32_t M, uint32_t N, uint3
 }
 
 void learner_release(Learner& L) {
+  if (L.train_exec) (void)hipGraphExecDestroy(L.train_exec);
+  rfree(L.loss_hist);
   rfree(L.prm);
   rfree(L.mom);
   rfree(L.vel);
@@ -740,9 +763,12 @@ void launch_mlp_adam(Learner& L, int mode, uint32_t t, const LearnConfig& c, flo
 void mlp_forward_backward(Learner& L, const float* X, const uint32_t* rows, uint32_t B,
                           uint32_t P, float margin, hipStream_t st) {
   const uint32_t D = L.D, M = B * (1 + P);
-  // row of ones closing Xn^T and H^T (bias gradients)
-  k_fill_u16<<<(M + 255) / 256, 256, 0, st>>>(L.XnT + size_t(D) * M, 0x3F80, M);
-  k_fill_u16<<<(M + 255) / 256, 256, 0, st>>>(L.HT + size_t(MLP_HID) * M, 0x3F80, M);
+  // row of ones closing Xn^T and H^T (bias gradients); unchanged while M is
+  if (L.ones_M != M) {
+    k_fill_u16<<<(M + 255) / 256, 256, 0, st>>>(L.XnT + size_t(D) * M, 0x3F80, M);
+    k_fill_u16<<<(M + 255) / 256, 256, 0, st>>>(L.HT + size_t(MLP_HID) * M, 0x3F80, M);
+    L.ones_M = M;
+  }
   k_mlp_gather<<<dim3((M + 63) / 64, (D + 63) / 64), 256, 0, st>>>(X, D, rows, 0, M, L.mu, L.scal,
                                                                    L.Xn, L.XnT, M);
   GemmArgs g{};
@@ -756,7 +782,7 @@ void mlp_forward_backward(Learner& L, const float* X, const uint32_t* rows, uint
   g.Kd = MLP_HID; g.Cf = L.Z; g.ldc = MLP_CODE; g.bias = L.prm + mlp_n1(D) + MLP_HID;
   g.bias_stride = MLP_HID + 1;
   gemm<EPI_Z>(g, 1, st);
-  k_mlp_loss<<<B, 64, 0, st>>>(L.Z, B, P, margin, M, L.dZb, L.dZT, L.lossp);
+  k_mlp_loss<<<B, 256, 0, st>>>(L.Z, B, P, margin, M, L.dZb, L.dZT, L.lossp);
   const uint32_t kt = M / GK;
   const uint32_t splits = std::min(L.splits, std::max(1u, kt));
   // unused split slots must read as zero in the reduction
@@ -803,7 +829,7 @@ hipError_t learned_seed(Learner& L, const float* X, uint32_t N, uint32_t D, uint
                         const float* Xs, const uint32_t* samp, uint32_t S, const uint32_t* subG,
                         uint32_t Ks,
                         const LearnConfig& c, uint64_t seed, uint32_t* G, hipStream_t st,
-                        LearnStats* stats, bool timed) {
+                        LearnStats* stats, bool timed, bool use_graph) {
   using clk = std::chrono::steady_clock;
   auto now_ms = [&](clk::time_point t0) -> double {
     if (!timed) return 0.0;
@@ -832,30 +858,70 @@ hipError_t learned_seed(Learner& L, const float* X, uint32_t N, uint32_t D, uint
   LX(mlp_set_params(L, h.data(), st));
   LX(hipStreamSynchronize(st));  // h must outlive the asynchronous upload
   auto tt = clk::now();
-  // training
+  // training: every epoch visits the sample in the order of a seeded affine permutation
   const uint32_t steps = S / B;
-  std::vector<float> lp(B);
-  uint32_t t = 0;
+  std::vector<uint64_t> perm_a(c.epochs), perm_c(c.epochs);
   for (uint32_t ep = 0; ep < c.epochs; ++ep) {
     uint64_t a = (host_splitmix(rs) % S) | 1, pc = host_splitmix(rs) % S;
     while (gcd64(a, S) != 1) a = (a + 2) % S == 0 ? 1 : (a + 2) % S;
-    for (uint32_t sidx = 0; sidx < steps; ++sidx) {
-      k_mlp_rows<<<(B + 255) / 256, 256, 0, st>>>(L.rows, samp, subG, Ks, S, B, P, a, pc, sidx);
-      mlp_forward_backward(L, X, L.rows, B, P, c.margin, st);
-      if (stats && (ep == 0 || ep + 1 == c.epochs) && sidx + 1 == steps) {
-        LX(hipMemcpyAsync(lp.data(), L.lossp, B * 4, hipMemcpyDeviceToHost, st));
-        LX(hipStreamSynchronize(st));
-        double sum = 0;
-        for (float v : lp) sum += v;
-        if (ep == 0) stats->loss_first = float(sum);
-        stats->loss_last = float(sum);
+    perm_a[ep] = a;
+    perm_c[ep] = pc;
+  }
+  if (!L.loss_hist) LX(rmalloc(&L.loss_hist, 2));
+  auto run_training = [&]() {
+    uint32_t t = 0;
+    for (uint32_t ep = 0; ep < c.epochs; ++ep) {
+      for (uint32_t sidx = 0; sidx < steps; ++sidx) {
+        k_mlp_rows<<<(B + 255) / 256, 256, 0, st>>>(L.rows, samp, subG, Ks, S, B, P,
+                                                   perm_a[ep], perm_c[ep], sidx);
+        mlp_forward_backward(L, X, L.rows, B, P, c.margin, st);
+        if (sidx + 1 == steps && (ep == 0 || ep + 1 == c.epochs))
+          k_loss_sum<<<1, 64, 0, st>>>(L.lossp, B, L.loss_hist + (ep == 0 ? 0 : 1));
+        launch_mlp_adam(L, 2, ++t, c, nullptr, st);
       }
-      launch_mlp_adam(L, 2, ++t, c, nullptr, st);
     }
+    return t;
+  };
+  uint32_t nsteps = steps * c.epochs;
+  if (use_graph) {
+    // one capture per (buffers, shapes, seed); later builds replay it
+    uint64_t key = 0x6A09E667F3BCC909ull;
+    auto mix = [&](uint64_t v) { key = (key ^ v) * 0x100000001B3ull; };
+    mix(uint64_t(reinterpret_cast<uintptr_t>(X)));
+    mix(uint64_t(reinterpret_cast<uintptr_t>(samp)));
+    mix(uint64_t(reinterpret_cast<uintptr_t>(subG)));
+    mix(uint64_t(reinterpret_cast<uintptr_t>(L.rows)));
+    mix(uint64_t(reinterpret_cast<uintptr_t>(L.prm)));
+    mix(uint64_t(reinterpret_cast<uintptr_t>(L.gpart)));
+    mix(N); mix(D); mix(S); mix(Ks); mix(B); mix(P); mix(c.epochs); mix(seed);
+    uint32_t lrb, mb;
+    std::memcpy(&lrb, &c.lr, 4);
+    std::memcpy(&mb, &c.margin, 4);
+    mix(lrb); mix(mb);
+    if (!L.train_exec || L.train_key != key) {
+      if (L.train_exec) (void)hipGraphExecDestroy(L.train_exec);
+      L.train_exec = nullptr;
+      hipGraph_t g = nullptr;
+      LX(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      run_training();
+      LX(hipStreamEndCapture(st, &g));
+      const hipError_t e = hipGraphInstantiate(&L.train_exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      LX(e);
+      L.train_key = key;
+    }
+    LX(hipGraphLaunch(L.train_exec, st));
+  } else {
+    nsteps = run_training();
   }
   if (stats) {
+    float lh[2] = {0.f, 0.f};
+    LX(hipMemcpyAsync(lh, L.loss_hist, sizeof(lh), hipMemcpyDeviceToHost, st));
+    LX(hipStreamSynchronize(st));
+    stats->loss_first = lh[0];
+    stats->loss_last = lh[1];
     stats->ms_train = now_ms(tt);
-    stats->steps = t;
+    stats->steps = nsteps;
   }
   auto te = clk::now();
   mlp_encode(L, X, 0, N, L.codes, st);

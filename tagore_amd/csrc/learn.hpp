@@ -39,6 +39,7 @@ struct Learner {
   uint16_t *Xn = nullptr, *XnT = nullptr, *Hb = nullptr, *HT = nullptr;
   uint16_t *dZb = nullptr, *dZT = nullptr, *dHT = nullptr;
   float *Z = nullptr, *lossp = nullptr;
+  uint32_t ones_M = 0;  // batch size whose rows of ones are in place in XnT / HT
   // codes and buckets
   float* codes = nullptr;  // [N * CODE]
   uint32_t Ncap = 0, Ccap = 0;
@@ -46,6 +47,11 @@ struct Learner {
   uint32_t *slab = nullptr, *lab = nullptr, *lab2 = nullptr, *gcnt = nullptr, *gtot = nullptr,
            *gstart = nullptr, *order = nullptr;
   uint32_t Scap = 0;
+  // the training loop captured as a HIP graph, replayed while its key (shapes, buffers,
+  // seed) is unchanged; loss sums at the end of the first and last epochs
+  hipGraphExec_t train_exec = nullptr;
+  uint64_t train_key = 0;
+  float* loss_hist = nullptr;  // [2]
 };
 
 struct LearnConfig {
@@ -98,6 +104,6 @@ hipError_t learned_seed(Learner& L, const float* X, uint32_t N, uint32_t D, uint
                         const float* Xs, const uint32_t* samp, uint32_t S, const uint32_t* subG,
                         uint32_t Ks,
                         const LearnConfig& c, uint64_t seed, uint32_t* G, hipStream_t st,
-                        LearnStats* stats, bool timed);
+                        LearnStats* stats, bool timed, bool use_graph = false);
 
 }  // namespace tg

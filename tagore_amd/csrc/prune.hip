@@ -312,6 +312,7 @@ __global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
     R[i] = i < RING ? mk_key(__uint_as_float(0x7F800000u), TG_INV) : TG_KEY_INF;
   __syncthreads();
   uint32_t n_col = 0, n_adj = 0, n_flt = 0;
+  const long long c0 = a.phase_clock ? clock64() : 0;
   if (KIND == 0) {
     uint32_t ring_c = 0;
     const uint32_t ep = a.ep[a.seg_of ? a.seg_of[u] : 0];
@@ -329,6 +330,7 @@ __global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
     n_adj = (RING + K - 1) / K;
   }
   __syncthreads();
+  const long long c1 = a.phase_clock ? clock64() : 0;
   // pool = merge(sorted own list, sorted ring); entries repeating their predecessor's id
   // get id INV
   wave_bitonic(O, ly.nK);
@@ -349,6 +351,11 @@ __global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
   __syncthreads();
   const int deg = filter_pool<MC, T>(a, P, L, u, a.fwd_rule, sel, cl, n_flt);
   __syncthreads();
+  if (a.phase_clock && lane == 0 && a.counters) {
+    const long long c2 = clock64();
+    atomicAdd(&a.counters[6], (unsigned long long)(c1 - c0));
+    atomicAdd(&a.counters[7], (unsigned long long)(c2 - c1));
+  }
   for (int k = lane; k < deg; k += TG_WAVE) {
     const uint64_t s = sel[k];
     a.fwd_ids[size_t(u) * a.FD + k] = key_id(s);

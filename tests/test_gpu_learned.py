@@ -131,3 +131,23 @@ def test_learned_init_gain(kind, N, D, K, ncomp):
     _say(f"  recall@10 after 2 iterations: random {r_rand:.4f}, learned {r_learn:.4f}")
     assert r_learn >= 0.85
     assert r_learn >= r_rand + 0.2
+
+
+def test_training_graph_replays_match_eager():
+    """The encoder's training loop runs as a captured HIP graph, replayed by later builds;
+    every replay (with other device work in between) equals the eager launches exactly."""
+    X = gmm_int(12000, 128, 20, 8)
+    with State(*X.shape) as s:
+        s.set_data(X)
+        s.knn_build(32, 0, 0x77, "exact", init="learned", sample=5000)
+        G_e, GD_e = s.get_knn()
+        Z_e = s.codes()
+        loss_e = s.stats()["learn_loss"]
+        for rep in range(3):
+            _say(f"graph build {rep}")
+            s.knn_build(32, 0, 0x77, "exact", init="learned", sample=5000, graph=True)
+            G_g, GD_g = s.get_knn()
+            assert np.array_equal(Z_g := s.codes(), Z_e), f"codes differ on replay {rep}"
+            assert np.array_equal(G_g, G_e) and np.array_equal(GD_g, GD_e), f"lists, replay {rep}"
+            assert s.stats()["learn_loss"] == loss_e
+            s.knn_build(32, 2, 1 + rep, "bf16")  # unrelated device work between replays
