@@ -300,8 +300,10 @@ __device__ void wave_bitonic(T* a, int n) {
   const int lane = threadIdx.x;
   for (int k = 2; k <= n; k <<= 1) {
     for (int j = k >> 1; j > 0; j >>= 1) {
+      // j is a power of two: 2 j (i / j) + i % j by shifts and a mask
+      const int sh = __builtin_ctz(unsigned(j));
       for (int i = lane; i < (n >> 1); i += TG_WAVE) {
-        const int lo = 2 * j * (i / j) + (i % j);
+        const int lo = ((i >> sh) << (sh + 1)) | (i & (j - 1));
         const int hi = lo + j;
         const bool up = (lo & k) == 0;
         T x = a[lo], y = a[hi];

@@ -274,6 +274,11 @@ __global__ __launch_bounds__(256) void k_localjoin_screen(KnnArgs a) {
   const int gg = tid >> 4, t = tid & 15;
   const float eta = a.eta;
   const int tB = (nB + LJ_TILE - 1) / LJ_TILE, tA = (nA + LJ_TILE - 1) / LJ_TILE;
+  // quotients by the tile counts (1..4) as multiply-shifts: exact for numerators < 21845
+  const uint32_t mB = (65536u + uint32_t(tB) - 1) / uint32_t(max(tB, 1));
+  const uint32_t mA = (65536u + uint32_t(tA) - 1) / uint32_t(max(tA, 1));
+  auto divB = [&](int x) { return int((uint32_t(x) * mB) >> 16); };
+  auto divA = [&](int x) { return int((uint32_t(x) * mA) >> 16); };
   const int nj0 = nA * tB, nj1 = nB * tA, njobs = nj0 + nj1 + nA * tA;
   uint32_t n_exact = 0;
   for (int base = 0; base < njobs; base += QJ) {
@@ -283,11 +288,11 @@ __global__ __launch_bounds__(256) void k_localjoin_screen(KnnArgs a) {
     const int J = base + tid;
     if (tid < QJ && J < njobs) {
       if (J < nj0) {
-        kind = 0; r = J / tB; lo = (J % tB) * LJ_TILE;
+        kind = 0; r = divB(J); lo = (J - r * tB) * LJ_TILE;
       } else if (J < nj0 + nj1) {
-        kind = 1; r = (J - nj0) / tA; lo = ((J - nj0) % tA) * LJ_TILE;
+        kind = 1; r = divA(J - nj0); lo = (J - nj0 - r * tA) * LJ_TILE;
       } else {
-        kind = 2; r = (J - nj0 - nj1) / tA; lo = ((J - nj0 - nj1) % tA) * LJ_TILE;
+        kind = 2; r = divA(J - nj0 - nj1); lo = (J - nj0 - nj1 - r * tA) * LJ_TILE;
       }
       const bool col = kind == 1, cb = kind == 0;
       const int nc = cb ? nB : nA, hi = min(nc, lo + LJ_TILE);
@@ -357,7 +362,7 @@ __global__ __launch_bounds__(256) void k_localjoin_screen(KnnArgs a) {
           const uint32_t q = qent[e];
           const int jk = (q >> 6) & 3, jc = q & 63;
           const int J2 = base + int(q >> 8);
-          const int rr = jk == 0 ? J2 / tB : (jk == 1 ? (J2 - nj0) / tA : (J2 - nj0 - nj1) / tA);
+          const int rr = jk == 0 ? divB(J2) : (jk == 1 ? divA(J2 - nj0) : divA(J2 - nj0 - nj1));
           const uint32_t dest = jk == 1 ? B[rr] : A[rr];
           const uint32_t cid = jk == 0 ? B[jc] : A[jc];
           ra[b] = Xr + size_t(dest) * D;
