@@ -473,27 +473,30 @@ __device__ void merge_lds(const uint64_t* A, int na, const uint64_t* B, int nb, 
 }
 
 // Evaluates the targets listed in ids[0..n) (selected through idx when given) against the
-// register row `tgt`, four candidates per 16-lane group at a time; calls emit(q, d) in lane
-// t == 0 of the group that owns item q.
+// register row `tgt`, four candidates per 16-lane group at a time (item q0 + g + 4b in group
+// g).  The 16 results are then moved to lanes 0..15 (lane j holds item q0 + j) and emit(q, d)
+// runs once in those 16 lanes instead of four times in one lane per group.
 template <int MC, class T, class IdOf, class Emit>
 __device__ __forceinline__ void eval_batch(const T* X, int D, const Row16<MC>& tgt, int n,
                                            IdOf id_of, Emit emit) {
   const int lane = threadIdx.x, g = lane >> 4, t = lane & 15;
+  const int src = 16 * (lane & 3), mine = (lane >> 2) & 3;
   for (int q0 = 0; q0 < n; q0 += 16) {
     const T* rows[4];
-    int qs[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-      qs[b] = q0 + g + 4 * b;
-      rows[b] = qs[b] < n ? X + size_t(id_of(qs[b])) * D : nullptr;
+      const int q = q0 + g + 4 * b;
+      rows[b] = q < n ? X + size_t(id_of(q)) * D : nullptr;
     }
     float dd[4];
     dist16_batch<MC, 4>(tgt, rows, D, t, dd);
-    if (t == 0) {
+    float v = 0.f;
 #pragma unroll
-      for (int b = 0; b < 4; ++b)
-        if (qs[b] < n) emit(qs[b], dd[b]);
+    for (int b = 0; b < 4; ++b) {
+      const float x = __shfl(dd[b], src, TG_WAVE);
+      if (b == mine) v = x;
     }
+    if (lane < 16 && q0 + lane < n) emit(q0 + lane, v);
   }
 }
 
