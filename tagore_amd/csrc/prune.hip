@@ -195,8 +195,32 @@ __device__ int next_valid(const uint64_t* P, int from, int L, uint32_t skip) {
 }
 
 // -------------------------------------------------------------------------------------
+// Keeps, in order, the entries cl[from..from+n) for which keep(index) holds, writing them to
+// cl[0..); returns how many.  Each 64-entry chunk is read before any of its kept entries is
+// written, and writes land at or below the read position, so the compaction is in place.
+// -------------------------------------------------------------------------------------
+template <class Keep>
+__device__ __forceinline__ int compact_cl(uint16_t* cl, int from, int n, Keep keep) {
+  const int lane = threadIdx.x;
+  int out = 0;
+  for (int base = 0; base < n; base += TG_WAVE) {
+    const int j = base + lane;
+    const uint16_t c = j < n ? cl[from + j] : uint16_t(0);
+    const bool k = j < n && keep(int(c));
+    const uint64_t m = ballot(k);
+    __syncthreads();
+    if (k) cl[out + mask_rank(m)] = c;
+    out += __popcll(m);
+    __syncthreads();
+  }
+  return out;
+}
+
+// -------------------------------------------------------------------------------------
 // Incremental Filter over the sorted pool P[0..L) (spec A.4 in the oracle).  Writes the
-// kept entries to sel[] and returns their number.
+// kept entries to sel[] and returns their number.  cl holds the live list: the pool
+// indices after the current selection that are not invalidated, in pool order, so a
+// round touches only the survivors of the previous round.
 // -------------------------------------------------------------------------------------
 template <int MC, class T>
 __device__ int filter_pool(const PruneArgs& a, uint64_t* P, int L, uint32_t u, int rule,
@@ -210,25 +234,25 @@ __device__ int filter_pool(const PruneArgs& a, uint64_t* P, int L, uint32_t u, i
   float cur = key_d(P[tix]);
   uint32_t pid = key_id(P[tix]);
   ++tix;
+  int nl = 0;
+  for (int base = tix; base < L; base += TG_WAVE) {
+    const int i = base + lane;
+    const bool live = i < L && key_id(P[i]) != TG_INV;
+    const uint64_t m = ballot(live);
+    if (live) cl[nl + mask_rank(m)] = uint16_t(i);
+    nl += __popcll(m);
+  }
+  int from = 0;  // cl[from..from+nl) is the live list
   __syncthreads();
   while (nsel < FD) {
     Row16<MC> prow;
     load_row16<MC>(prow, reinterpret_cast<const T*>(a.rows) + size_t(pid) * D, D, t);
-    // compact the live candidates after tix; entries repeating p's id are invalidated
-    int cnt = 0;
-    for (int base = tix; base < L; base += TG_WAVE) {
-      const int i = base + lane;
-      bool live = false;
-      if (i < L) {
-        const uint32_t id = key_id(P[i]);
-        if (id == pid) P[i] = key_with_id(P[i], TG_INV);
-        live = id != TG_INV && id != pid;
-      }
-      const uint64_t m = ballot(live);
-      if (live) cl[cnt + mask_rank(m)] = uint16_t(i);
-      cnt += __popcll(m);
-    }
-    __syncthreads();
+    // candidates: the live entries; those repeating p's id are invalidated
+    const int cnt = compact_cl(cl, from, nl, [&](int i) {
+      const uint32_t id = key_id(P[i]);
+      if (id == pid) P[i] = key_with_id(P[i], TG_INV);
+      return id != TG_INV && id != pid;
+    });
     eval_batch<MC>(reinterpret_cast<const T*>(a.rows), D, prow, cnt, [&](int q) { return key_id(P[cl[q]]); },
                    [&](int q, float res) {
                      const int i = cl[q];
@@ -238,11 +262,18 @@ __device__ int filter_pool(const PruneArgs& a, uint64_t* P, int L, uint32_t u, i
                    });
     n_eval += cnt;
     __syncthreads();
-    tix = next_valid(P, tix, L, TG_INV);
-    if (tix < L) {
-      if (lane == 0) sel[nsel] = P[tix];
-      cur = key_d(P[tix]);
-      pid = key_id(P[tix]);
+    // survivors in pool order; the first is the next selection
+    nl = compact_cl(cl, 0, cnt, [&](int i) { return key_id(P[i]) != TG_INV; });
+    if (nl > 0) {
+      const int pi = cl[0];
+      if (lane == 0) sel[nsel] = P[pi];
+      cur = key_d(P[pi]);
+      pid = key_id(P[pi]);
+      tix = pi;
+      from = 1;
+      --nl;
+    } else {
+      tix = L;
     }
     ++nsel;  // an exhausted pool appends a placeholder, dropped below
     ++tix;
