@@ -41,7 +41,9 @@ constexpr int CENTROID_CHUNK = 1024;
 
 // ---------------------------------------------------------------------------------------
 // Canonical FP32 squared L2.  64 partial sums p[l] over i == l (mod 64), each folded in
-// ascending i with fmaf(t, t, acc), then the tree p[l] += p[l+s] for s = 32..1.
+// ascending i with fmaf(t, t, acc); then the four partials of each run l = 4t..4t+3 are
+// summed as q[t] = (p[4t] + p[4t+1]) + (p[4t+2] + p[4t+3]), and the tree q[t] += q[t+s]
+// for s = 8..1 gives the result (six levels in all).
 // Pinned by: closed form vs FP64 (tests/test_oracle.py), symmetry, D=96/128/960.
 // (Replaces the FP16 __hfma2 loops, e.g. Tagore_src.cu:261-286, 1429-1446.)
 // ---------------------------------------------------------------------------------------
@@ -52,9 +54,11 @@ inline float cdist(const float* a, const float* b, int D) {
     float t = a[i] - b[i];
     p[i & 63] = std::fmaf(t, t, p[i & 63]);
   }
-  for (int s = 32; s >= 1; s >>= 1)
-    for (int l = 0; l < s; ++l) p[l] = p[l] + p[l + s];
-  return p[0];
+  float q[16];
+  for (int t = 0; t < 16; ++t) q[t] = (p[4 * t] + p[4 * t + 1]) + (p[4 * t + 2] + p[4 * t + 3]);
+  for (int s = 8; s >= 1; s >>= 1)
+    for (int l = 0; l < s; ++l) q[l] = q[l] + q[l + s];
+  return q[0];
 }
 
 struct Ent {

@@ -3,8 +3,8 @@
 // * (distance, id) keys packed into u64 so that unsigned order == (d, id) order for the
 //   non-negative distances used everywhere (+inf sorts after every finite distance).
 // * Canonical FP32 squared L2 evaluated by a 16-lane group with float4 loads: lane t owns
-//   the partial sums p[4t..4t+3] over i == l (mod 64); the tree p[l] += p[l+s] runs as
-//   DPP row shifts 8,4,2,1 across the group and the last two levels inside lane 0.
+//   the partial sums p[4t..4t+3] over i == l (mod 64); they are summed inside the lane,
+//   (p[4t] + p[4t+1]) + (p[4t+2] + p[4t+3]), then across the group by DPP row shifts 8,4,2,1.
 //   This is the same arithmetic, in the same order, as the CPU oracle (bit-identical).
 // * Wave-level (64-lane) bitonic sorts and scans in LDS.
 // * PushBuf: per-destination append lists with a bounded slot area and a global overflow
@@ -128,29 +128,16 @@ __device__ __forceinline__ float row_shl(float v) {
                                                     false));
 }
 
-// Tree reduction of the group's partials (lane t holds p[4t..4t+3]): the steps s = 32,16,8,4
-// pair lane t with lane t+8, t+4, t+2, t+1 (DPP row shifts), s = 2,1 run inside lane 0.
-// The result is valid in lane 0 of the group (t == 0) only.
+// Reduction of the group's partials (lane t holds p[4t..4t+3]): first inside the lane,
+// q[t] = (p[4t] + p[4t+1]) + (p[4t+2] + p[4t+3]), then the tree q[t] += q[t+s] for
+// s = 8,4,2,1 as DPP row shifts across the group.  Six levels, like any balanced tree of
+// the 64 partials.  The result is valid in lane 0 of the group (t == 0) only.
 __device__ __forceinline__ float tree16(float p0, float p1, float p2, float p3) {
-  p0 = p0 + row_shl<8>(p0);
-  p1 = p1 + row_shl<8>(p1);
-  p2 = p2 + row_shl<8>(p2);
-  p3 = p3 + row_shl<8>(p3);
-  p0 = p0 + row_shl<4>(p0);
-  p1 = p1 + row_shl<4>(p1);
-  p2 = p2 + row_shl<4>(p2);
-  p3 = p3 + row_shl<4>(p3);
-  p0 = p0 + row_shl<2>(p0);
-  p1 = p1 + row_shl<2>(p1);
-  p2 = p2 + row_shl<2>(p2);
-  p3 = p3 + row_shl<2>(p3);
-  p0 = p0 + row_shl<1>(p0);
-  p1 = p1 + row_shl<1>(p1);
-  p2 = p2 + row_shl<1>(p2);
-  p3 = p3 + row_shl<1>(p3);
-  p0 = p0 + p2;
-  p1 = p1 + p3;
-  return p0 + p1;
+  float q = (p0 + p1) + (p2 + p3);
+  q = q + row_shl<8>(q);
+  q = q + row_shl<4>(q);
+  q = q + row_shl<2>(q);
+  return q + row_shl<1>(q);
 }
 
 // Row chunks held in flight per pair / candidate: at most 4 x 64 elements, so wide rows

@@ -42,6 +42,20 @@ def test_dist_is_a_tree_not_a_running_sum():
     assert float(oracle.dist(a, np.zeros(64, np.float32))) == 2.0 ** 24 + 62
 
 
+def test_dist_sums_each_lane_run_first():
+    # p = [2^24, 1, 0, 1, 0...]: (p0 + p1) rounds to 2^24 (tie to even) and (p2 + p3) = 1,
+    # so the in-run order gives 2^24 + 1 -> 2^24; pairing p0 with p2 and p1 with p3 (a tree
+    # over s = 32..1) would give 2^24 + 2, and an exact sum 2^24 + 2 as well.
+    a = np.zeros(64, np.float32)
+    a[0], a[1], a[3] = 4096.0, 1.0, 1.0
+    assert float(oracle.dist(a, np.zeros(64, np.float32))) == 2.0 ** 24
+    # with ones elsewhere: q[0] = (2^24 + 1 -> 2^24) + 2, q[t] = 4 for t >= 1, and the
+    # tree over the 16 runs adds the remaining 60 exactly
+    b = np.ones(64, np.float32)
+    b[0] = 4096.0
+    assert float(oracle.dist(b, np.zeros(64, np.float32))) == 2.0 ** 24 + 62
+
+
 # --------------------------------------------------------------------------- brute force
 def test_knn_bruteforce_matches_numpy():
     rng = np.random.default_rng(3)
