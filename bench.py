@@ -283,6 +283,8 @@ def main():
     ap.add_argument("--prune-rows", default="fp32", choices=["fp32", "bf16"],
                     help="rows the headline prune reads (bf16 only when the data is exact in BF16)")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--phase-clocks", action="store_true",
+                    help="also report the prune forward kernel's per-phase shader cycles")
     ap.add_argument("--seed", type=int, default=0x7A600002)
     ap.add_argument("--recall-queries", type=int, default=10000)
     ap.add_argument("--cpu-sample", type=int, default=100000,
@@ -329,6 +331,18 @@ def main():
             ablation = measure(st, args, args.precision, args.ablation_iters, "random",
                                "ablation")
     head = measure(st, args, args.precision, args.iters, args.init, "headline")
+    if args.phase_clocks:
+        # one more prune of the last headline graph with the forward kernel's phase counters
+        # (instrumented, so not part of any timing above)
+        st.set_profiling(2)
+        st.prune("path", "dist", "dist", args.fd, args.topm, 1.0,
+                 args.prune_rows if args.precision == "bf16" else "fp32")
+        pcs = st.stats()["prune_counters"]
+        head["phase_cycles_per_node"] = {k: pcs[k] / N for k in
+                                         ("collect_cycles", "filter_cycles", "pool_cycles",
+                                          "eval_cycles")}
+        log(f"phase cycles per node: {head['phase_cycles_per_node']}")
+        st.set_profiling(True)
 
     # roofline of the pruning forward kernel (Collect + Filter): algorithmic bytes = every
     # canonical distance evaluation reads one FP32 row, plus adjacency rows, own lists, output

@@ -201,10 +201,12 @@ typedef struct {
   double prune_kernel_ms[4]; /* ep, forward, reverse, other */
   uint64_t knn_evals;        /* distance evaluations of the last build */
   uint64_t knn_rescored;     /* canonical FP32 re-scores after the screen (phase 2 + join) */
-  uint64_t prune_counters[8];/* collect, filter, reverse, adjacency rows, ep evaluations,
-                                exact re-scores of screened DPG pairs; with profiling on,
+  uint64_t prune_counters[10];/* collect, filter, reverse, adjacency rows, ep evaluations,
+                                exact re-scores of screened DPG pairs; with profiling 2,
                                 summed shader-clock cycles of the path / two-hop forward
-                                kernel's collect (6) and pool + filter (7) phases */
+                                kernel's collect (6), filter (7) and pool sort + merge (8)
+                                phases, and of the distance batches inside collect and
+                                filter (9) */
   double learn_ms[5];        /* learned init: sample graph, training, encoding, buckets, seeding
                                 (filled when profiling is on) */
   float learn_loss[2];       /* training loss at the end of the first and the last epoch */
@@ -212,7 +214,8 @@ typedef struct {
   uint32_t learn_steps;      /* Adam steps */
 } tg_stats;
 int tg_get_stats(tg_state* s, tg_stats* out);
-/* 1: time each kernel with events (adds a little overhead), 0: only the windows. */
+/* 0: only the windows; 1: time each kernel with events (adds a little overhead); 2: as 1,
+ * plus the shader-clock phase counters of the prune forward kernel (adds work to it). */
 int tg_set_profiling(tg_state* s, int on);
 
 /* Codes [N*32] of the last learned initialisation (FP32). */

@@ -64,7 +64,7 @@ class Stats(ctypes.Structure):
     _fields_ = [("knn_ms", ctypes.c_double), ("prune_ms", ctypes.c_double),
                 ("knn_kernel_ms", ctypes.c_double * 8), ("prune_kernel_ms", ctypes.c_double * 4),
                 ("knn_evals", ctypes.c_uint64), ("knn_rescored", ctypes.c_uint64),
-                ("prune_counters", ctypes.c_uint64 * 8),
+                ("prune_counters", ctypes.c_uint64 * 10),
                 ("learn_ms", ctypes.c_double * 5), ("learn_loss", ctypes.c_float * 2),
                 ("learn_buckets", ctypes.c_uint32), ("learn_steps", ctypes.c_uint32)]
 
@@ -298,15 +298,18 @@ class State:
                     prune_kernel_ms=dict(zip(PRUNE_SECTIONS, list(s.prune_kernel_ms))),
                     knn_evals=int(s.knn_evals), knn_rescored=int(s.knn_rescored),
                     prune_counters=dict(zip(["collect", "filter", "reverse", "adj_rows", "ep",
-                                             "rescored", "collect_cycles", "filter_cycles"],
+                                             "rescored", "collect_cycles", "filter_cycles",
+                                             "pool_cycles", "eval_cycles"],
                                             list(s.prune_counters))),
                     learn_ms=dict(zip(["sample_graph", "train", "encode", "buckets", "seed"],
                                       list(s.learn_ms))),
                     learn_loss=list(s.learn_loss), learn_buckets=int(s.learn_buckets),
                     learn_steps=int(s.learn_steps))
 
-    def set_profiling(self, on: bool):
-        _check(lib().tg_set_profiling(self._h, 1 if on else 0))
+    def set_profiling(self, on):
+        """False/0: device windows only; True/1: per-kernel event times; 2: also the prune
+        forward kernel's shader-clock phase counters (adds work to that kernel)."""
+        _check(lib().tg_set_profiling(self._h, int(on)))
 
     def buffer(self, idx: int):
         p, n = ctypes.c_void_p(), ctypes.c_size_t()

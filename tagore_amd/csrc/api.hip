@@ -149,7 +149,7 @@ struct tg_state {
   bool has_codes = false;
   bool order_valid = false;    // learn.order (bucket order) belongs to the current graph
   // stats
-  bool profiling = false;
+  int profiling = 0;  // 1: per-kernel events, 2: also prune phase clocks
   tg_stats stats{};
   std::vector<hipEvent_t> ev_pool;
 };
@@ -535,7 +535,7 @@ void tg_host_free(void* p) { free(p); }
 
 int tg_set_profiling(tg_state* s, int on) {
   if (!s) return fail(TG_ERR_ARG, "null handle");
-  s->profiling = on != 0;
+  s->profiling = on < 0 ? 0 : (on > 2 ? 2 : on);
   return TG_OK;
 }
 
@@ -940,7 +940,7 @@ int tg_prune(tg_state* s, const tg_prune_params* p) {
   a.out_ids = s->out_ids;
   a.rev = s->rev.d;
   a.counters = s->counters;
-  a.phase_clock = s->profiling ? 1 : 0;
+  a.phase_clock = s->profiling >= 2 ? 1 : 0;
   a.rows = s->X;
   a.rows_bf16 = 0;
   if (p->precision == TG_PREC_BF16 || p->precision == TG_PREC_FP8) {
@@ -1018,9 +1018,9 @@ int tg_prune(tg_state* s, const tg_prune_params* p) {
   s->stats.prune_ms = ms;
   for (double& v : s->stats.prune_kernel_ms) v = 0.0;
   if (sec.collect(s->stats.prune_kernel_ms)) return TG_ERR_HIP;
-  unsigned long long cnt[8];
+  unsigned long long cnt[10];
   HIPX(hipMemcpy(cnt, s->counters, sizeof(cnt), hipMemcpyDeviceToHost));
-  for (int i = 0; i < 8; ++i) s->stats.prune_counters[i] = cnt[i];
+  for (int i = 0; i < 10; ++i) s->stats.prune_counters[i] = cnt[i];
   if (s->nseg) {
     s->seg_ep_host.resize(s->nseg);
     if (p->kind == TG_KIND_PATH)
@@ -1107,8 +1107,8 @@ int tg_autotune_knn(tg_state* s, uint32_t K, uint64_t seed, tg_tune_result* out)
   const int32_t precs[7] = {TG_PREC_EXACT, TG_PREC_FP32, TG_PREC_FP32, TG_PREC_BF16,
                             TG_PREC_BF16, TG_PREC_FP8, TG_PREC_FP8};
   const uint32_t tiles[7] = {0, 0, 1, 0, 1, 0, 1};
-  const bool prof = s->profiling;
-  s->profiling = false;
+  const int prof = s->profiling;
+  s->profiling = 0;
   tg_knn_params kp{};
   kp.K = K;
   kp.iters = 2;

@@ -148,8 +148,28 @@ struct RowChunk {
   static constexpr int N = MC < 4 ? MC : 4;
 };
 
-// NB distances from row a (registers) to rows b[0..NB) (b[i] may be null: result unused);
-// the loads of a chunk are issued before its arithmetic.  Results valid in lane t == 0.
+// Squared differences of one chunk pair folded into the four partials.  Elements past D are
+// zero in both operands, so they add fmaf(0, 0, p) = p and leave the partials unchanged.
+__device__ __forceinline__ void fold4(const float4& x, const float4& y, float (&p)[4]) {
+  const float e0 = x.x - y.x, e1 = x.y - y.y, e2 = x.z - y.z, e3 = x.w - y.w;
+  p[0] = __builtin_fmaf(e0, e0, p[0]);
+  p[1] = __builtin_fmaf(e1, e1, p[1]);
+  p[2] = __builtin_fmaf(e2, e2, p[2]);
+  p[3] = __builtin_fmaf(e3, e3, p[3]);
+}
+
+// Elements m*64 + 4t .. +3 of row x, zero past D.
+template <class T>
+__device__ __forceinline__ float4 load_chunk_tail(const T* x, int m, int D, int t) {
+  const int i = m * 64 + 4 * t;
+  return i < D ? load4(x + i) : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// NB distances from row a (registers) to rows b[0..NB).  Every b[k] must point at a
+// readable row (callers aim unused slots at any valid row and ignore the result).  Chunks
+// of 64 elements that lie wholly inside D are loaded and folded without per-lane masks
+// (the test is uniform across the wave); only a partial last chunk masks its loads.  The
+// loads of a group of chunks are issued before its arithmetic.  Results valid in lane t == 0.
 template <int MC, int NB, class T>
 __device__ __forceinline__ void dist16_batch(const Row16<MC>& a, const T* const (&b)[NB], int D,
                                              int t, float (&out)[NB]) {
@@ -159,36 +179,35 @@ __device__ __forceinline__ void dist16_batch(const Row16<MC>& a, const T* const 
   for (int k = 0; k < NB; ++k) p[k][0] = p[k][1] = p[k][2] = p[k][3] = 0.f;
 #pragma unroll
   for (int m0 = 0; m0 < MC; m0 += CH) {
+    if (m0 * 64 >= D) break;
     float4 y[NB][CH];
+    if ((m0 + CH) * 64 <= D) {
 #pragma unroll
-    for (int k = 0; k < NB; ++k) {
+      for (int k = 0; k < NB; ++k)
 #pragma unroll
-      for (int c = 0; c < CH; ++c) {
-        const int i = (m0 + c) * 64 + 4 * t;
-        y[k][c] = (b[k] && i < D) ? load4(b[k] + i) : make_float4(0.f, 0.f, 0.f, 0.f);
-      }
-    }
+        for (int c = 0; c < CH; ++c) y[k][c] = load4(b[k] + (m0 + c) * 64 + 4 * t);
 #pragma unroll
-    for (int k = 0; k < NB; ++k) {
+      for (int k = 0; k < NB; ++k)
 #pragma unroll
-      for (int c = 0; c < CH; ++c) {
-        const int m = m0 + c;
-        if (m * 64 + 4 * t < D) {
-          const float e0 = a.v[m].x - y[k][c].x, e1 = a.v[m].y - y[k][c].y,
-                      e2 = a.v[m].z - y[k][c].z, e3 = a.v[m].w - y[k][c].w;
-          p[k][0] = __builtin_fmaf(e0, e0, p[k][0]);
-          p[k][1] = __builtin_fmaf(e1, e1, p[k][1]);
-          p[k][2] = __builtin_fmaf(e2, e2, p[k][2]);
-          p[k][3] = __builtin_fmaf(e3, e3, p[k][3]);
-        }
-      }
+        for (int c = 0; c < CH; ++c) fold4(a.v[m0 + c], y[k][c], p[k]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < NB; ++k)
+#pragma unroll
+        for (int c = 0; c < CH; ++c) y[k][c] = load_chunk_tail(b[k], m0 + c, D, t);
+#pragma unroll
+      for (int k = 0; k < NB; ++k)
+#pragma unroll
+        for (int c = 0; c < CH; ++c)
+          if ((m0 + c) * 64 < D) fold4(a.v[m0 + c], y[k][c], p[k]);
     }
   }
 #pragma unroll
   for (int k = 0; k < NB; ++k) out[k] = tree16(p[k][0], p[k][1], p[k][2], p[k][3]);
 }
 
-// NB distances between the row pairs (a[k], b[k]), both read from memory (null: unused).
+// NB distances between the row pairs (a[k], b[k]), both read from memory; every pointer
+// must be a readable row, as for dist16_batch.
 template <int MC, int NB, class T>
 __device__ __forceinline__ void dist16_pairs(const T* const (&a)[NB], const T* const (&b)[NB],
                                              int D, int t, float (&out)[NB]) {
@@ -198,30 +217,34 @@ __device__ __forceinline__ void dist16_pairs(const T* const (&a)[NB], const T* c
   for (int k = 0; k < NB; ++k) p[k][0] = p[k][1] = p[k][2] = p[k][3] = 0.f;
 #pragma unroll
   for (int m0 = 0; m0 < MC; m0 += CH) {
+    if (m0 * 64 >= D) break;
     float4 x[NB][CH], y[NB][CH];
+    if ((m0 + CH) * 64 <= D) {
 #pragma unroll
-    for (int k = 0; k < NB; ++k) {
+      for (int k = 0; k < NB; ++k)
 #pragma unroll
-      for (int c = 0; c < CH; ++c) {
-        const int i = (m0 + c) * 64 + 4 * t;
-        const bool ok = a[k] && i < D;
-        x[k][c] = ok ? load4(a[k] + i) : make_float4(0.f, 0.f, 0.f, 0.f);
-        y[k][c] = ok ? load4(b[k] + i) : make_float4(0.f, 0.f, 0.f, 0.f);
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < NB; ++k) {
-#pragma unroll
-      for (int c = 0; c < CH; ++c) {
-        if ((m0 + c) * 64 + 4 * t < D) {
-          const float e0 = x[k][c].x - y[k][c].x, e1 = x[k][c].y - y[k][c].y,
-                      e2 = x[k][c].z - y[k][c].z, e3 = x[k][c].w - y[k][c].w;
-          p[k][0] = __builtin_fmaf(e0, e0, p[k][0]);
-          p[k][1] = __builtin_fmaf(e1, e1, p[k][1]);
-          p[k][2] = __builtin_fmaf(e2, e2, p[k][2]);
-          p[k][3] = __builtin_fmaf(e3, e3, p[k][3]);
+        for (int c = 0; c < CH; ++c) {
+          const int i = (m0 + c) * 64 + 4 * t;
+          x[k][c] = load4(a[k] + i);
+          y[k][c] = load4(b[k] + i);
         }
-      }
+#pragma unroll
+      for (int k = 0; k < NB; ++k)
+#pragma unroll
+        for (int c = 0; c < CH; ++c) fold4(x[k][c], y[k][c], p[k]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < NB; ++k)
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+          x[k][c] = load_chunk_tail(a[k], m0 + c, D, t);
+          y[k][c] = load_chunk_tail(b[k], m0 + c, D, t);
+        }
+#pragma unroll
+      for (int k = 0; k < NB; ++k)
+#pragma unroll
+        for (int c = 0; c < CH; ++c)
+          if ((m0 + c) * 64 < D) fold4(x[k][c], y[k][c], p[k]);
     }
   }
 #pragma unroll
@@ -473,7 +496,7 @@ __device__ __forceinline__ void eval_batch(const T* X, int D, const Row16<MC>& t
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
       const int q = q0 + g + 4 * b;
-      rows[b] = q < n ? X + size_t(id_of(q)) * D : nullptr;
+      rows[b] = q < n ? X + size_t(id_of(q)) * D : X;  // unused slots read row 0
     }
     float dd[4];
     dist16_batch<MC, 4>(tgt, rows, D, t, dd);

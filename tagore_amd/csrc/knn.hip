@@ -185,12 +185,15 @@ __global__ __launch_bounds__(256) void k_localjoin_exact(KnnArgs a) {
     const int nj = nB + (nA - i - 1);
     for (int j0 = 0; j0 < nj; j0 += 4) {
       const float* rows[4];
+      bool same[4];
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
         const int j = j0 + b;
-        rows[b] = nullptr;
+        rows[b] = a.X;  // unused slots read row 0
+        same[b] = false;
         if (j < nB) {
-          if (A[i] != B[j]) rows[b] = a.X + size_t(B[j]) * D;
+          same[b] = A[i] == B[j];
+          rows[b] = a.X + size_t(B[j]) * D;
         } else if (j < nj) {
           rows[b] = a.X + size_t(A[i + 1 + (j - nB)]) * D;
         }
@@ -202,7 +205,7 @@ __global__ __launch_bounds__(256) void k_localjoin_exact(KnnArgs a) {
         for (int b = 0; b < 4; ++b) {
           const int j = j0 + b;
           if (j < nB) {
-            MAB[i * 64 + j] = rows[b] ? dd[b] : INF;
+            MAB[i * 64 + j] = same[b] ? INF : dd[b];
           } else if (j < nj) {
             const int jj = i + 1 + (j - nB);
             MAA[i * 64 + jj] = dd[b];

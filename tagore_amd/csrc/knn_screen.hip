@@ -357,7 +357,7 @@ __global__ __launch_bounds__(256) void k_localjoin_screen(KnnArgs a) {
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
         const int e = e0 + gg + 16 * b;
-        ra[b] = rb[b] = nullptr;
+        ra[b] = rb[b] = Xr;  // unused slots read row 0
         if (e < nq) {
           const uint32_t q = qent[e];
           const int jk = (q >> 6) & 3, jc = q & 63;

@@ -18,6 +18,11 @@
 namespace tg {
 
 constexpr int RING = 480;
+
+// shader clock when phase profiling is on (PruneArgs::phase_clock), else 0
+__device__ __forceinline__ long long pclock(const PruneArgs& a) {
+  return a.phase_clock ? clock64() : 0;
+}
 constexpr int POOL_MAX = 1024;  // K + 480 <= 608, padded to a power of two
 constexpr int W_MAX = 256;      // TOPM + K <= 256
 constexpr int EXPL_MAX = 32;
@@ -62,7 +67,8 @@ template <int MC, class T>
 __device__ uint64_t* beam_search(const PruneArgs& a, const Row16<MC>& tgt, uint32_t start,
                                  int max_exp, uint64_t* M0, uint64_t* M1, uint64_t* S,
                                  uint32_t* tab, uint32_t* expl, uint16_t* cl, uint64_t* ring,
-                                 uint32_t& ring_c, uint32_t& n_eval, uint32_t& n_adj) {
+                                 uint32_t& ring_c, uint32_t& n_eval, uint32_t& n_adj,
+                                 long long* ev = nullptr) {
   const int lane = threadIdx.x;
   const int K = a.K, TOPM = a.TOPM, D = a.D, nK = pow2_at_least(K);
   const T* X = reinterpret_cast<const T*>(a.rows);
@@ -133,6 +139,7 @@ __device__ uint64_t* beam_search(const PruneArgs& a, const Row16<MC>& tgt, uint3
     }
     __syncthreads();
     const uint32_t c0 = ring_c;
+    const long long e0 = ev ? pclock(a) : 0;
     eval_batch<MC>(X, D, tgt, cnt, [&](int q) { return key_id(S[cl[q]]); },
                    [&](int q, float d) {
                      const int j = cl[q];
@@ -140,6 +147,7 @@ __device__ uint64_t* beam_search(const PruneArgs& a, const Row16<MC>& tgt, uint3
                      S[j] = k;
                      if (ring) ring[(c0 + q) % RING] = k;
                    });
+    if (ev) *ev += pclock(a) - e0;
     ring_c += cnt;
     n_eval += cnt;
     __syncthreads();
@@ -224,7 +232,8 @@ __device__ __forceinline__ int compact_cl(uint16_t* cl, int from, int n, Keep ke
 // -------------------------------------------------------------------------------------
 template <int MC, class T>
 __device__ int filter_pool(const PruneArgs& a, uint64_t* P, int L, uint32_t u, int rule,
-                           uint64_t* sel, uint16_t* cl, uint32_t& n_eval) {
+                           uint64_t* sel, uint16_t* cl, uint32_t& n_eval,
+                           long long* ev = nullptr) {
   const int lane = threadIdx.x, t = lane & 15;
   const int D = a.D, FD = a.FD;
   int tix = next_valid(P, 0, L, u);
@@ -253,6 +262,7 @@ __device__ int filter_pool(const PruneArgs& a, uint64_t* P, int L, uint32_t u, i
       if (id == pid) P[i] = key_with_id(P[i], TG_INV);
       return id != TG_INV && id != pid;
     });
+    const long long e0 = ev ? pclock(a) : 0;
     eval_batch<MC>(reinterpret_cast<const T*>(a.rows), D, prow, cnt, [&](int q) { return key_id(P[cl[q]]); },
                    [&](int q, float res) {
                      const int i = cl[q];
@@ -260,6 +270,7 @@ __device__ int filter_pool(const PruneArgs& a, uint64_t* P, int L, uint32_t u, i
                      if (rule_eval(rule, res, key_d(k), cur, a.th) < 0.0f)
                        P[i] = key_with_id(k, TG_INV);
                    });
+    if (ev) *ev += pclock(a) - e0;
     n_eval += cnt;
     __syncthreads();
     // survivors in pool order; the first is the next selection
@@ -343,11 +354,13 @@ __global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
     R[i] = i < RING ? mk_key(__uint_as_float(0x7F800000u), TG_INV) : TG_KEY_INF;
   __syncthreads();
   uint32_t n_col = 0, n_adj = 0, n_flt = 0;
-  const long long c0 = a.phase_clock ? clock64() : 0;
+  long long t_ev = 0;
+  long long* ev = a.phase_clock ? &t_ev : nullptr;
+  const long long c0 = pclock(a);
   if (KIND == 0) {
     uint32_t ring_c = 0;
     const uint32_t ep = a.ep[a.seg_of ? a.seg_of[u] : 0];
-    beam_search<MC, T>(a, xu, ep, 10, M0, M1, S, tab, expl, cl, R, ring_c, n_col, n_adj);
+    beam_search<MC, T>(a, xu, ep, 10, M0, M1, S, tab, expl, cl, R, ring_c, n_col, n_adj, ev);
   } else {
     // two-hop ids y_i = G[G[u][i/K]][i%K], i < 480
     for (int i = lane; i < RING; i += TG_WAVE) {
@@ -361,7 +374,7 @@ __global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
     n_adj = (RING + K - 1) / K;
   }
   __syncthreads();
-  const long long c1 = a.phase_clock ? clock64() : 0;
+  const long long c1 = pclock(a);
   // pool = merge(sorted own list, sorted ring); entries repeating their predecessor's id
   // get id INV
   wave_bitonic(O, ly.nK);
@@ -380,12 +393,15 @@ __global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
   for (int q = 0; q < PQ; ++q)
     if (dup[q]) P[q * TG_WAVE + lane] = key_with_id(P[q * TG_WAVE + lane], TG_INV);
   __syncthreads();
-  const int deg = filter_pool<MC, T>(a, P, L, u, a.fwd_rule, sel, cl, n_flt);
+  const long long c2 = pclock(a);
+  const int deg = filter_pool<MC, T>(a, P, L, u, a.fwd_rule, sel, cl, n_flt, ev);
   __syncthreads();
   if (a.phase_clock && lane == 0 && a.counters) {
-    const long long c2 = clock64();
+    const long long c3 = clock64();
     atomicAdd(&a.counters[6], (unsigned long long)(c1 - c0));
-    atomicAdd(&a.counters[7], (unsigned long long)(c2 - c1));
+    atomicAdd(&a.counters[7], (unsigned long long)(c3 - c2));
+    atomicAdd(&a.counters[8], (unsigned long long)(c2 - c1));
+    atomicAdd(&a.counters[9], (unsigned long long)t_ev);
   }
   for (int k = lane; k < deg; k += TG_WAVE) {
     const uint64_t s = sel[k];

@@ -137,7 +137,7 @@ __global__ __launch_bounds__(256) void k_dpg_screen(PruneArgs a, DpgScreen q) {
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
       const int e = e0 + gg + 16 * b;
-      ra[b] = rb[b] = nullptr;
+      ra[b] = rb[b] = rows;  // unused slots read row 0
       jj[b] = -1;
       if (e < nqd) {
         const int r = queue[e] >> 8, j = queue[e] & 255;
