@@ -340,7 +340,9 @@ def main():
         pcs = st.stats()["prune_counters"]
         head["phase_cycles_per_node"] = {k: pcs[k] / N for k in
                                          ("collect_cycles", "filter_cycles", "pool_cycles",
-                                          "eval_cycles")}
+                                          "eval_cycles", "beam_pre_cycles", "beam_sort_cycles",
+                                          "beam_merge_cycles", "beam_hash_cycles",
+                                          "pool_sort_cycles")}
         log(f"phase cycles per node: {head['phase_cycles_per_node']}")
         st.set_profiling(True)
 

@@ -201,12 +201,13 @@ typedef struct {
   double prune_kernel_ms[4]; /* ep, forward, reverse, other */
   uint64_t knn_evals;        /* distance evaluations of the last build */
   uint64_t knn_rescored;     /* canonical FP32 re-scores after the screen (phase 2 + join) */
-  uint64_t prune_counters[10];/* collect, filter, reverse, adjacency rows, ep evaluations,
+  uint64_t prune_counters[15];/* collect, filter, reverse, adjacency rows, ep evaluations,
                                 exact re-scores of screened DPG pairs; with profiling 2,
                                 summed shader-clock cycles of the path / two-hop forward
                                 kernel's collect (6), filter (7) and pool sort + merge (8)
-                                phases, and of the distance batches inside collect and
-                                filter (9) */
+                                phases, of the distance batches inside collect and filter
+                                (9), of the beam rounds' candidate step, sort, merge and
+                                id-set rebuild (10-13) and of the pool's sorts (14) */
   double learn_ms[5];        /* learned init: sample graph, training, encoding, buckets, seeding
                                 (filled when profiling is on) */
   float learn_loss[2];       /* training loss at the end of the first and the last epoch */
@@ -232,6 +233,14 @@ int tg_mlp_eval(int device, uint32_t D, uint32_t B, uint32_t P, const float* X, 
                 float* codes);
 int tg_mlp_adam(int device, uint32_t D, uint32_t t, float lr, const float* grad, float* params,
                 float* m, float* v);
+
+/* Self-test of the register sorts the prune kernels use (no counterpart in the reference).
+ * keys/out: host [groups * 64 * npl] u64, npl 1, 2 or 8.  mode 0: sort each group
+ * ascending; mode 1 (npl 1): each group is a bitonic sequence, sort it; mode 2 (npl 2): the
+ * two halves of a group are ascending lists, the first half of the output is the 64 smallest
+ * keys of both, ascending (the second half is returned unchanged). */
+int tg_selftest_sort(int device, uint32_t npl, uint32_t groups, int32_t mode,
+                     const uint64_t* keys, uint64_t* out);
 
 #ifdef __cplusplus
 }

@@ -64,7 +64,7 @@ class Stats(ctypes.Structure):
     _fields_ = [("knn_ms", ctypes.c_double), ("prune_ms", ctypes.c_double),
                 ("knn_kernel_ms", ctypes.c_double * 8), ("prune_kernel_ms", ctypes.c_double * 4),
                 ("knn_evals", ctypes.c_uint64), ("knn_rescored", ctypes.c_uint64),
-                ("prune_counters", ctypes.c_uint64 * 10),
+                ("prune_counters", ctypes.c_uint64 * 15),
                 ("learn_ms", ctypes.c_double * 5), ("learn_loss", ctypes.c_float * 2),
                 ("learn_buckets", ctypes.c_uint32), ("learn_steps", ctypes.c_uint32)]
 
@@ -147,6 +147,7 @@ def lib():
     L.tg_mlp_eval.argtypes = [c.c_int, c.c_uint32, c.c_uint32, c.c_uint32, vp, vp, c.c_float, vp,
                               c.c_float, P(c.c_float), vp, vp]
     L.tg_mlp_adam.argtypes = [c.c_int, c.c_uint32, c.c_uint32, c.c_float, vp, vp, vp, vp]
+    L.tg_selftest_sort.argtypes = [c.c_int, c.c_uint32, c.c_uint32, c.c_int32, vp, vp]
     for name in exported_symbols():
         f = getattr(L, name)
         if name not in ("tg_last_error", "tg_host_free"):
@@ -299,7 +300,9 @@ class State:
                     knn_evals=int(s.knn_evals), knn_rescored=int(s.knn_rescored),
                     prune_counters=dict(zip(["collect", "filter", "reverse", "adj_rows", "ep",
                                              "rescored", "collect_cycles", "filter_cycles",
-                                             "pool_cycles", "eval_cycles"],
+                                             "pool_cycles", "eval_cycles", "beam_pre_cycles",
+                                             "beam_sort_cycles", "beam_merge_cycles",
+                                             "beam_hash_cycles", "pool_sort_cycles"],
                                             list(s.prune_counters))),
                     learn_ms=dict(zip(["sample_graph", "train", "encode", "buckets", "seed"],
                                       list(s.learn_ms))),
@@ -369,6 +372,15 @@ def mlp_adam(D: int, t: int, lr: float, grad, params, m, v, device: int = 0):
     v = np.array(v, np.float32)
     _check(lib().tg_mlp_adam(device, D, t, lr, _ptr(g), _ptr(p), _ptr(m), _ptr(v)))
     return p, m, v
+
+
+def selftest_sort(keys, npl: int, mode: int = 0, device: int = 0):
+    """Runs the prune kernels' register sorts on keys [groups, 64*npl] (u64) on the GPU."""
+    k = np.ascontiguousarray(keys, np.uint64)
+    assert k.ndim == 2 and k.shape[1] == 64 * npl
+    out = np.empty_like(k)
+    _check(lib().tg_selftest_sort(device, npl, k.shape[0], mode, _ptr(k), _ptr(out)))
+    return out
 
 
 def nssg_entry_points(N: int, seed: int):

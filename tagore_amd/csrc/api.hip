@@ -566,6 +566,35 @@ struct MlpScratch {
 
 }  // namespace
 
+int tg_selftest_sort(int device, uint32_t npl, uint32_t groups, int32_t mode,
+                     const uint64_t* keys, uint64_t* out) {
+  if (!keys || !out) return fail(TG_ERR_ARG, "null argument");
+  if (npl != 1 && npl != 2 && npl != 8) return fail(TG_ERR_ARG, "npl must be 1, 2 or 8");
+  if (mode < 0 || mode > 2 || (mode == 1 && npl != 1) || (mode == 2 && npl != 2))
+    return fail(TG_ERR_ARG, "mode 0, mode 1 with npl 1, or mode 2 with npl 2");
+  if (groups == 0 || groups > 65536) return fail(TG_ERR_ARG, "groups must be in [1, 65536]");
+  HIPX(hipSetDevice(device));
+  const size_t n = size_t(groups) * 64 * npl;
+  uint64_t *din = nullptr, *dout = nullptr;
+  if (dalloc(&din, n) || dalloc(&dout, n)) {
+    dfree(din);
+    return TG_ERR_HIP;
+  }
+  hipStream_t st;
+  HIPX(hipStreamCreate(&st));
+  HIPX(hipMemcpyAsync(din, keys, n * 8, hipMemcpyHostToDevice, st));
+  launch_selftest_sort(din, dout, int(npl), mode, groups, st);
+  int rc = check_launch("sort self-test");
+  if (!rc) {
+    HIPX(hipMemcpyAsync(out, dout, n * 8, hipMemcpyDeviceToHost, st));
+    HIPX(hipStreamSynchronize(st));
+  }
+  HIPX(hipStreamDestroy(st));
+  dfree(din);
+  dfree(dout);
+  return rc;
+}
+
 int tg_mlp_eval(int device, uint32_t D, uint32_t B, uint32_t P, const float* X, const float* mu,
                 float scale, const float* params, float margin, float* loss, float* grad,
                 float* codes) {
@@ -1018,9 +1047,9 @@ int tg_prune(tg_state* s, const tg_prune_params* p) {
   s->stats.prune_ms = ms;
   for (double& v : s->stats.prune_kernel_ms) v = 0.0;
   if (sec.collect(s->stats.prune_kernel_ms)) return TG_ERR_HIP;
-  unsigned long long cnt[10];
+  unsigned long long cnt[15];
   HIPX(hipMemcpy(cnt, s->counters, sizeof(cnt), hipMemcpyDeviceToHost));
-  for (int i = 0; i < 10; ++i) s->stats.prune_counters[i] = cnt[i];
+  for (int i = 0; i < 15; ++i) s->stats.prune_counters[i] = cnt[i];
   if (s->nseg) {
     s->seg_ep_host.resize(s->nseg);
     if (p->kind == TG_KIND_PATH)

@@ -302,6 +302,84 @@ __device__ __forceinline__ float rule_eval(int rule, float res, float dis, float
 }
 
 // ---------------------------------------------------------------------------------------
+// Register sorts.  A wave holds 64*NPL keys; key e = r*64 + lane sits in slot r of lane
+// `lane`.  A compare-exchange step with distance j < 64 takes the partner's key from lane
+// lane ^ j: DPP for j = 1, 2 (quad permutations), 4 (row shifts by 4, selected per 4-lane
+// bank) and 8 (row rotation by 8), ds_swizzle for j = 16 and ds_bpermute for j = 32; steps
+// with j >= 64 swap slots inside the lane.  No LDS storage and no barriers.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t lane_xor32(uint32_t v, int j) {
+  const int x = int(v);
+  switch (j) {
+    case 1: return uint32_t(__builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false));
+    case 2: return uint32_t(__builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false));
+    case 4: {
+      // banks 0 and 2 of each 16-lane row read lane + 4, banks 1 and 3 read lane - 4
+      const int lo = __builtin_amdgcn_update_dpp(x, x, 0x104, 0xF, 0x5, false);
+      return uint32_t(__builtin_amdgcn_update_dpp(lo, x, 0x114, 0xF, 0xA, false));
+    }
+    case 8: return uint32_t(__builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, false));
+    case 16: return uint32_t(__builtin_amdgcn_ds_swizzle(x, 0x401F));  // xor 16 within 32
+    default: return uint32_t(__builtin_amdgcn_ds_bpermute(int((lane_id() ^ j) << 2), x));
+  }
+}
+__device__ __forceinline__ uint64_t lane_xor64(uint64_t v, int j) {
+  return (uint64_t(lane_xor32(uint32_t(v >> 32), j)) << 32) | lane_xor32(uint32_t(v), j);
+}
+
+// Ascending bitonic sort of the 64*NPL keys v (layout above).  The direction tests depend
+// only on lane bits, so they are formed once as lane masks; a step then costs one 64-bit
+// compare and two selects per key besides the exchange.
+template <int NPL>
+__device__ __forceinline__ void reg_bitonic(uint64_t (&v)[NPL]) {
+  const int lane = lane_id();
+  bool bit[6];
+#pragma unroll
+  for (int b = 0; b < 6; ++b) bit[b] = (lane >> b) & 1;
+  constexpr int N = 64 * NPL;
+#pragma unroll
+  for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j >= 64) {
+        const int rj = j >> 6;
+#pragma unroll
+        for (int r = 0; r < NPL; ++r) {
+          if (r & rj) continue;
+          const int r2 = r | rj;
+          const bool asc = ((r << 6) & k) == 0;  // k >= 128 here
+          const uint64_t x = v[r], y = v[r2];
+          const bool sw = (y < x) == asc;
+          v[r] = sw ? y : x;
+          v[r2] = sw ? x : y;
+        }
+      } else {
+        const bool lower = !bit[__builtin_ctz(unsigned(j))];
+#pragma unroll
+        for (int r = 0; r < NPL; ++r) {
+          const bool asc = k >= 64 ? ((r << 6) & k) == 0 : !bit[__builtin_ctz(unsigned(k))];
+          const uint64_t x = v[r], p = lane_xor64(x, j);
+          // the lower position of an ascending block keeps the smaller key
+          const bool take = (p < x) == (lower == asc);
+          v[r] = take ? p : x;
+        }
+      }
+    }
+  }
+}
+
+// Sorts the bitonic sequence v (one key per lane, 64 keys) ascending.
+__device__ __forceinline__ void reg_bitonic_merge64(uint64_t& v) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int j = 32; j > 0; j >>= 1) {
+    const bool lower = (lane & j) == 0;
+    const uint64_t p = lane_xor64(v, j);
+    v = ((p < v) == lower) ? p : v;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
 // Wave-level sorts in LDS.  Called by all 64 lanes of a one-wave workgroup; n is a power of
 // two.  __syncthreads() orders the LDS traffic between network stages.
 // ---------------------------------------------------------------------------------------

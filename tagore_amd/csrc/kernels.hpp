@@ -144,6 +144,8 @@ void pb_group_overflow(const PushBuf& pb, uint32_t N, uint32_t* fill, uint32_t* 
 size_t scan_tmp_words(uint32_t N);
 
 void fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+void launch_selftest_sort(const uint64_t* in, uint64_t* out, int npl, int mode, uint32_t groups,
+                          hipStream_t s);
 void fill_u8(uint8_t* p, uint8_t v, size_t n, hipStream_t s);
 
 // synthetic data: kind 0 integer GMM in [0,255], 1 unit-norm GMM, 2 |GMM| scaled to [0,1]

@@ -68,7 +68,7 @@ __device__ uint64_t* beam_search(const PruneArgs& a, const Row16<MC>& tgt, uint3
                                  int max_exp, uint64_t* M0, uint64_t* M1, uint64_t* S,
                                  uint32_t* tab, uint32_t* expl, uint16_t* cl, uint64_t* ring,
                                  uint32_t& ring_c, uint32_t& n_eval, uint32_t& n_adj,
-                                 long long* ev = nullptr) {
+                                 long long* ev = nullptr, long long* sub = nullptr) {
   const int lane = threadIdx.x;
   const int K = a.K, TOPM = a.TOPM, D = a.D, nK = pow2_at_least(K);
   const T* X = reinterpret_cast<const T*>(a.rows);
@@ -99,6 +99,7 @@ __device__ uint64_t* beam_search(const PruneArgs& a, const Row16<MC>& tgt, uint3
   uint64_t* nxt = M1;
   int nexpl = 0;
   for (int r = 0; r < max_exp; ++r) {
+    const long long s0 = sub ? pclock(a) : 0;
     // first explorable beam entry
     int first = -1;
     for (int base = 0; base < TOPM && first < 0; base += TG_WAVE) {
@@ -140,6 +141,7 @@ __device__ uint64_t* beam_search(const PruneArgs& a, const Row16<MC>& tgt, uint3
     __syncthreads();
     const uint32_t c0 = ring_c;
     const long long e0 = ev ? pclock(a) : 0;
+    if (sub) sub[0] += e0 - s0;
     eval_batch<MC>(X, D, tgt, cnt, [&](int q) { return key_id(S[cl[q]]); },
                    [&](int q, float d) {
                      const int j = cl[q];
@@ -151,17 +153,147 @@ __device__ uint64_t* beam_search(const PruneArgs& a, const Row16<MC>& tgt, uint3
     ring_c += cnt;
     n_eval += cnt;
     __syncthreads();
+    const long long s1 = sub ? pclock(a) : 0;
     wave_bitonic(S, nK);
+    const long long s2 = sub ? pclock(a) : 0;
     merge_lds(cur, TOPM, S, K, nxt);
+    __syncthreads();
+    const long long s3 = sub ? pclock(a) : 0;
     hset_clear(tab);
     __syncthreads();
     for (int i = lane; i < TOPM + K; i += TG_WAVE) hset_insert(tab, key_id(nxt[i]));
     __syncthreads();
+    if (sub) {
+      const long long s4 = pclock(a);
+      sub[1] += s2 - s1;
+      sub[2] += s3 - s2;
+      sub[3] += s4 - s3;
+    }
     uint64_t* tmp = cur;
     cur = nxt;
     nxt = tmp;
   }
   return cur;
+}
+
+// -------------------------------------------------------------------------------------
+// The same beam search for TOPM <= 64 with the beam in registers: lane l holds the beam's
+// l-th key (lanes >= TOPM hold TG_KEY_INF).  A round sorts the K new keys in registers
+// (64*NPL >= K slots, reg_bitonic) and takes the TOPM smallest of beam and candidates with
+// one bitonic merge: min(beam[l], cand[63-l]) is bitonic and holds the 64 smallest of both,
+// so sorting it gives the first 64 of merge(beam, candidates).  Explored entries are those
+// whose id is in `expl`, as above.  S (64*NPL keys) receives the candidates by list
+// position; `out` (optional) receives the final beam (TOPM keys).
+// -------------------------------------------------------------------------------------
+template <int MC, class T, int NPL>
+__device__ void beam_search_reg(const PruneArgs& a, const Row16<MC>& tgt, uint32_t start,
+                                int max_exp, uint64_t* S, uint32_t* tab, uint32_t* expl,
+                                uint16_t* cl, uint64_t* ring, uint64_t* out, uint32_t& ring_c,
+                                uint32_t& n_eval, uint32_t& n_adj, long long* ev = nullptr,
+                                long long* sub = nullptr) {
+  const int lane = threadIdx.x;
+  const int K = a.K, TOPM = a.TOPM, D = a.D;
+  constexpr int NS = 64 * NPL;
+  const T* X = reinterpret_cast<const T*>(a.rows);
+  const uint64_t INFKEY = mk_key(__uint_as_float(0x7F800000u), 0u);
+  for (int j = lane; j < NS; j += TG_WAVE)
+    S[j] = j < K ? (INFKEY | a.G[size_t(start) * K + j]) : TG_KEY_INF;
+  __syncthreads();
+  eval_batch<MC>(X, D, tgt, K, [&](int q) { return key_id(S[q]); },
+                 [&](int q, float d) {
+                   const uint64_t k = mk_key(d, key_id(S[q]));
+                   S[q] = k;
+                   if (ring) ring[q % RING] = k;
+                 });
+  n_eval += K;
+  n_adj += 1;
+  ring_c = K;
+  hset_clear(tab);
+  __syncthreads();
+  uint64_t v[NPL];
+#pragma unroll
+  for (int r = 0; r < NPL; ++r) {
+    v[r] = S[r * 64 + lane];
+    hset_insert(tab, key_id(v[r]));  // the first dedup set is ids(S); padding ids are INV
+  }
+  reg_bitonic<NPL>(v);
+  uint64_t bk = lane < TOPM ? v[0] : TG_KEY_INF;
+  __syncthreads();
+  int nexpl = 0;
+  for (int r = 0; r < max_exp; ++r) {
+    const long long s0 = sub ? pclock(a) : 0;
+    // first explorable beam entry
+    bool ok = key_finite(bk);
+    if (ok) {
+      const uint32_t id = key_id(bk);
+      for (int e = 0; e < nexpl; ++e) ok = ok && (expl[e] != id);
+    }
+    const uint64_t m = ballot(ok);
+    if (!m) break;
+    const int first = __ffsll((unsigned long long)m) - 1;
+    const uint32_t x = uint32_t(__shfl(int(key_id(bk)), first, TG_WAVE));
+    __syncthreads();
+    if (lane == 0) expl[nexpl] = x;
+    ++nexpl;
+    n_adj += 1;
+    // new candidates: duplicates of the dedup set get +inf; the rest compacted in j order
+    int cnt = 0;
+#pragma unroll
+    for (int base = 0; base < NS; base += TG_WAVE) {
+      const int j = base + lane;
+      bool fresh = false;
+      if (j < K) {
+        const uint32_t y = a.G[size_t(x) * K + j];
+        fresh = !hset_has(tab, y);
+        S[j] = INFKEY | y;
+      } else {
+        S[j] = TG_KEY_INF;
+      }
+      const uint64_t fm = ballot(fresh);
+      if (fresh) cl[cnt + mask_rank(fm)] = uint16_t(j);
+      cnt += __popcll(fm);
+    }
+    __syncthreads();
+    const uint32_t c0 = ring_c;
+    const long long e0 = ev ? pclock(a) : 0;
+    if (sub) sub[0] += e0 - s0;
+    eval_batch<MC>(X, D, tgt, cnt, [&](int q) { return key_id(S[cl[q]]); },
+                   [&](int q, float d) {
+                     const int j = cl[q];
+                     const uint64_t k = mk_key(d, key_id(S[j]));
+                     S[j] = k;
+                     if (ring) ring[(c0 + q) % RING] = k;
+                   });
+    if (ev) *ev += pclock(a) - e0;
+    ring_c += cnt;
+    n_eval += cnt;
+    __syncthreads();
+    const long long s1 = sub ? pclock(a) : 0;
+#pragma unroll
+    for (int q = 0; q < NPL; ++q) v[q] = S[q * 64 + lane];
+    // next dedup set: ids of this round's beam and candidates
+    hset_clear(tab);
+    __syncthreads();
+    hset_insert(tab, key_id(bk));
+#pragma unroll
+    for (int q = 0; q < NPL; ++q) hset_insert(tab, key_id(v[q]));
+    const long long s2 = sub ? pclock(a) : 0;
+    reg_bitonic<NPL>(v);
+    const long long s3 = sub ? pclock(a) : 0;
+    const uint64_t c = lane_xor64(v[0], 63);  // candidate 63 - lane
+    uint64_t nb = bk < c ? bk : c;
+    reg_bitonic_merge64(nb);
+    bk = lane < TOPM ? nb : TG_KEY_INF;
+    __syncthreads();
+    if (sub) {
+      const long long s4 = pclock(a);
+      sub[1] += s3 - s2;
+      sub[2] += s4 - s3;
+      sub[3] += s2 - s1;
+    }
+  }
+  if (out && lane < TOPM) out[lane] = bk;
+  __syncthreads();
 }
 
 // -------------------------------------------------------------------------------------
@@ -312,14 +444,17 @@ __device__ __forceinline__ void count_add(unsigned long long* c, int idx, uint32
 constexpr int RING_P = 512;
 
 struct FwdLayout {
-  int nK, W, offB, offCl, bytes;
+  int nK, W, NS, offB, offCl, bytes;
+  bool reg;  // register beam (TOPM <= 64): B holds S | tab | expl, no M0 / M1
 };
 __host__ __device__ inline FwdLayout fwd_layout(int K, int TOPM, int FD) {
   FwdLayout l;
   l.nK = pow2_at_least(K);
   l.W = TOPM + K;
+  l.reg = TOPM <= 64;
+  l.NS = l.reg ? (l.nK <= 64 ? 64 : 128) : l.nK;
   l.offB = (l.nK + RING_P) * 8;
-  const int beam = (2 * l.W + l.nK) * 8 + HSET * 4 + EXPL_MAX * 4;
+  const int beam = ((l.reg ? 0 : 2 * l.W) + l.NS) * 8 + HSET * 4 + EXPL_MAX * 4;
   const int pool = (K + RING + FD + 1) * 8;
   const int szB = ((beam > pool ? beam : pool) + 15) / 16 * 16;
   l.offCl = l.offB + szB;
@@ -339,8 +474,8 @@ __global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
   uint64_t* R = O + ly.nK;
   uint64_t* M0 = reinterpret_cast<uint64_t*>(dsm + ly.offB);
   uint64_t* M1 = M0 + ly.W;
-  uint64_t* S = M1 + ly.W;
-  uint32_t* tab = reinterpret_cast<uint32_t*>(S + ly.nK);
+  uint64_t* S = ly.reg ? M0 : M1 + ly.W;
+  uint32_t* tab = reinterpret_cast<uint32_t*>(S + ly.NS);
   uint32_t* expl = tab + HSET;
   uint64_t* P = reinterpret_cast<uint64_t*>(dsm + ly.offB);  // after the beam
   const int L = K + RING;
@@ -354,13 +489,22 @@ __global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
     R[i] = i < RING ? mk_key(__uint_as_float(0x7F800000u), TG_INV) : TG_KEY_INF;
   __syncthreads();
   uint32_t n_col = 0, n_adj = 0, n_flt = 0;
-  long long t_ev = 0;
+  long long t_ev = 0, t_sub[4] = {0, 0, 0, 0};
   long long* ev = a.phase_clock ? &t_ev : nullptr;
+  long long* sub = a.phase_clock ? t_sub : nullptr;
   const long long c0 = pclock(a);
   if (KIND == 0) {
     uint32_t ring_c = 0;
     const uint32_t ep = a.ep[a.seg_of ? a.seg_of[u] : 0];
-    beam_search<MC, T>(a, xu, ep, 10, M0, M1, S, tab, expl, cl, R, ring_c, n_col, n_adj, ev);
+    if (!ly.reg)
+      beam_search<MC, T>(a, xu, ep, 10, M0, M1, S, tab, expl, cl, R, ring_c, n_col, n_adj, ev,
+                         sub);
+    else if (ly.NS == 64)
+      beam_search_reg<MC, T, 1>(a, xu, ep, 10, S, tab, expl, cl, R, nullptr, ring_c, n_col,
+                                n_adj, ev, sub);
+    else
+      beam_search_reg<MC, T, 2>(a, xu, ep, 10, S, tab, expl, cl, R, nullptr, ring_c, n_col,
+                                n_adj, ev, sub);
   } else {
     // two-hop ids y_i = G[G[u][i/K]][i%K], i < 480
     for (int i = lane; i < RING; i += TG_WAVE) {
@@ -377,8 +521,27 @@ __global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
   const long long c1 = pclock(a);
   // pool = merge(sorted own list, sorted ring); entries repeating their predecessor's id
   // get id INV
-  wave_bitonic(O, ly.nK);
-  wave_bitonic(R, RING_P);
+  // sort the own list and the ring in registers
+  if (ly.nK <= 64) {
+    uint64_t o[1] = {lane < ly.nK ? O[lane] : TG_KEY_INF};
+    reg_bitonic<1>(o);
+    if (lane < ly.nK) O[lane] = o[0];
+  } else {
+    uint64_t o[2] = {O[lane], O[64 + lane]};
+    reg_bitonic<2>(o);
+    O[lane] = o[0];
+    O[64 + lane] = o[1];
+  }
+  {
+    uint64_t rr[RING_P / 64];
+#pragma unroll
+    for (int q = 0; q < RING_P / 64; ++q) rr[q] = R[q * 64 + lane];
+    reg_bitonic<RING_P / 64>(rr);
+#pragma unroll
+    for (int q = 0; q < RING_P / 64; ++q) R[q * 64 + lane] = rr[q];
+  }
+  __syncthreads();
+  const long long c1s = pclock(a);
   merge_lds(O, K, R, RING, P);
   __syncthreads();
   constexpr int PQ = (128 + RING + TG_WAVE - 1) / TG_WAVE;
@@ -402,6 +565,8 @@ __global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
     atomicAdd(&a.counters[7], (unsigned long long)(c3 - c2));
     atomicAdd(&a.counters[8], (unsigned long long)(c2 - c1));
     atomicAdd(&a.counters[9], (unsigned long long)t_ev);
+    for (int i = 0; i < 4; ++i) atomicAdd(&a.counters[10 + i], (unsigned long long)t_sub[i]);
+    atomicAdd(&a.counters[14], (unsigned long long)(c1s - c1));
   }
   for (int k = lane; k < deg; k += TG_WAVE) {
     const uint64_t s = sel[k];
@@ -615,7 +780,13 @@ __global__ __launch_bounds__(64) void k_cal_ep(PruneArgs a, const float* centroi
   Row16<MC> c;
   load_row16<MC>(c, centroid + size_t(seg) * a.D, a.D, threadIdx.x & 15);
   uint32_t rc = 0, ne = 0, na = 0;
-  const uint64_t* B = beam_search<MC, T>(a, c, start, 30, M0, M1, S, tab, expl, cl, nullptr, rc, ne, na);
+  const uint64_t* B = M0;
+  if (a.TOPM > 64)
+    B = beam_search<MC, T>(a, c, start, 30, M0, M1, S, tab, expl, cl, nullptr, rc, ne, na);
+  else if (a.K <= 64)
+    beam_search_reg<MC, T, 1>(a, c, start, 30, S, tab, expl, cl, nullptr, M0, rc, ne, na);
+  else
+    beam_search_reg<MC, T, 2>(a, c, start, 30, S, tab, expl, cl, nullptr, M0, rc, ne, na);
   if (threadIdx.x == 0) ep[seg] = key_id(B[0]);
   count_add(a.counters, 4, ne);
 }

@@ -386,4 +386,36 @@ void launch_recall(const float* X, uint32_t N, uint32_t D, const uint32_t* G, ui
   k_recall_full<<<nq, 256, 0, s>>>(X, N, D, q, nq, k, gd64, kth, hits);
 }
 
+// ------------------------------------------------------- register sort self-test
+// One wave per group of 64*NPL keys.  mode 0: reg_bitonic; mode 1 (NPL 1): the 64 keys form
+// a bitonic sequence, reg_bitonic_merge64 sorts it; mode 2 (NPL 2): slots 0 and 1 hold two
+// ascending lists A and B (key l of each in lane l), the output's slot 0 is the 64 smallest
+// keys of A and B, ascending -- the beam update of the prune kernels.
+template <int NPL>
+__global__ __launch_bounds__(64) void k_selftest_sort(const uint64_t* in, uint64_t* out, int mode) {
+  const int lane = threadIdx.x;
+  const size_t base = size_t(blockIdx.x) * 64 * NPL;
+  uint64_t v[NPL];
+#pragma unroll
+  for (int r = 0; r < NPL; ++r) v[r] = in[base + r * 64 + lane];
+  if (mode == 0) {
+    reg_bitonic<NPL>(v);
+  } else if (mode == 1) {
+    reg_bitonic_merge64(v[0]);
+  } else if (NPL >= 2) {
+    const uint64_t b = lane_xor64(v[1], 63);
+    v[0] = v[0] < b ? v[0] : b;
+    reg_bitonic_merge64(v[0]);
+  }
+#pragma unroll
+  for (int r = 0; r < NPL; ++r) out[base + r * 64 + lane] = v[r];
+}
+
+void launch_selftest_sort(const uint64_t* in, uint64_t* out, int npl, int mode, uint32_t groups,
+                          hipStream_t s) {
+  if (npl == 1) k_selftest_sort<1><<<groups, 64, 0, s>>>(in, out, mode);
+  else if (npl == 2) k_selftest_sort<2><<<groups, 64, 0, s>>>(in, out, mode);
+  else k_selftest_sort<8><<<groups, 64, 0, s>>>(in, out, mode);
+}
+
 }  // namespace tg
