@@ -60,7 +60,7 @@ typedef struct {
   int32_t precision;  /* TG_PREC_* */
   int32_t init;       /* TG_INIT_* */
   uint32_t sample;    /* learned init: sample size (0: min(N, 50000)) */
-  uint32_t epochs;    /* learned init: training epochs (0: 5) */
+  uint32_t epochs;    /* learned init: training epochs (0: 2) */
   uint32_t tile;      /* local-join MFMA staging depth: 0 = 256-byte rows per LDS round,
                          1 = 64-byte rows (results are identical; see tg_autotune_knn) */
   uint32_t flags;     /* TG_FLAG_*: 0 for the defaults */

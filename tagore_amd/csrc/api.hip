@@ -692,7 +692,7 @@ int learned_init(tg_state* s, const tg_knn_params* p, uint64_t* evals) {
   s->sub->has_knn = s->sub->prep_valid = s->sub->has_prune = false;
   tg_knn_params sp{};
   sp.K = Ks;
-  sp.iters = 6;
+  sp.iters = 4;
   sp.seed = p->seed ^ 0x5A3D1E5Bull;
   sp.precision = TG_PREC_EXACT;
   sp.init = TG_INIT_RANDOM;

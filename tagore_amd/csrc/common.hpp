@@ -326,12 +326,14 @@ __device__ __forceinline__ uint32_t lane_xor32(uint32_t v, int j) {
 __device__ __forceinline__ uint64_t lane_xor64(uint64_t v, int j) {
   return (uint64_t(lane_xor32(uint32_t(v >> 32), j)) << 32) | lane_xor32(uint32_t(v), j);
 }
+__device__ __forceinline__ uint32_t lane_xor(uint32_t v, int j) { return lane_xor32(v, j); }
+__device__ __forceinline__ uint64_t lane_xor(uint64_t v, int j) { return lane_xor64(v, j); }
 
 // Ascending bitonic sort of the 64*NPL keys v (layout above).  The direction tests depend
 // only on lane bits, so they are formed once as lane masks; a step then costs one 64-bit
 // compare and two selects per key besides the exchange.
-template <int NPL>
-__device__ __forceinline__ void reg_bitonic(uint64_t (&v)[NPL]) {
+template <int NPL, class T>
+__device__ __forceinline__ void reg_bitonic(T (&v)[NPL]) {
   const int lane = lane_id();
   bool bit[6];
 #pragma unroll
@@ -348,7 +350,7 @@ __device__ __forceinline__ void reg_bitonic(uint64_t (&v)[NPL]) {
           if (r & rj) continue;
           const int r2 = r | rj;
           const bool asc = ((r << 6) & k) == 0;  // k >= 128 here
-          const uint64_t x = v[r], y = v[r2];
+          const T x = v[r], y = v[r2];
           const bool sw = (y < x) == asc;
           v[r] = sw ? y : x;
           v[r2] = sw ? x : y;
@@ -358,7 +360,7 @@ __device__ __forceinline__ void reg_bitonic(uint64_t (&v)[NPL]) {
 #pragma unroll
         for (int r = 0; r < NPL; ++r) {
           const bool asc = k >= 64 ? ((r << 6) & k) == 0 : !bit[__builtin_ctz(unsigned(k))];
-          const uint64_t x = v[r], p = lane_xor64(x, j);
+          const T x = v[r], p = lane_xor(x, j);
           // the lower position of an ascending block keeps the smaller key
           const bool take = (p < x) == (lower == asc);
           v[r] = take ? p : x;
@@ -403,6 +405,35 @@ __device__ void wave_bitonic(T* a, int n) {
       __syncthreads();
     }
   }
+}
+
+// In-place ascending sort of a[0..n) (n a power of two) by a one-wave workgroup: the keys
+// go through registers (reg_bitonic, 64*NPL >= n slots, padding ~0 sorts last).  Larger n,
+// or workgroups of several waves, use the LDS network above.  MAXN (<= 512) bounds the
+// register variants compiled at a call site.
+template <int NPL, class T>
+__device__ __forceinline__ void reg_sort_lds(T* a, int n) {
+  const int lane = lane_id();
+  T v[NPL];
+#pragma unroll
+  for (int r = 0; r < NPL; ++r) v[r] = r * 64 + lane < n ? a[r * 64 + lane] : ~T(0);
+  reg_bitonic<NPL>(v);
+#pragma unroll
+  for (int r = 0; r < NPL; ++r)
+    if (r * 64 + lane < n) a[r * 64 + lane] = v[r];
+}
+template <int MAXN, class T>
+__device__ void wave_sort(T* a, int n) {
+  static_assert(MAXN <= 512, "register sorts hold at most 512 keys");
+  if (blockDim.x != TG_WAVE || n > MAXN) {
+    wave_bitonic(a, n);
+    return;
+  }
+  if (n <= 64) reg_sort_lds<1>(a, n);
+  else if (MAXN >= 128 && n <= 128) reg_sort_lds<(MAXN >= 128 ? 2 : 1)>(a, n);
+  else if (MAXN >= 256 && n <= 256) reg_sort_lds<(MAXN >= 256 ? 4 : 1)>(a, n);
+  else reg_sort_lds<(MAXN >= 512 ? 8 : 1)>(a, n);
+  __syncthreads();
 }
 
 __host__ __device__ __forceinline__ int pow2_at_least(int n) {
@@ -488,7 +519,7 @@ __device__ int pb_top(const PushBuf& pb, uint32_t v, int R, bool uniq, Pred pred
       buf[128 + i] = k;
     }
     __syncthreads();
-    wave_bitonic(buf, 256);
+    wave_sort<256>(buf, 256);
     // compaction: keep first R (unique if requested) into buf[0..R)
     uint64_t mine[4];
     bool keep[4];

@@ -79,7 +79,7 @@ __global__ __launch_bounds__(64) void k_init_sort(KnnArgs a) {
   eval_batch<MC>(a.X, D, xu, K, [&](int q) { return row[q]; },
                  [&](int q, float d) { kk[q] = mk_key(d, row[q]); });
   __syncthreads();
-  wave_bitonic(kk, n);
+  wave_sort<128>(kk, n);
   for (int j = lane; j < K; j += TG_WAVE) {
     a.G[size_t(u) * K + j] = key_id(kk[j]);
     a.GD[size_t(u) * K + j] = key_d(kk[j]);
@@ -299,7 +299,7 @@ __device__ void load_row_lds(const KnnArgs& a, const uint32_t* G, const float* G
     }
   }
   __syncthreads();
-  wave_bitonic(r.gid, n);
+  wave_sort<128>(r.gid, n);
 }
 
 __device__ __forceinline__ bool listed(const RowLds& r, int K, uint32_t id) {
@@ -490,7 +490,7 @@ __global__ __launch_bounds__(64) void k_phase2(KnnArgs a) {
   const int n = pow2_at_least(max(tot, 1));
   for (int i = tot + lane; i < n; i += TG_WAVE) ids[i] = TG_INV;
   __syncthreads();
-  wave_bitonic(ids, n);
+  wave_sort<512>(ids, n);
   // unique, not u, not listed
   int cnt = 0;
   for (int base = 0; base < tot; base += TG_WAVE) {
@@ -567,7 +567,7 @@ __global__ __launch_bounds__(64) void k_phase2(KnnArgs a) {
                  });
   if (a.counters && lane == 0 && nre) atomicAdd(&a.counters[1], (unsigned long long)nre);
   __syncthreads();
-  wave_bitonic(kv, nk);
+  wave_sort<512>(kv, nk);
   int nc = 0;
   for (int base = 0; base < nk; base += TG_WAVE) {
     const int i = base + lane;

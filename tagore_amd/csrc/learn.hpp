@@ -57,7 +57,7 @@ struct Learner {
 struct LearnConfig {
   uint32_t B = 1024;      // anchors per batch
   uint32_t P = 4;         // positives per anchor
-  uint32_t epochs = 5;
+  uint32_t epochs = 2;
   uint32_t kmeans_iters = 5;
   uint32_t bucket = 1024; // target vectors per bucket: C = clamp(N / bucket, 1, 4096)
   float lr = 3e-3f, margin = 1.0f;

@@ -302,7 +302,7 @@ __device__ void beam_search_reg(const PruneArgs& a, const Row16<MC>& tgt, uint32
 // -------------------------------------------------------------------------------------
 __device__ void sort_dedup(uint64_t* P, int n) {
   const int lane = threadIdx.x;
-  wave_bitonic(P, n);
+  wave_sort<256>(P, n);
   bool dup[POOL_MAX / TG_WAVE];
 #pragma unroll
   for (int q = 0; q < POOL_MAX / TG_WAVE; ++q) {
@@ -668,7 +668,7 @@ __global__ __launch_bounds__(64) void k_rank_forward(PruneArgs a) {
     for (int j = lane; j < n; j += TG_WAVE)
       srt[j] = j < K ? ((uint64_t(nei[j]) << 32) | uint32_t(j)) : TG_KEY_INF;
     __syncthreads();
-    wave_bitonic(srt, n);
+    wave_sort<128>(srt, n);
     for (int r = 0; r < K; ++r) {
       const uint32_t v = nei[r];
       for (int l = lane; l < K; l += TG_WAVE) {
@@ -702,7 +702,7 @@ __global__ __launch_bounds__(64) void k_rank_forward(PruneArgs a) {
   for (int j = lane; j < n; j += TG_WAVE)
     srt[j] = j < K ? ((uint64_t(det[j]) << 32) | nei[j]) : TG_KEY_INF;
   __syncthreads();
-  wave_bitonic(srt, n);
+  wave_sort<128>(srt, n);
   for (int i = lane; i < d; i += TG_WAVE) {
     const uint64_t k = srt[i];
     a.fwd_ids[size_t(u) * d + i] = uint32_t(k);
