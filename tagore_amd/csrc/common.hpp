@@ -329,10 +329,10 @@ __device__ __forceinline__ uint64_t lane_xor64(uint64_t v, int j) {
 __device__ __forceinline__ uint32_t lane_xor(uint32_t v, int j) { return lane_xor32(v, j); }
 __device__ __forceinline__ uint64_t lane_xor(uint64_t v, int j) { return lane_xor64(v, j); }
 
-// Ascending bitonic sort of the 64*NPL keys v (layout above).  The direction tests depend
-// only on lane bits, so they are formed once as lane masks; a step then costs one 64-bit
-// compare and two selects per key besides the exchange.
-template <int NPL, class T>
+// Bitonic sort of the 64*NPL keys v (layout above), ascending (descending with DESC).  The
+// direction tests depend only on lane bits, so they are formed once as lane masks; a step
+// then costs one 64-bit compare and two selects per key besides the exchange.
+template <int NPL, class T, bool DESC = false>
 __device__ __forceinline__ void reg_bitonic(T (&v)[NPL]) {
   const int lane = lane_id();
   bool bit[6];
@@ -349,7 +349,7 @@ __device__ __forceinline__ void reg_bitonic(T (&v)[NPL]) {
         for (int r = 0; r < NPL; ++r) {
           if (r & rj) continue;
           const int r2 = r | rj;
-          const bool asc = ((r << 6) & k) == 0;  // k >= 128 here
+          const bool asc = (((r << 6) & k) == 0) != DESC;  // k >= 128 here
           const T x = v[r], y = v[r2];
           const bool sw = (y < x) == asc;
           v[r] = sw ? y : x;
@@ -359,7 +359,7 @@ __device__ __forceinline__ void reg_bitonic(T (&v)[NPL]) {
         const bool lower = !bit[__builtin_ctz(unsigned(j))];
 #pragma unroll
         for (int r = 0; r < NPL; ++r) {
-          const bool asc = k >= 64 ? ((r << 6) & k) == 0 : !bit[__builtin_ctz(unsigned(k))];
+          const bool asc = (k >= 64 ? ((r << 6) & k) == 0 : !bit[__builtin_ctz(unsigned(k))]) != DESC;
           const T x = v[r], p = lane_xor(x, j);
           // the lower position of an ascending block keeps the smaller key
           const bool take = (p < x) == (lower == asc);
@@ -370,15 +370,36 @@ __device__ __forceinline__ void reg_bitonic(T (&v)[NPL]) {
   }
 }
 
-// Sorts the bitonic sequence v (one key per lane, 64 keys) ascending.
-__device__ __forceinline__ void reg_bitonic_merge64(uint64_t& v) {
+// Sorts the bitonic sequence of the 64*NPL keys v (layout above) ascending.
+template <int NPL, class T>
+__device__ __forceinline__ void reg_bitonic_merge(T (&v)[NPL]) {
   const int lane = lane_id();
 #pragma unroll
-  for (int j = 32; j > 0; j >>= 1) {
-    const bool lower = (lane & j) == 0;
-    const uint64_t p = lane_xor64(v, j);
-    v = ((p < v) == lower) ? p : v;
+  for (int j = 32 * NPL; j > 0; j >>= 1) {
+    if (j >= 64) {
+      const int rj = j >> 6;
+#pragma unroll
+      for (int r = 0; r < NPL; ++r) {
+        if (r & rj) continue;
+        const T x = v[r], y = v[r | rj];
+        const bool sw = y < x;
+        v[r] = sw ? y : x;
+        v[r | rj] = sw ? x : y;
+      }
+    } else {
+      const bool lower = (lane & j) == 0;
+#pragma unroll
+      for (int r = 0; r < NPL; ++r) {
+        const T x = v[r], p = lane_xor(x, j);
+        v[r] = ((p < x) == lower) ? p : x;
+      }
+    }
   }
+}
+__device__ __forceinline__ void reg_bitonic_merge64(uint64_t& v) {
+  uint64_t w[1] = {v};
+  reg_bitonic_merge<1>(w);
+  v = w[0];
 }
 
 // ---------------------------------------------------------------------------------------

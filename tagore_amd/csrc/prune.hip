@@ -24,7 +24,6 @@ __device__ __forceinline__ long long pclock(const PruneArgs& a) {
   return a.phase_clock ? clock64() : 0;
 }
 constexpr int POOL_MAX = 1024;  // K + 480 <= 608, padded to a power of two
-constexpr int W_MAX = 256;      // TOPM + K <= 256
 constexpr int EXPL_MAX = 32;
 
 // -------------------------------------------------------------------------------------
@@ -56,146 +55,29 @@ __device__ __forceinline__ bool hset_has(const uint32_t* tab, uint32_t id) {
 
 // -------------------------------------------------------------------------------------
 // Beam search shared by cal_ep and path Collect (spec in the oracle's beam_search).
-// M0/M1 (ping-pong, TOPM + K sorted keys): the beam is the first TOPM; S: the K newest
-// candidates; tab: id set of the previous round's beam and candidates (the dedup set).
-// The explored flag of a beam entry is represented by membership of its id in `expl`:
-// the beam never holds two finite entries with one id, and an expanded node that falls
-// out of the beam can never re-enter it (the beam's TOPM-th key only decreases), so the
-// two representations agree.  Returns the final sorted array (beam = first TOPM).
+// The beam lives in registers: position p = b*64 + lane is slot b of lane `lane`
+// (b < W, TOPM <= 64*W; positions >= TOPM hold TG_KEY_INF).  A round evaluates the fresh
+// candidates of the expanded node, sorts all K candidates in registers (S holds them by
+// list position, 64*W >= K slots) and takes the TOPM smallest of beam and candidates with
+// one bitonic merge: min(beam[p], cand[64W-1-p]) is bitonic and holds the 64W smallest of
+// both, so sorting it gives the first 64W keys of merge(beam, candidates).
+// tab: id set of the previous round's beam and candidates (the dedup set).  An entry is
+// explored when its id is in `expl`: the beam never holds two finite entries with one id,
+// and an expanded node that falls out of the beam can never re-enter it (the beam's TOPM-th
+// key only decreases).  `out` (optional) receives the final beam (TOPM keys).
 // -------------------------------------------------------------------------------------
-template <int MC, class T>
-__device__ uint64_t* beam_search(const PruneArgs& a, const Row16<MC>& tgt, uint32_t start,
-                                 int max_exp, uint64_t* M0, uint64_t* M1, uint64_t* S,
-                                 uint32_t* tab, uint32_t* expl, uint16_t* cl, uint64_t* ring,
-                                 uint32_t& ring_c, uint32_t& n_eval, uint32_t& n_adj,
-                                 long long* ev = nullptr, long long* sub = nullptr) {
-  const int lane = threadIdx.x;
-  const int K = a.K, TOPM = a.TOPM, D = a.D, nK = pow2_at_least(K);
-  const T* X = reinterpret_cast<const T*>(a.rows);
-  const uint64_t INFKEY = mk_key(__uint_as_float(0x7F800000u), 0u);
-  // start list S = G[start] with distances (ring slots 0..K-1 in list order), sorted
-  for (int j = lane; j < nK; j += TG_WAVE)
-    S[j] = j < K ? (INFKEY | a.G[size_t(start) * K + j]) : TG_KEY_INF;
-  __syncthreads();
-  eval_batch<MC>(X, D, tgt, K, [&](int q) { return key_id(S[q]); },
-                 [&](int q, float d) {
-                   const uint64_t k = mk_key(d, key_id(S[q]));
-                   S[q] = k;
-                   if (ring) ring[q % RING] = k;
-                 });
-  n_eval += K;
-  n_adj += 1;
-  ring_c = K;
-  hset_clear(tab);
-  __syncthreads();
-  wave_bitonic(S, nK);
-  // beam = S[0..min(TOPM,K)) padded; the first dedup set is ids(S)
-  for (int i = lane; i < TOPM + K; i += TG_WAVE) {
-    M0[i] = i < TOPM ? (i < K ? S[i] : TG_KEY_INF) : TG_KEY_INF;
-    if (i < K) hset_insert(tab, key_id(S[i]));
-  }
-  __syncthreads();
-  uint64_t* cur = M0;
-  uint64_t* nxt = M1;
-  int nexpl = 0;
-  for (int r = 0; r < max_exp; ++r) {
-    const long long s0 = sub ? pclock(a) : 0;
-    // first explorable beam entry
-    int first = -1;
-    for (int base = 0; base < TOPM && first < 0; base += TG_WAVE) {
-      const int i = base + lane;
-      bool ok = false;
-      if (i < TOPM) {
-        const uint64_t k = cur[i];
-        ok = key_finite(k);
-        if (ok) {
-          const uint32_t id = key_id(k);
-          for (int e = 0; e < nexpl; ++e) ok = ok && (expl[e] != id);
-        }
-      }
-      const uint64_t m = ballot(ok);
-      if (m) first = base + __ffsll((unsigned long long)m) - 1;
-    }
-    if (first < 0) break;
-    const uint32_t x = key_id(cur[first]);
-    __syncthreads();
-    if (lane == 0) expl[nexpl] = x;
-    ++nexpl;
-    n_adj += 1;
-    // new candidates: duplicates of the dedup set get +inf; the rest compacted in j order
-    int cnt = 0;
-    for (int base = 0; base < nK; base += TG_WAVE) {
-      const int j = base + lane;
-      bool fresh = false;
-      if (j < K) {
-        const uint32_t y = a.G[size_t(x) * K + j];
-        fresh = !hset_has(tab, y);
-        S[j] = INFKEY | y;
-      } else if (j < nK) {
-        S[j] = TG_KEY_INF;
-      }
-      const uint64_t m = ballot(fresh);
-      if (fresh) cl[cnt + mask_rank(m)] = uint16_t(j);
-      cnt += __popcll(m);
-    }
-    __syncthreads();
-    const uint32_t c0 = ring_c;
-    const long long e0 = ev ? pclock(a) : 0;
-    if (sub) sub[0] += e0 - s0;
-    eval_batch<MC>(X, D, tgt, cnt, [&](int q) { return key_id(S[cl[q]]); },
-                   [&](int q, float d) {
-                     const int j = cl[q];
-                     const uint64_t k = mk_key(d, key_id(S[j]));
-                     S[j] = k;
-                     if (ring) ring[(c0 + q) % RING] = k;
-                   });
-    if (ev) *ev += pclock(a) - e0;
-    ring_c += cnt;
-    n_eval += cnt;
-    __syncthreads();
-    const long long s1 = sub ? pclock(a) : 0;
-    wave_bitonic(S, nK);
-    const long long s2 = sub ? pclock(a) : 0;
-    merge_lds(cur, TOPM, S, K, nxt);
-    __syncthreads();
-    const long long s3 = sub ? pclock(a) : 0;
-    hset_clear(tab);
-    __syncthreads();
-    for (int i = lane; i < TOPM + K; i += TG_WAVE) hset_insert(tab, key_id(nxt[i]));
-    __syncthreads();
-    if (sub) {
-      const long long s4 = pclock(a);
-      sub[1] += s2 - s1;
-      sub[2] += s3 - s2;
-      sub[3] += s4 - s3;
-    }
-    uint64_t* tmp = cur;
-    cur = nxt;
-    nxt = tmp;
-  }
-  return cur;
-}
-
-// -------------------------------------------------------------------------------------
-// The same beam search for TOPM <= 64 with the beam in registers: lane l holds the beam's
-// l-th key (lanes >= TOPM hold TG_KEY_INF).  A round sorts the K new keys in registers
-// (64*NPL >= K slots, reg_bitonic) and takes the TOPM smallest of beam and candidates with
-// one bitonic merge: min(beam[l], cand[63-l]) is bitonic and holds the 64 smallest of both,
-// so sorting it gives the first 64 of merge(beam, candidates).  Explored entries are those
-// whose id is in `expl`, as above.  S (64*NPL keys) receives the candidates by list
-// position; `out` (optional) receives the final beam (TOPM keys).
-// -------------------------------------------------------------------------------------
-template <int MC, class T, int NPL>
-__device__ void beam_search_reg(const PruneArgs& a, const Row16<MC>& tgt, uint32_t start,
-                                int max_exp, uint64_t* S, uint32_t* tab, uint32_t* expl,
-                                uint16_t* cl, uint64_t* ring, uint64_t* out, uint32_t& ring_c,
-                                uint32_t& n_eval, uint32_t& n_adj, long long* ev = nullptr,
-                                long long* sub = nullptr) {
+template <int MC, class T, int W>
+__device__ void beam_search(const PruneArgs& a, const Row16<MC>& tgt, uint32_t start,
+                            int max_exp, uint64_t* S, uint32_t* tab, uint32_t* expl,
+                            uint16_t* cl, uint64_t* ring, uint64_t* out, uint32_t& ring_c,
+                            uint32_t& n_eval, uint32_t& n_adj, long long* ev = nullptr,
+                            long long* sub = nullptr) {
   const int lane = threadIdx.x;
   const int K = a.K, TOPM = a.TOPM, D = a.D;
-  constexpr int NS = 64 * NPL;
+  constexpr int NS = 64 * W;
   const T* X = reinterpret_cast<const T*>(a.rows);
   const uint64_t INFKEY = mk_key(__uint_as_float(0x7F800000u), 0u);
+  // start list S = G[start] with distances (ring slots 0..K-1 in list order)
   for (int j = lane; j < NS; j += TG_WAVE)
     S[j] = j < K ? (INFKEY | a.G[size_t(start) * K + j]) : TG_KEY_INF;
   __syncthreads();
@@ -210,28 +92,39 @@ __device__ void beam_search_reg(const PruneArgs& a, const Row16<MC>& tgt, uint32
   ring_c = K;
   hset_clear(tab);
   __syncthreads();
-  uint64_t v[NPL];
+  uint64_t v[W], bk[W];
 #pragma unroll
-  for (int r = 0; r < NPL; ++r) {
+  for (int r = 0; r < W; ++r) {
     v[r] = S[r * 64 + lane];
     hset_insert(tab, key_id(v[r]));  // the first dedup set is ids(S); padding ids are INV
   }
-  reg_bitonic<NPL>(v);
-  uint64_t bk = lane < TOPM ? v[0] : TG_KEY_INF;
+  reg_bitonic<W>(v);
+#pragma unroll
+  for (int b = 0; b < W; ++b) bk[b] = b * 64 + lane < TOPM ? v[b] : TG_KEY_INF;
   __syncthreads();
   int nexpl = 0;
   for (int r = 0; r < max_exp; ++r) {
     const long long s0 = sub ? pclock(a) : 0;
-    // first explorable beam entry
-    bool ok = key_finite(bk);
-    if (ok) {
-      const uint32_t id = key_id(bk);
-      for (int e = 0; e < nexpl; ++e) ok = ok && (expl[e] != id);
+    // first explorable beam entry (position order)
+    int first = -1;
+    uint64_t fk = 0;
+#pragma unroll
+    for (int b = 0; b < W; ++b) {
+      if (first >= 0) break;
+      bool ok = key_finite(bk[b]);
+      if (ok) {
+        const uint32_t id = key_id(bk[b]);
+        for (int e = 0; e < nexpl; ++e) ok = ok && (expl[e] != id);
+      }
+      const uint64_t m = ballot(ok);
+      if (m) {
+        const int fl = __ffsll((unsigned long long)m) - 1;
+        first = b * 64 + fl;
+        fk = uint64_t(__shfl(int(key_id(bk[b])), fl, TG_WAVE));
+      }
     }
-    const uint64_t m = ballot(ok);
-    if (!m) break;
-    const int first = __ffsll((unsigned long long)m) - 1;
-    const uint32_t x = uint32_t(__shfl(int(key_id(bk)), first, TG_WAVE));
+    if (first < 0) break;
+    const uint32_t x = uint32_t(fk);
     __syncthreads();
     if (lane == 0) expl[nexpl] = x;
     ++nexpl;
@@ -270,20 +163,27 @@ __device__ void beam_search_reg(const PruneArgs& a, const Row16<MC>& tgt, uint32
     __syncthreads();
     const long long s1 = sub ? pclock(a) : 0;
 #pragma unroll
-    for (int q = 0; q < NPL; ++q) v[q] = S[q * 64 + lane];
+    for (int q = 0; q < W; ++q) v[q] = S[q * 64 + lane];
     // next dedup set: ids of this round's beam and candidates
     hset_clear(tab);
     __syncthreads();
-    hset_insert(tab, key_id(bk));
 #pragma unroll
-    for (int q = 0; q < NPL; ++q) hset_insert(tab, key_id(v[q]));
+    for (int q = 0; q < W; ++q) {
+      hset_insert(tab, key_id(bk[q]));
+      hset_insert(tab, key_id(v[q]));
+    }
     const long long s2 = sub ? pclock(a) : 0;
-    reg_bitonic<NPL>(v);
+    reg_bitonic<W>(v);
     const long long s3 = sub ? pclock(a) : 0;
-    const uint64_t c = lane_xor64(v[0], 63);  // candidate 63 - lane
-    uint64_t nb = bk < c ? bk : c;
-    reg_bitonic_merge64(nb);
-    bk = lane < TOPM ? nb : TG_KEY_INF;
+    uint64_t c[W];
+#pragma unroll
+    for (int b = 0; b < W; ++b) {
+      const uint64_t rv = lane_xor64(v[W - 1 - b], 63);  // candidate 64W-1-p
+      c[b] = bk[b] < rv ? bk[b] : rv;
+    }
+    reg_bitonic_merge<W>(c);
+#pragma unroll
+    for (int b = 0; b < W; ++b) bk[b] = b * 64 + lane < TOPM ? c[b] : TG_KEY_INF;
     __syncthreads();
     if (sub) {
       const long long s4 = pclock(a);
@@ -292,7 +192,11 @@ __device__ void beam_search_reg(const PruneArgs& a, const Row16<MC>& tgt, uint32
       sub[3] += s2 - s1;
     }
   }
-  if (out && lane < TOPM) out[lane] = bk;
+  if (out) {
+#pragma unroll
+    for (int b = 0; b < W; ++b)
+      if (b * 64 + lane < TOPM) out[b * 64 + lane] = bk[b];
+  }
   __syncthreads();
 }
 
@@ -434,35 +338,31 @@ __device__ __forceinline__ void count_add(unsigned long long* c, int idx, uint32
 // Forward pass (Collect + Filter + Store reverse pushes).  kind 0: path, 1: two-hop.
 //
 // LDS (dynamic, sized from K, TOPM and FD so that more workgroups fit a CU):
-//   O | R : own list (pow2(K) keys) and the 480-slot ring padded to 512;
-//   B     : beam scratch M0 | M1 | S | tab | expl; once the beam is done, the merged pool
-//           (K + 480 keys) and the selected list;
-//   cl    : compaction indices (K + 480).
-// The pool is the merge of the sorted own list and the sorted ring, the same sequence a
-// sort of their union gives.
+//   A : the 480-slot ring (padded to 512 for its sort), then the pool (K + 480 keys);
+//   B : during the beam S | tab | expl | beam compaction indices; afterwards the selected
+//       list and the filter's compaction indices (K + 480).
+// The pool is the merge of the sorted own list and the sorted ring, formed in registers as
+// one bitonic merge of [own list ascending | +inf | ring descending] (1024 slots); it is
+// the sequence a sort of their union gives.
 // -------------------------------------------------------------------------------------
 constexpr int RING_P = 512;
 
 struct FwdLayout {
-  int nK, W, NS, offB, offCl, bytes;
-  bool reg;  // register beam (TOPM <= 64): B holds S | tab | expl, no M0 / M1
+  int W, NS, L, offB, bytes;
 };
 __host__ __device__ inline FwdLayout fwd_layout(int K, int TOPM, int FD) {
   FwdLayout l;
-  l.nK = pow2_at_least(K);
-  l.W = TOPM + K;
-  l.reg = TOPM <= 64;
-  l.NS = l.reg ? (l.nK <= 64 ? 64 : 128) : l.nK;
-  l.offB = (l.nK + RING_P) * 8;
-  const int beam = ((l.reg ? 0 : 2 * l.W) + l.NS) * 8 + HSET * 4 + EXPL_MAX * 4;
-  const int pool = (K + RING + FD + 1) * 8;
-  const int szB = ((beam > pool ? beam : pool) + 15) / 16 * 16;
-  l.offCl = l.offB + szB;
-  l.bytes = l.offCl + ((K + RING) * 2 + 15) / 16 * 16;
+  l.W = (K <= 64 && TOPM <= 64) ? 1 : 2;
+  l.NS = 64 * l.W;
+  l.L = K + RING;
+  l.offB = ((l.L > RING_P ? l.L : RING_P) * 8 + 15) / 16 * 16;
+  const int beam = l.NS * 8 + HSET * 4 + EXPL_MAX * 4 + l.NS * 2;
+  const int filt = (FD + 1) * 8 + l.L * 2;
+  l.bytes = l.offB + ((beam > filt ? beam : filt) + 15) / 16 * 16;
   return l;
 }
 
-template <int MC, class T, int KIND>
+template <int MC, class T, int KIND, int W>
 __global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
   extern __shared__ __attribute__((aligned(16))) unsigned char dsm[];
   const uint32_t u = row_of_block(a.perm, a.N);
@@ -470,21 +370,17 @@ __global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
   const int lane = threadIdx.x, t = lane & 15;
   const int K = a.K, D = a.D;
   const FwdLayout ly = fwd_layout(K, int(a.TOPM), int(a.FD));
-  uint64_t* O = reinterpret_cast<uint64_t*>(dsm);
-  uint64_t* R = O + ly.nK;
-  uint64_t* M0 = reinterpret_cast<uint64_t*>(dsm + ly.offB);
-  uint64_t* M1 = M0 + ly.W;
-  uint64_t* S = ly.reg ? M0 : M1 + ly.W;
+  uint64_t* R = reinterpret_cast<uint64_t*>(dsm);
+  uint64_t* P = R;  // after the pool is formed
+  uint64_t* S = reinterpret_cast<uint64_t*>(dsm + ly.offB);
   uint32_t* tab = reinterpret_cast<uint32_t*>(S + ly.NS);
   uint32_t* expl = tab + HSET;
-  uint64_t* P = reinterpret_cast<uint64_t*>(dsm + ly.offB);  // after the beam
-  const int L = K + RING;
-  uint64_t* sel = P + L;
-  uint16_t* cl = reinterpret_cast<uint16_t*>(dsm + ly.offCl);
+  uint16_t* clb = reinterpret_cast<uint16_t*>(expl + EXPL_MAX);
+  uint64_t* sel = S;  // after the beam
+  uint16_t* cl = reinterpret_cast<uint16_t*>(sel + a.FD + 1);
+  const int L = ly.L;
   Row16<MC> xu;
   load_row16<MC>(xu, reinterpret_cast<const T*>(a.rows) + size_t(u) * D, D, t);
-  for (int j = lane; j < ly.nK; j += TG_WAVE)
-    O[j] = j < K ? mk_key(a.GD[size_t(u) * K + j], a.G[size_t(u) * K + j]) : TG_KEY_INF;
   for (int i = lane; i < RING_P; i += TG_WAVE)
     R[i] = i < RING ? mk_key(__uint_as_float(0x7F800000u), TG_INV) : TG_KEY_INF;
   __syncthreads();
@@ -496,15 +392,8 @@ __global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
   if (KIND == 0) {
     uint32_t ring_c = 0;
     const uint32_t ep = a.ep[a.seg_of ? a.seg_of[u] : 0];
-    if (!ly.reg)
-      beam_search<MC, T>(a, xu, ep, 10, M0, M1, S, tab, expl, cl, R, ring_c, n_col, n_adj, ev,
-                         sub);
-    else if (ly.NS == 64)
-      beam_search_reg<MC, T, 1>(a, xu, ep, 10, S, tab, expl, cl, R, nullptr, ring_c, n_col,
-                                n_adj, ev, sub);
-    else
-      beam_search_reg<MC, T, 2>(a, xu, ep, 10, S, tab, expl, cl, R, nullptr, ring_c, n_col,
-                                n_adj, ev, sub);
+    beam_search<MC, T, W>(a, xu, ep, 10, S, tab, expl, clb, R, nullptr, ring_c, n_col, n_adj,
+                          ev, sub);
   } else {
     // two-hop ids y_i = G[G[u][i/K]][i%K], i < 480
     for (int i = lane; i < RING; i += TG_WAVE) {
@@ -519,31 +408,35 @@ __global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
   }
   __syncthreads();
   const long long c1 = pclock(a);
-  // pool = merge(sorted own list, sorted ring); entries repeating their predecessor's id
-  // get id INV
-  // sort the own list and the ring in registers
-  if (ly.nK <= 64) {
-    uint64_t o[1] = {lane < ly.nK ? O[lane] : TG_KEY_INF};
-    reg_bitonic<1>(o);
-    if (lane < ly.nK) O[lane] = o[0];
-  } else {
-    uint64_t o[2] = {O[lane], O[64 + lane]};
-    reg_bitonic<2>(o);
-    O[lane] = o[0];
-    O[64 + lane] = o[1];
-  }
+  // pool = merge(sorted own list, sorted ring), in registers
+  uint64_t pv[16];
   {
+    uint64_t o[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int j = r * 64 + lane;
+      o[r] = j < K ? mk_key(a.GD[size_t(u) * K + j], a.G[size_t(u) * K + j]) : TG_KEY_INF;
+    }
     uint64_t rr[RING_P / 64];
 #pragma unroll
     for (int q = 0; q < RING_P / 64; ++q) rr[q] = R[q * 64 + lane];
-    reg_bitonic<RING_P / 64>(rr);
+    reg_bitonic<2>(o);
+    reg_bitonic<RING_P / 64, uint64_t, true>(rr);
+    pv[0] = o[0];
+    pv[1] = o[1];
 #pragma unroll
-    for (int q = 0; q < RING_P / 64; ++q) R[q * 64 + lane] = rr[q];
+    for (int q = 2; q < 8; ++q) pv[q] = TG_KEY_INF;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) pv[8 + q] = rr[q];
   }
-  __syncthreads();
   const long long c1s = pclock(a);
-  merge_lds(O, K, R, RING, P);
+  reg_bitonic_merge<16>(pv);
   __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 16; ++q)
+    if (q * 64 + lane < L) P[q * 64 + lane] = pv[q];
+  __syncthreads();
+  // entries repeating their predecessor's id get id INV
   constexpr int PQ = (128 + RING + TG_WAVE - 1) / TG_WAVE;
   bool dup[PQ];
 #pragma unroll
@@ -771,7 +664,7 @@ __global__ __launch_bounds__(64) void k_rank_reverse(PruneArgs a) {
 // -------------------------------------------------------------------------------------
 template <int MC, class T>
 __global__ __launch_bounds__(64) void k_cal_ep(PruneArgs a, const float* centroid, uint32_t* ep) {
-  __shared__ uint64_t M0[W_MAX], M1[W_MAX], S[128];
+  __shared__ uint64_t M0[128], S[128];
   __shared__ uint32_t tab[HSET];
   __shared__ uint32_t expl[EXPL_MAX];
   __shared__ uint16_t cl[128];
@@ -780,14 +673,11 @@ __global__ __launch_bounds__(64) void k_cal_ep(PruneArgs a, const float* centroi
   Row16<MC> c;
   load_row16<MC>(c, centroid + size_t(seg) * a.D, a.D, threadIdx.x & 15);
   uint32_t rc = 0, ne = 0, na = 0;
-  const uint64_t* B = M0;
-  if (a.TOPM > 64)
-    B = beam_search<MC, T>(a, c, start, 30, M0, M1, S, tab, expl, cl, nullptr, rc, ne, na);
-  else if (a.K <= 64)
-    beam_search_reg<MC, T, 1>(a, c, start, 30, S, tab, expl, cl, nullptr, M0, rc, ne, na);
+  if (a.K <= 64 && a.TOPM <= 64)
+    beam_search<MC, T, 1>(a, c, start, 30, S, tab, expl, cl, nullptr, M0, rc, ne, na);
   else
-    beam_search_reg<MC, T, 2>(a, c, start, 30, S, tab, expl, cl, nullptr, M0, rc, ne, na);
-  if (threadIdx.x == 0) ep[seg] = key_id(B[0]);
+    beam_search<MC, T, 2>(a, c, start, 30, S, tab, expl, cl, nullptr, M0, rc, ne, na);
+  if (threadIdx.x == 0) ep[seg] = key_id(M0[0]);
   count_add(a.counters, 4, ne);
 }
 
@@ -817,10 +707,13 @@ static void launch_prune_mc(const PruneArgs& a, int phase, const float* centroid
   switch (phase) {
     case PH_EP: k_cal_ep<MC, T><<<nseg, 64, 0, s>>>(a, centroid, ep); break;
     case PH_FWD_PATH:
-      k_forward<MC, T, 0><<<grid, block, fwd_layout(a.K, a.TOPM, a.FD).bytes, s>>>(a);
+      if (fwd_layout(a.K, a.TOPM, a.FD).W == 1)
+        k_forward<MC, T, 0, 1><<<grid, block, fwd_layout(a.K, a.TOPM, a.FD).bytes, s>>>(a);
+      else
+        k_forward<MC, T, 0, 2><<<grid, block, fwd_layout(a.K, a.TOPM, a.FD).bytes, s>>>(a);
       break;
     case PH_FWD_2HOP:
-      k_forward<MC, T, 1><<<grid, block, fwd_layout(a.K, a.TOPM, a.FD).bytes, s>>>(a);
+      k_forward<MC, T, 1, 1><<<grid, block, fwd_layout(a.K, a.TOPM, a.FD).bytes, s>>>(a);
       break;
     case PH_REV: k_reverse<MC, T><<<grid, block, 0, s>>>(a); break;
     case PH_CAGRA: k_rank_forward<MC, T, 0><<<grid, block, 0, s>>>(a); break;
