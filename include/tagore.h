@@ -234,6 +234,33 @@ int tg_mlp_eval(int device, uint32_t D, uint32_t B, uint32_t P, const float* X, 
 int tg_mlp_adam(int device, uint32_t D, uint32_t t, float lr, const float* grad, float* params,
                 float* m, float* v);
 
+/* ------------------------------------------------------------------ search validator
+ * Beam search over the last pruned graph of s, restating get_neighbors (common.cu:65-127),
+ * the NSG search the reference carries for its entry point but never runs on its indexes.
+ * Initial pool: the first min(L, deg) neighbours of the entry point, then ids drawn from a
+ * per-query SplitMix64 stream of `seed` (the reference: rand() % N) that skip visited ids;
+ * the pool is kept sorted by (canonical squared distance, id) (the reference: sqrt of an
+ * FP32 sum); the first unchecked entry is expanded until none is left; the first k entries
+ * are returned.  queries [nq*D] (host), ids / dists [nq*k] (host; dists may be NULL). */
+#define TG_INV_ID 0xFFFFFFFFu
+typedef struct {
+  uint32_t L;      /* pool size, 1..256 */
+  uint32_t k;      /* results per query, 1..L */
+  uint64_t seed;   /* random fill of the initial pool */
+  uint32_t ep;     /* entry point; TG_INV_ID: the pruned graph's (node 0 without one) */
+} tg_search_params;
+typedef struct {
+  double ms;            /* device time of the search kernel (HIP events) */
+  uint64_t evals;       /* canonical distance evaluations */
+  uint64_t expansions;  /* expanded pool entries */
+} tg_search_stats;
+int tg_search(tg_state* s, uint32_t nq, const float* queries, const tg_search_params* p,
+              uint32_t* ids, float* dists, tg_search_stats* stats);
+/* recall@k (k <= 16) of result rows ids [nq*k] for host queries [nq*D] against an FP64
+ * brute force over all N vectors of s (a tie at the k-th distance counts as a hit). */
+int tg_search_recall(tg_state* s, uint32_t nq, const float* queries, uint32_t k,
+                     const uint32_t* ids, double* recall);
+
 /* Self-test of the register sorts the prune kernels use (no counterpart in the reference).
  * keys/out: host [groups * 64 * npl] u64, npl 1, 2 or 8.  mode 0: sort each group
  * ascending; mode 1 (npl 1): each group is a bitonic sequence, sort it; mode 2 (npl 2): the

@@ -94,6 +94,10 @@ def lib():
         L.or_collect_path.argtypes = [f32p, ctypes.c_int, u32p, f32p, ctypes.c_int,
                                       ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, u32p,
                                       f32p]
+        L.or_search.restype = ctypes.c_int
+        L.or_search.argtypes = [f32p, ctypes.c_int64, ctypes.c_int, u32p, u32p, ctypes.c_int,
+                                ctypes.c_uint32, f32p, ctypes.c_int64, ctypes.c_int,
+                                ctypes.c_int, ctypes.c_uint64, u32p, f32p, u64p]
         L.or_num_threads.restype = ctypes.c_int
         L.or_set_threads.restype = None
         L.or_set_threads.argtypes = [ctypes.c_int]
@@ -206,6 +210,22 @@ def prune(X, G, GD, FD: int, TOPM: int, kind: int, fwd_rule: int, rev_rule: int,
         adj_rows=int(cnt[3]), ep_evals=int(cnt[4])))
     out.update(st)
     return out
+
+
+def search(X, deg, ids, ep: int, Q, L: int, k: int, seed: int):
+    """Beam search (get_neighbors restatement) of queries Q over the pruned rows
+    (deg, ids).  Returns (ids [nq, k], dists [nq, k], dict(evals, expansions))."""
+    X, Q = _f32(X), _f32(Q)
+    deg, ids = _u32(deg), _u32(ids)
+    nq = Q.shape[0]
+    oi = np.zeros((nq, k), np.uint32)
+    od = np.zeros((nq, k), np.float32)
+    cnt = np.zeros(2, np.uint64)
+    rc = lib().or_search(X, X.shape[0], X.shape[1], deg, ids, ids.shape[1], ep, Q, nq, L, k,
+                         seed & 0xFFFFFFFFFFFFFFFF, oi, od, cnt)
+    if rc != 0:
+        raise ValueError(f"or_search rejected the arguments (code {rc})")
+    return oi, od, dict(evals=int(cnt[0]), expansions=int(cnt[1]))
 
 
 STORE_KIND = {"NSG": 0, "Vamana": 1, "NSSG": 2, "DPG": 3, "CAGRA": 4}

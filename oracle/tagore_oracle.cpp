@@ -1203,6 +1203,90 @@ uint64_t or_collect_path(const float* X, int D, const uint32_t* G, const float* 
 }
 
 // ---------------------------------------------------------------------------------------
+// Beam search over a pruned index (get_neighbors, common.cu:65-127), restated for the
+// variable-degree rows (deg[u], ids[u*FD + j]) of a pruned graph:
+//   * initial ids: the first min(L, deg(ep)) neighbours of ep, then draws
+//     bounded(stream_seed(qs, t), N), t = 0, 1, ... with qs = seed ^ ((q+1) * 0xA0761D6478BD642F)
+//     (the reference: rand() % N), skipping visited ids, until min(L, N) ids or 4096 draws;
+//     every initial id is marked visited;
+//   * the pool: those ids with canonical distances, sorted by (d, id), all unchecked;
+//   * k = 0; repeat: k = first unchecked position >= k (stop if none); mark it checked and
+//     expand its node n: every neighbour not yet visited is marked visited and evaluated;
+//     candidates below the pool's last entry enter it one at a time, the last entry leaving
+//     when the pool holds L (InsertIntoPool, common.cu:25-52, with (d, id) order); nk = the
+//     smallest insertion position (L if none); k = nk <= k ? nk : k + 1.
+//   * output: the first k entries (ids, canonical distances); INV / +inf past the pool.
+// The reference's compare() returns the square root of an FP32 running sum; here the
+// canonical squared distance orders the pool.  counters: [0] evaluations, [1] expansions.
+// ---------------------------------------------------------------------------------------
+int or_search(const float* X, int64_t N, int D, const uint32_t* deg, const uint32_t* ids,
+              int FD, uint32_t ep, const float* Q, int64_t nq, int L, int k, uint64_t seed,
+              uint32_t* out_ids, float* out_d, uint64_t* counters) {
+  uint64_t evals = 0, expansions = 0;
+  std::vector<uint8_t> vis(N);
+  for (int64_t q = 0; q < nq; ++q) {
+    std::fill(vis.begin(), vis.end(), uint8_t(0));
+    const float* xq = Q + q * D;
+    const int64_t target = std::min<int64_t>(L, N);
+    std::vector<uint32_t> init;
+    const int d0 = std::min<int>(int(deg[ep]), L);
+    for (int j = 0; j < d0; ++j) {
+      const uint32_t y = ids[uint64_t(ep) * FD + j];
+      init.push_back(y);
+      vis[y] = 1;
+    }
+    const uint64_t qs = seed ^ ((uint64_t(q) + 1) * 0xA0761D6478BD642FULL);
+    for (uint64_t t = 0; int64_t(init.size()) < target && t < 4096; ++t) {
+      const uint32_t y = bounded(stream_seed(qs, t), uint32_t(N));
+      if (vis[y]) continue;
+      vis[y] = 1;
+      init.push_back(y);
+    }
+    std::vector<Ent> pool;
+    std::vector<uint8_t> checked;
+    for (uint32_t y : init) pool.push_back({cdist(xq, X + uint64_t(y) * D, D), y});
+    evals += init.size();
+    std::sort(pool.begin(), pool.end(), ent_lt);
+    checked.assign(pool.size(), 0);
+    int kk = 0;
+    for (;;) {
+      while (kk < int(pool.size()) && checked[kk]) ++kk;
+      if (kk >= int(pool.size())) break;
+      checked[kk] = 1;
+      const uint32_t n = pool[kk].id;
+      ++expansions;
+      int nk = L;
+      for (uint32_t j = 0; j < deg[n]; ++j) {
+        const uint32_t y = ids[uint64_t(n) * FD + j];
+        if (vis[y]) continue;
+        vis[y] = 1;
+        const Ent e{cdist(xq, X + uint64_t(y) * D, D), y};
+        ++evals;
+        if (int(pool.size()) == L && !ent_lt(e, pool.back())) continue;
+        const int r = int(std::lower_bound(pool.begin(), pool.end(), e, ent_lt) - pool.begin());
+        pool.insert(pool.begin() + r, e);
+        checked.insert(checked.begin() + r, 0);
+        if (int(pool.size()) > L) {
+          pool.pop_back();
+          checked.pop_back();
+        }
+        nk = std::min(nk, r);
+      }
+      kk = nk <= kk ? nk : kk + 1;
+    }
+    for (int j = 0; j < k; ++j) {
+      out_ids[q * k + j] = j < int(pool.size()) ? pool[j].id : INV;
+      out_d[q * k + j] = j < int(pool.size()) ? pool[j].d : FINF;
+    }
+  }
+  if (counters) {
+    counters[0] = evals;
+    counters[1] = expansions;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------------------
 // Learned-init encoder (no counterpart in the reference, whose lists start random:
 // initialize_graph, Tagore_src.cu:158-166).  FP64 restatement of the HIP forward/backward
 // with the same BF16 rounding points when round != 0:

@@ -89,6 +89,16 @@ class LargeStats(ctypes.Structure):
                 ("tile", ctypes.c_uint32), ("flags", ctypes.c_uint32)]
 
 
+class SearchParams(ctypes.Structure):
+    _fields_ = [("L", ctypes.c_uint32), ("k", ctypes.c_uint32), ("seed", ctypes.c_uint64),
+                ("ep", ctypes.c_uint32)]
+
+
+class SearchStats(ctypes.Structure):
+    _fields_ = [("ms", ctypes.c_double), ("evals", ctypes.c_uint64),
+                ("expansions", ctypes.c_uint64)]
+
+
 class TuneResult(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("tile", ctypes.c_uint32),
                 ("ncand", ctypes.c_uint32), ("cand_precision", ctypes.c_int32 * 8),
@@ -148,6 +158,8 @@ def lib():
                               c.c_float, P(c.c_float), vp, vp]
     L.tg_mlp_adam.argtypes = [c.c_int, c.c_uint32, c.c_uint32, c.c_float, vp, vp, vp, vp]
     L.tg_selftest_sort.argtypes = [c.c_int, c.c_uint32, c.c_uint32, c.c_int32, vp, vp]
+    L.tg_search.argtypes = [vp, c.c_uint32, vp, P(SearchParams), vp, vp, P(SearchStats)]
+    L.tg_search_recall.argtypes = [vp, c.c_uint32, vp, c.c_uint32, vp, P(c.c_double)]
     for name in exported_symbols():
         f = getattr(L, name)
         if name not in ("tg_last_error", "tg_host_free"):
@@ -285,6 +297,27 @@ class State:
 
     def store_index(self, path: str, index_type: str, nssg_seed: int = 0x5EED):
         _check(lib().tg_store_index(self._h, path.encode(), index_type.encode(), nssg_seed))
+
+    def search(self, queries, k: int = 10, L: int = 64, seed: int = 0x5EA4C4, ep=None):
+        """Beam search of host queries [nq, D] over the last pruned graph.  Returns
+        (ids [nq, k], dists [nq, k], dict(ms, evals, expansions))."""
+        Q = np.ascontiguousarray(queries, np.float32)
+        nq = Q.shape[0]
+        ids = np.empty((nq, k), np.uint32)
+        d = np.empty((nq, k), np.float32)
+        p = SearchParams(L, k, seed & 0xFFFFFFFFFFFFFFFF, 0xFFFFFFFF if ep is None else ep)
+        st = SearchStats()
+        _check(lib().tg_search(self._h, nq, _ptr(Q), ctypes.byref(p), _ptr(ids), _ptr(d),
+                               ctypes.byref(st)))
+        return ids, d, dict(ms=st.ms, evals=int(st.evals), expansions=int(st.expansions))
+
+    def search_recall(self, queries, ids, k: int = 10):
+        """recall@k of result rows ids [nq, >=k] for host queries (FP64 brute force)."""
+        Q = np.ascontiguousarray(queries, np.float32)
+        I = np.ascontiguousarray(np.asarray(ids)[:, :k], np.uint32)
+        r = ctypes.c_double()
+        _check(lib().tg_search_recall(self._h, Q.shape[0], _ptr(Q), k, _ptr(I), ctypes.byref(r)))
+        return r.value
 
     def recall(self, nq: int, k: int = 10, seed: int = 0x5A11):
         r = ctypes.c_double()

@@ -131,6 +131,10 @@ struct tg_state {
   bool has_prune = false;
   tg_prune_params last_prune{};
   uint32_t ep = 0;
+  // search validator: per-workgroup visited bitmaps (all zero between calls) and id lists
+  uint32_t* sbm = nullptr;
+  uint32_t* svl = nullptr;
+  unsigned long long* scnt = nullptr;
   // segments (clusters stored back to back; nseg == 0: one data set)
   uint32_t nseg = 0;
   std::vector<uint32_t> seg_lo_host;
@@ -436,6 +440,9 @@ int tg_free(tg_state* s) {
   dfree(s->fill);
   dfree(s->scan_tmp);
   dfree(s->counters);
+  dfree(s->sbm);
+  dfree(s->svl);
+  dfree(s->scnt);
   dfree(s->fwd_deg);
   dfree(s->fwd_ids);
   dfree(s->fwd_d);
@@ -1267,6 +1274,138 @@ int tg_store_index(tg_state* s, const char* path, const char* index_type, uint64
 }
 
 // ------------------------------------------------------------------------ recall
+// ------------------------------------------------------------------------ search
+namespace {
+constexpr uint32_t SEARCH_GRID = 256 * 12;  // workgroups: 12 queries in flight per CU
+constexpr uint32_t SEARCH_VCAP = 8192;      // listed visited ids per query
+}  // namespace
+
+int tg_search(tg_state* s, uint32_t nq, const float* queries, const tg_search_params* p,
+              uint32_t* ids, float* dists, tg_search_stats* out) {
+  if (!s || !queries || !p || !ids) return fail(TG_ERR_ARG, "null argument");
+  if (!s->has_prune) return fail(TG_ERR_STATE, "search needs a pruned graph");
+  if (s->nseg) return fail(TG_ERR_STATE, "search runs on a handle without segments");
+  if (nq < 1) return fail(TG_ERR_ARG, "nq must be >= 1");
+  if (p->L < 1 || p->L > 256) return fail(TG_ERR_ARG, "L must be in [1, 256]");
+  if (p->k < 1 || p->k > p->L) return fail(TG_ERR_ARG, "k must be in [1, L]");
+  const uint32_t ep = p->ep == TG_INV_ID ? s->ep : p->ep;
+  if (ep >= s->N) return fail(TG_ERR_ARG, "entry point out of range");
+  HIPX(hipSetDevice(s->device));
+  const uint32_t N = s->N, D = s->D;
+  const uint32_t words = (N + 31) / 32;
+  if (!s->sbm) {
+    if (dalloc(&s->sbm, size_t(SEARCH_GRID) * words) ||
+        dalloc(&s->svl, size_t(SEARCH_GRID) * SEARCH_VCAP) || dalloc(&s->scnt, 2))
+      return TG_ERR_HIP;
+    fill_u32(s->sbm, 0, size_t(SEARCH_GRID) * words, s->stream);
+  }
+  float* Q = nullptr;
+  uint32_t* oi = nullptr;
+  float* od = nullptr;
+  if (dalloc(&Q, size_t(nq) * D) || dalloc(&oi, size_t(nq) * p->k) ||
+      dalloc(&od, size_t(nq) * p->k)) {
+    dfree(Q);
+    dfree(oi);
+    return TG_ERR_HIP;
+  }
+  int rc = TG_OK;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMemcpyAsync(Q, queries, size_t(nq) * D * 4, hipMemcpyHostToDevice, s->stream);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) {
+    fill_u32(reinterpret_cast<uint32_t*>(s->scnt), 0, 4, s->stream);
+    SearchArgs a{};
+    a.X = s->X;
+    a.N = N;
+    a.D = D;
+    a.deg = s->out_deg;
+    a.ids = s->out_ids;
+    a.FD = s->last_prune.FD;
+    a.ep = ep;
+    a.Q = Q;
+    a.nq = nq;
+    a.L = p->L;
+    a.k = p->k;
+    a.seed = p->seed;
+    a.bitmap = s->sbm;
+    a.words = words;
+    a.vlist = s->svl;
+    a.vcap = SEARCH_VCAP;
+    a.out_ids = oi;
+    a.out_d = od;
+    a.counters = s->scnt;
+    e = hipEventRecord(e0, s->stream);
+    launch_search(a, std::min(nq, SEARCH_GRID), s->stream);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(e1, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  }
+  unsigned long long cnt[2] = {0, 0};
+  if (e == hipSuccess)
+    e = hipMemcpy(ids, oi, size_t(nq) * p->k * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && dists)
+    e = hipMemcpy(dists, od, size_t(nq) * p->k * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(cnt, s->scnt, sizeof(cnt), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && out) {
+    float ms = 0.f;
+    e = hipEventElapsedTime(&ms, e0, e1);
+    out->ms = ms;
+    out->evals = cnt[0];
+    out->expansions = cnt[1];
+  }
+  if (e != hipSuccess) rc = fail(TG_ERR_HIP, std::string("search: ") + hipGetErrorString(e));
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  dfree(Q);
+  dfree(oi);
+  dfree(od);
+  return rc;
+}
+
+int tg_search_recall(tg_state* s, uint32_t nq, const float* queries, uint32_t k,
+                     const uint32_t* ids, double* recall) {
+  if (!s || !queries || !ids || !recall) return fail(TG_ERR_ARG, "null argument");
+  if (k < 1 || k > 16) return fail(TG_ERR_ARG, "k must be in [1, 16]");
+  if (nq < 1) return fail(TG_ERR_ARG, "nq must be >= 1");
+  HIPX(hipSetDevice(s->device));
+  const uint32_t cap = 256;
+  float* Q = nullptr;
+  uint32_t *G = nullptr, *cand_n = nullptr, *hits = nullptr;
+  double *kth = nullptr, *cand_d = nullptr;
+  int rc = TG_OK;
+  if (dalloc(&Q, size_t(nq) * s->D) || dalloc(&G, size_t(nq) * k) || dalloc(&cand_n, nq) ||
+      dalloc(&hits, nq) || dalloc(&kth, nq) ||
+      dalloc(&cand_d, size_t(nq) * cap + size_t(nq) * k)) {
+    rc = TG_ERR_HIP;
+  } else {
+    hipError_t e = hipMemcpy(Q, queries, size_t(nq) * s->D * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(G, ids, size_t(nq) * k * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      launch_recall(s->X, s->N, s->D, G, k, nullptr, nq, k, kth, cand_n, cand_d, cap, hits,
+                    s->stream, Q);
+      e = hipStreamSynchronize(s->stream);
+    }
+    if (e == hipSuccess) e = hipGetLastError();
+    std::vector<uint32_t> h(nq);
+    if (e == hipSuccess) e = hipMemcpy(h.data(), hits, size_t(nq) * 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+      rc = fail(TG_ERR_HIP, std::string("search recall: ") + hipGetErrorString(e));
+    } else {
+      double tot = 0.0;
+      for (uint32_t i = 0; i < nq; ++i) tot += double(std::min(h[i], k)) / k;
+      *recall = tot / nq;
+    }
+  }
+  dfree(Q);
+  dfree(G);
+  dfree(cand_n);
+  dfree(hits);
+  dfree(kth);
+  dfree(cand_d);
+  return rc;
+}
+
 int tg_recall(tg_state* s, uint32_t nq, uint64_t seed, uint32_t k, double* recall) {
   if (s && s->nseg) return fail(TG_ERR_STATE, "recall is judged on a handle without segments");
   if (!s || !recall) return fail(TG_ERR_ARG, "null argument");

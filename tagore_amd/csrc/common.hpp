@@ -396,6 +396,38 @@ __device__ __forceinline__ void reg_bitonic_merge(T (&v)[NPL]) {
     }
   }
 }
+// reg_bitonic_merge with a 32-bit payload moved along with each key
+template <int NPL>
+__device__ __forceinline__ void reg_bitonic_merge_kv(uint64_t (&v)[NPL], uint32_t (&f)[NPL]) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int j = 32 * NPL; j > 0; j >>= 1) {
+    if (j >= 64) {
+      const int rj = j >> 6;
+#pragma unroll
+      for (int r = 0; r < NPL; ++r) {
+        if (r & rj) continue;
+        const bool sw = v[r | rj] < v[r];
+        const uint64_t x = v[r], y = v[r | rj];
+        const uint32_t fx = f[r], fy = f[r | rj];
+        v[r] = sw ? y : x;
+        v[r | rj] = sw ? x : y;
+        f[r] = sw ? fy : fx;
+        f[r | rj] = sw ? fx : fy;
+      }
+    } else {
+      const bool lower = (lane & j) == 0;
+#pragma unroll
+      for (int r = 0; r < NPL; ++r) {
+        const uint64_t x = v[r], p = lane_xor64(x, j);
+        const uint32_t pf = lane_xor32(f[r], j);
+        const bool take = lower ? (p < x) : (x < p);  // equal keys keep their own payload
+        v[r] = take ? p : x;
+        f[r] = take ? pf : f[r];
+      }
+    }
+  }
+}
 __device__ __forceinline__ void reg_bitonic_merge64(uint64_t& v) {
   uint64_t w[1] = {v};
   reg_bitonic_merge<1>(w);

@@ -144,6 +144,27 @@ void pb_group_overflow(const PushBuf& pb, uint32_t N, uint32_t* fill, uint32_t* 
 size_t scan_tmp_words(uint32_t N);
 
 void fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+// Beam search over a pruned graph (search.hip): queries Q [nq*D], pool L <= 256, k <= L;
+// bitmap: `words` u32 per workgroup, all zero on entry and exit; vlist: vcap u32 per
+// workgroup.  counters: [0] distance evaluations, [1] expansions.
+struct SearchArgs {
+  const float* X;
+  uint32_t N, D;
+  const uint32_t* deg;
+  const uint32_t* ids;
+  uint32_t FD, ep;
+  const float* Q;
+  uint32_t nq, L, k;
+  uint64_t seed;
+  uint32_t* bitmap;
+  uint32_t words;
+  uint32_t* vlist;
+  uint32_t vcap;
+  uint32_t* out_ids;
+  float* out_d;
+  unsigned long long* counters;
+};
+void launch_search(const SearchArgs& a, uint32_t grid, hipStream_t s);
 void launch_selftest_sort(const uint64_t* in, uint64_t* out, int npl, int mode, uint32_t groups,
                           hipStream_t s);
 void fill_u8(uint8_t* p, uint8_t v, size_t n, hipStream_t s);
@@ -155,6 +176,7 @@ void launch_gen_data(int kind, uint32_t N, uint32_t D, uint32_t ncomp, uint64_t 
 // FP64 recall judge
 void launch_recall(const float* X, uint32_t N, uint32_t D, const uint32_t* G, uint32_t K,
                    const uint32_t* q, uint32_t nq, uint32_t k, double* kth, uint32_t* cand_n,
-                   double* cand_d, uint32_t cand_cap, uint32_t* hits, hipStream_t s);
+                   double* cand_d, uint32_t cand_cap, uint32_t* hits, hipStream_t s,
+                   const float* Q = nullptr);
 
 }  // namespace tg

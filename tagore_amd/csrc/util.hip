@@ -225,20 +225,29 @@ __device__ double d64_rows(const float* a, const float* b, uint32_t D) {
   return s;
 }
 
+// Queries are base rows q[i] (then the row itself is excluded and the graph row G[q[i]]
+// is judged) or, with Q, external rows Q[i] (nothing excluded, G row i is judged).
+__device__ __forceinline__ const float* judge_row(const float* X, const float* Q,
+                                                  const uint32_t* q, uint32_t qi, uint32_t D) {
+  return Q ? Q + size_t(qi) * D : X + size_t(q[qi]) * D;
+}
+
 // pass 1: the graph's first k valid entries bound the k-th exact distance from above
 __global__ void k_recall_bound(const float* X, uint32_t D, const uint32_t* G, uint32_t K,
                                const uint32_t* q, uint32_t nq, uint32_t k, double* bound,
-                               double* gd64, uint32_t N) {
+                               double* gd64, uint32_t N, const float* Q) {
   const uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
   if (qi >= nq) return;
-  const uint32_t u = q[qi];
+  const uint32_t u = Q ? 0xFFFFFFFFu : q[qi];
+  const uint32_t grow = Q ? qi : u;
+  const float* xq = judge_row(X, Q, q, qi, D);
   double b = 0.0;
   uint32_t nvalid = 0;
   for (uint32_t j = 0; j < k; ++j) {
-    const uint32_t v = G[size_t(u) * K + j];
+    const uint32_t v = G[size_t(grow) * K + j];
     double d = __longlong_as_double(0x7FF0000000000000ll);
     if (v < N && v != u) {
-      d = d64_rows(X + size_t(u) * D, X + size_t(v) * D, D);
+      d = d64_rows(xq, X + size_t(v) * D, D);
       ++nvalid;
       b = fmax(b, d);
     }
@@ -252,19 +261,20 @@ template <int QT>
 __global__ __launch_bounds__(256) void k_recall_collect(const float* X, uint32_t N, uint32_t D,
                                                         const uint32_t* q, uint32_t nq,
                                                         const double* bound, uint32_t* cand_n,
-                                                        double* cand_d, uint32_t cap) {
+                                                        double* cand_d, uint32_t cap,
+                                                        const float* Q) {
   extern __shared__ float qrows[];  // QT * D
   const uint32_t q0 = blockIdx.y * QT;
   for (uint32_t e = threadIdx.x; e < QT * D; e += blockDim.x) {
     const uint32_t qq = e / D;
-    qrows[e] = (q0 + qq < nq) ? X[size_t(q[q0 + qq]) * D + (e % D)] : 0.f;
+    qrows[e] = (q0 + qq < nq) ? judge_row(X, Q, q, q0 + qq, D)[e % D] : 0.f;
   }
   __syncthreads();
   double bnd[QT];
   uint32_t qid[QT];
   for (int qq = 0; qq < QT; ++qq) {
     bnd[qq] = (q0 + qq < nq) ? bound[q0 + qq] : -1.0;
-    qid[qq] = (q0 + qq < nq) ? q[q0 + qq] : 0xFFFFFFFFu;
+    qid[qq] = (q0 + qq < nq && !Q) ? q[q0 + qq] : 0xFFFFFFFFu;
   }
   for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
     double s[QT];
@@ -318,16 +328,17 @@ __global__ void k_recall_final(uint32_t nq, uint32_t k, const uint32_t* cand_n, 
 __global__ __launch_bounds__(256) void k_recall_full(const float* X, uint32_t N, uint32_t D,
                                                      const uint32_t* q, uint32_t nq, uint32_t k,
                                                      const double* gd64, double* kth,
-                                                     uint32_t* hits) {
+                                                     uint32_t* hits, const float* Q) {
   const uint32_t qi = blockIdx.x;
   if (qi >= nq || hits[qi] != 0xFFFFFFFFu) return;
   __shared__ double best[256 * 16];
   double loc[16];
   for (int j = 0; j < 16; ++j) loc[j] = __longlong_as_double(0x7FF0000000000000ll);
-  const uint32_t u = q[qi];
+  const uint32_t u = Q ? 0xFFFFFFFFu : q[qi];
+  const float* xq = judge_row(X, Q, q, qi, D);
   for (uint32_t p = threadIdx.x; p < N; p += blockDim.x) {
     if (p == u) continue;
-    double d = d64_rows(X + size_t(u) * D, X + size_t(p) * D, D);
+    double d = d64_rows(xq, X + size_t(p) * D, D);
     if (d < loc[k - 1]) {
       int j = int(k) - 1;
       while (j > 0 && loc[j - 1] > d) {
@@ -364,26 +375,27 @@ __global__ __launch_bounds__(256) void k_recall_full(const float* X, uint32_t N,
 
 void launch_recall(const float* X, uint32_t N, uint32_t D, const uint32_t* G, uint32_t K,
                    const uint32_t* q, uint32_t nq, uint32_t k, double* kth, uint32_t* cand_n,
-                   double* cand_d, uint32_t cand_cap, uint32_t* hits, hipStream_t s) {
+                   double* cand_d, uint32_t cand_cap, uint32_t* hits, hipStream_t s,
+                   const float* Q) {
   // kth doubles as the bound buffer in pass 1/2; gd64 lives after cand_d
   double* bound = kth;
   double* gd64 = cand_d + size_t(nq) * cand_cap;
-  k_recall_bound<<<(nq + 255) / 256, 256, 0, s>>>(X, D, G, K, q, nq, k, bound, gd64, N);
+  k_recall_bound<<<(nq + 255) / 256, 256, 0, s>>>(X, D, G, K, q, nq, k, bound, gd64, N, Q);
   fill_u32(cand_n, 0, nq, s);
   if (D <= 256) {
     constexpr int QT = 16;
     dim3 grid(64, (nq + QT - 1) / QT);
     k_recall_collect<QT><<<grid, 256, QT * D * sizeof(float), s>>>(X, N, D, q, nq, bound, cand_n,
-                                                                  cand_d, cand_cap);
+                                                                  cand_d, cand_cap, Q);
   } else {
     constexpr int QT = 4;
     dim3 grid(64, (nq + QT - 1) / QT);
     k_recall_collect<QT><<<grid, 256, QT * D * sizeof(float), s>>>(X, N, D, q, nq, bound, cand_n,
-                                                                  cand_d, cand_cap);
+                                                                  cand_d, cand_cap, Q);
   }
   k_recall_final<<<(nq + 255) / 256, 256, 0, s>>>(nq, k, cand_n, cand_d, cand_cap, gd64, kth,
                                                   hits);
-  k_recall_full<<<nq, 256, 0, s>>>(X, N, D, q, nq, k, gd64, kth, hits);
+  k_recall_full<<<nq, 256, 0, s>>>(X, N, D, q, nq, k, gd64, kth, hits, Q);
 }
 
 // ------------------------------------------------------- register sort self-test

@@ -120,6 +120,24 @@ def test_prune_parity_bf16_rows(int_case, name, kind, fr, rr, th, prec):
         _compare(Y, GY, GDY, kind, fr, rr, 32, 64, th, prec)
 
 
+@pytest.mark.parametrize("D,lo,hi", [(128, 0, 255), (96, 0, 255), (200, -60, 60), (64, -256, 256)])
+def test_prune_parity_integer_bf16_rows(D, lo, hi):
+    """Integer data under precision "bf16" (the rows are read from the BF16 copy, which holds
+    these values exactly): unsigned and signed values, D not a multiple of 32 or 64, values
+    at +-256.  Every rule, and a beam of two register slots (TOPM > 64), must equal the
+    oracle bit for bit."""
+    rng = np.random.default_rng(D + hi)
+    C = rng.integers(lo, hi + 1, (24, D))
+    X = np.clip(C[rng.integers(0, 24, 2500)] + rng.integers(-12, 13, (2500, D)), lo, hi)
+    X = X.astype(np.float32)
+    G, GD = oracle.knn_bruteforce(X, 48)
+    for name, kind, fr, rr, th in [("NSG", 0, 0, 0, 1.0), ("Vamana", 0, 0, 0, 1.2),
+                                   ("CFS path/angle", 0, 0, 1, 0.5), ("NSSG", 1, 1, 1, 0.5),
+                                   ("CFS 2-hop/dist", 1, 1, 0, 1.0)]:
+        _compare(X, G, GD, kind, fr, rr, 32, 64, th, "bf16")
+    _compare(X, G, GD, 0, 0, 0, 24, 100, 1.0, "bf16")  # two beam slots (TOPM > 64)
+
+
 @pytest.mark.parametrize("prec", ["fp32", "bf16", "fp8"])
 def test_dpg_parity_d960(prec):
     """DPG and CAGRA (1-hop/rank) at dimension 960, the C5 shape, in every precision."""
