@@ -77,6 +77,24 @@ def measure(st, args, precision, iters, init, label):
                 prune_counters=last["prune_counters"])
 
 
+def reachable_fraction(st):
+    """Fraction of the vectors reachable from the pruned graph's entry point (host BFS)."""
+    import numpy as np
+
+    deg, ids, ep = st.get_pruned()
+    N = deg.shape[0]
+    cols = np.arange(ids.shape[1])[None, :]
+    seen = np.zeros(N, bool)
+    seen[ep] = True
+    front = np.array([ep], np.int64)
+    while front.size:
+        nb = ids[front][cols < deg[front][:, None]].astype(np.int64)
+        nb = np.unique(nb[~seen[nb]])
+        seen[nb] = True
+        front = nb
+    return float(seen.mean())
+
+
 def measured_traffic(N, D, K, FD, TOPM):
     """HBM bytes per k_forward launch from the committed rocprofv3 counter runs of the same
     workload (FETCH_SIZE and WRITE_SIZE, each scaled by its copy-kernel calibration), or
@@ -283,6 +301,8 @@ def main():
     ap.add_argument("--prune-rows", default="fp32", choices=["fp32", "bf16"],
                     help="rows the headline prune reads (bf16 only when the data is exact in BF16)")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--search-queries", type=int, default=10000,
+                    help="held-out queries for the beam-search check of the NSG index (0: off)")
     ap.add_argument("--phase-clocks", action="store_true",
                     help="also report the prune forward kernel's per-phase shader cycles")
     ap.add_argument("--seed", type=int, default=0x7A600002)
@@ -346,6 +366,27 @@ def main():
         log(f"phase cycles per node: {head['phase_cycles_per_node']}")
         st.set_profiling(True)
 
+    # index quality: beam search (get_neighbors) over the headline's NSG graph with held-out
+    # queries, the next vectors N.. of the same generator
+    search = None
+    if args.search_queries > 0:
+        nq = args.search_queries
+        with tagore_amd.State(N + nq, D, 0) as gq:
+            gq.gen_data(0, args.ncomp, args.seed, 20.0)
+            Q = gq.get_data(N, nq)
+        search = {"reachable_from_entry": reachable_fraction(st), "sweep": []}
+        log(f"search: {search['reachable_from_entry']:.4f} of the vectors reachable from the "
+            "entry point")
+        for L in (10, 16, 32, 64, 128):
+            st.search(Q[:min(nq, 1000)], k=10, L=L)  # warm-up
+            ids, _, ss = st.search(Q, k=10, L=L)
+            rec = st.search_recall(Q, ids, 10)
+            row = {"L": L, "qps": nq / (ss["ms"] / 1e3), "ms": ss["ms"], "recall_at_10": rec,
+                   "evals_per_query": ss["evals"] / nq,
+                   "expansions_per_query": ss["expansions"] / nq}
+            search["sweep"].append(row)
+            log(f"search L={L}: {row['qps']:.0f} queries/s, recall@10 {rec:.4f}")
+
     # roofline of the pruning forward kernel (Collect + Filter): algorithmic bytes = every
     # canonical distance evaluation reads one FP32 row, plus adjacency rows, own lists, output
     pc = head["prune_counters"]
@@ -396,6 +437,7 @@ def main():
         "speedup_vs_plain": (plain["ms_per_step"] / head["ms_per_step"]) if plain else None,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "search_nsg": search,
     }
     print(json.dumps(out), flush=True)
 
