@@ -23,6 +23,13 @@ constexpr int RING = 480;
 __device__ __forceinline__ long long pclock(const PruneArgs& a) {
   return a.phase_clock ? clock64() : 0;
 }
+
+// phase-clock sums of one workgroup (profiling): distance batches, and the beam rounds'
+// candidate step, sort, merge and id-set rebuild.  Held in registers (no address taken).
+struct PhaseAcc {
+  long long ev = 0;
+  long long sub[4] = {0, 0, 0, 0};
+};
 constexpr int POOL_MAX = 1024;  // K + 480 <= 608, padded to a power of two
 constexpr int EXPL_MAX = 32;
 
@@ -67,11 +74,11 @@ __device__ __forceinline__ bool hset_has(const uint32_t* tab, uint32_t id) {
 // key only decreases).  `out` (optional) receives the final beam (TOPM keys).
 // -------------------------------------------------------------------------------------
 template <int MC, class T, int W>
-__device__ void beam_search(const PruneArgs& a, const Row16<MC>& tgt, uint32_t start,
+__device__ __forceinline__ void beam_search(const PruneArgs& a, const Row16<MC>& tgt,
+                                            uint32_t start,
                             int max_exp, uint64_t* S, uint32_t* tab, uint32_t* expl,
                             uint16_t* cl, uint64_t* ring, uint64_t* out, uint32_t& ring_c,
-                            uint32_t& n_eval, uint32_t& n_adj, long long* ev = nullptr,
-                            long long* sub = nullptr) {
+                            uint32_t& n_eval, uint32_t& n_adj, PhaseAcc& ph) {
   const int lane = threadIdx.x;
   const int K = a.K, TOPM = a.TOPM, D = a.D;
   constexpr int NS = 64 * W;
@@ -104,7 +111,7 @@ __device__ void beam_search(const PruneArgs& a, const Row16<MC>& tgt, uint32_t s
   __syncthreads();
   int nexpl = 0;
   for (int r = 0; r < max_exp; ++r) {
-    const long long s0 = sub ? pclock(a) : 0;
+    const long long s0 = pclock(a);
     // first explorable beam entry (position order)
     int first = -1;
     uint64_t fk = 0;
@@ -148,8 +155,8 @@ __device__ void beam_search(const PruneArgs& a, const Row16<MC>& tgt, uint32_t s
     }
     __syncthreads();
     const uint32_t c0 = ring_c;
-    const long long e0 = ev ? pclock(a) : 0;
-    if (sub) sub[0] += e0 - s0;
+    const long long e0 = pclock(a);
+    ph.sub[0] += e0 - s0;
     eval_batch<MC>(X, D, tgt, cnt, [&](int q) { return key_id(S[cl[q]]); },
                    [&](int q, float d) {
                      const int j = cl[q];
@@ -157,11 +164,11 @@ __device__ void beam_search(const PruneArgs& a, const Row16<MC>& tgt, uint32_t s
                      S[j] = k;
                      if (ring) ring[(c0 + q) % RING] = k;
                    });
-    if (ev) *ev += pclock(a) - e0;
+    ph.ev += pclock(a) - e0;
     ring_c += cnt;
     n_eval += cnt;
     __syncthreads();
-    const long long s1 = sub ? pclock(a) : 0;
+    const long long s1 = pclock(a);
 #pragma unroll
     for (int q = 0; q < W; ++q) v[q] = S[q * 64 + lane];
     // next dedup set: ids of this round's beam and candidates
@@ -172,9 +179,9 @@ __device__ void beam_search(const PruneArgs& a, const Row16<MC>& tgt, uint32_t s
       hset_insert(tab, key_id(bk[q]));
       hset_insert(tab, key_id(v[q]));
     }
-    const long long s2 = sub ? pclock(a) : 0;
+    const long long s2 = pclock(a);
     reg_bitonic<W>(v);
-    const long long s3 = sub ? pclock(a) : 0;
+    const long long s3 = pclock(a);
     uint64_t c[W];
 #pragma unroll
     for (int b = 0; b < W; ++b) {
@@ -185,12 +192,10 @@ __device__ void beam_search(const PruneArgs& a, const Row16<MC>& tgt, uint32_t s
 #pragma unroll
     for (int b = 0; b < W; ++b) bk[b] = b * 64 + lane < TOPM ? c[b] : TG_KEY_INF;
     __syncthreads();
-    if (sub) {
-      const long long s4 = pclock(a);
-      sub[1] += s3 - s2;
-      sub[2] += s4 - s3;
-      sub[3] += s2 - s1;
-    }
+    const long long s4 = pclock(a);
+    ph.sub[1] += s3 - s2;
+    ph.sub[2] += s4 - s3;
+    ph.sub[3] += s2 - s1;
   }
   if (out) {
 #pragma unroll
@@ -267,9 +272,9 @@ __device__ __forceinline__ int compact_cl(uint16_t* cl, int from, int n, Keep ke
 // round touches only the survivors of the previous round.
 // -------------------------------------------------------------------------------------
 template <int MC, class T>
-__device__ int filter_pool(const PruneArgs& a, uint64_t* P, int L, uint32_t u, int rule,
-                           uint64_t* sel, uint16_t* cl, uint32_t& n_eval,
-                           long long* ev = nullptr) {
+__device__ __forceinline__ int filter_pool(const PruneArgs& a, uint64_t* P, int L, uint32_t u,
+                                           int rule, uint64_t* sel, uint16_t* cl,
+                                           uint32_t& n_eval, PhaseAcc& ph) {
   const int lane = threadIdx.x, t = lane & 15;
   const int D = a.D, FD = a.FD;
   int tix = next_valid(P, 0, L, u);
@@ -298,7 +303,7 @@ __device__ int filter_pool(const PruneArgs& a, uint64_t* P, int L, uint32_t u, i
       if (id == pid) P[i] = key_with_id(P[i], TG_INV);
       return id != TG_INV && id != pid;
     });
-    const long long e0 = ev ? pclock(a) : 0;
+    const long long e0 = pclock(a);
     eval_batch<MC>(reinterpret_cast<const T*>(a.rows), D, prow, cnt, [&](int q) { return key_id(P[cl[q]]); },
                    [&](int q, float res) {
                      const int i = cl[q];
@@ -306,7 +311,7 @@ __device__ int filter_pool(const PruneArgs& a, uint64_t* P, int L, uint32_t u, i
                      if (rule_eval(rule, res, key_d(k), cur, a.th) < 0.0f)
                        P[i] = key_with_id(k, TG_INV);
                    });
-    if (ev) *ev += pclock(a) - e0;
+    ph.ev += pclock(a) - e0;
     n_eval += cnt;
     __syncthreads();
     // survivors in pool order; the first is the next selection
@@ -385,15 +390,13 @@ __global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
     R[i] = i < RING ? mk_key(__uint_as_float(0x7F800000u), TG_INV) : TG_KEY_INF;
   __syncthreads();
   uint32_t n_col = 0, n_adj = 0, n_flt = 0;
-  long long t_ev = 0, t_sub[4] = {0, 0, 0, 0};
-  long long* ev = a.phase_clock ? &t_ev : nullptr;
-  long long* sub = a.phase_clock ? t_sub : nullptr;
+  PhaseAcc ph;
   const long long c0 = pclock(a);
   if (KIND == 0) {
     uint32_t ring_c = 0;
     const uint32_t ep = a.ep[a.seg_of ? a.seg_of[u] : 0];
     beam_search<MC, T, W>(a, xu, ep, 10, S, tab, expl, clb, R, nullptr, ring_c, n_col, n_adj,
-                          ev, sub);
+                          ph);
   } else {
     // two-hop ids y_i = G[G[u][i/K]][i%K], i < 480
     for (int i = lane; i < RING; i += TG_WAVE) {
@@ -450,15 +453,15 @@ __global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
     if (dup[q]) P[q * TG_WAVE + lane] = key_with_id(P[q * TG_WAVE + lane], TG_INV);
   __syncthreads();
   const long long c2 = pclock(a);
-  const int deg = filter_pool<MC, T>(a, P, L, u, a.fwd_rule, sel, cl, n_flt, ev);
+  const int deg = filter_pool<MC, T>(a, P, L, u, a.fwd_rule, sel, cl, n_flt, ph);
   __syncthreads();
   if (a.phase_clock && lane == 0 && a.counters) {
     const long long c3 = clock64();
     atomicAdd(&a.counters[6], (unsigned long long)(c1 - c0));
     atomicAdd(&a.counters[7], (unsigned long long)(c3 - c2));
     atomicAdd(&a.counters[8], (unsigned long long)(c2 - c1));
-    atomicAdd(&a.counters[9], (unsigned long long)t_ev);
-    for (int i = 0; i < 4; ++i) atomicAdd(&a.counters[10 + i], (unsigned long long)t_sub[i]);
+    atomicAdd(&a.counters[9], (unsigned long long)ph.ev);
+    for (int i = 0; i < 4; ++i) atomicAdd(&a.counters[10 + i], (unsigned long long)ph.sub[i]);
     atomicAdd(&a.counters[14], (unsigned long long)(c1s - c1));
   }
   for (int k = lane; k < deg; k += TG_WAVE) {
@@ -502,7 +505,8 @@ __global__ __launch_bounds__(64) void k_reverse(PruneArgs a) {
     }
     __syncthreads();
     sort_dedup(P, n);
-    const int nd = filter_pool<MC, T>(a, P, L, u, a.rev_rule, sel, cl, n_flt);
+    PhaseAcc ph;  // not reported for the reverse pass
+    const int nd = filter_pool<MC, T>(a, P, L, u, a.rev_rule, sel, cl, n_flt, ph);
     __syncthreads();
     for (int k = lane; k < FD; k += TG_WAVE) row[k] = k < nd ? key_id(sel[k]) : TG_INV;
     if (lane == 0) a.out_deg[u] = nd;
@@ -673,10 +677,11 @@ __global__ __launch_bounds__(64) void k_cal_ep(PruneArgs a, const float* centroi
   Row16<MC> c;
   load_row16<MC>(c, centroid + size_t(seg) * a.D, a.D, threadIdx.x & 15);
   uint32_t rc = 0, ne = 0, na = 0;
+  PhaseAcc ph;  // not reported for the entry point
   if (a.K <= 64 && a.TOPM <= 64)
-    beam_search<MC, T, 1>(a, c, start, 30, S, tab, expl, cl, nullptr, M0, rc, ne, na);
+    beam_search<MC, T, 1>(a, c, start, 30, S, tab, expl, cl, nullptr, M0, rc, ne, na, ph);
   else
-    beam_search<MC, T, 2>(a, c, start, 30, S, tab, expl, cl, nullptr, M0, rc, ne, na);
+    beam_search<MC, T, 2>(a, c, start, 30, S, tab, expl, cl, nullptr, M0, rc, ne, na, ph);
   if (threadIdx.x == 0) ep[seg] = key_id(M0[0]);
   count_add(a.counters, 4, ne);
 }
