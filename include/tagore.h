@@ -102,6 +102,22 @@ int tg_get_data(tg_state* s, float* host_rows, uint32_t first, uint32_t count);
 
 /* .fvecs reader (common.cu:6-22): *rows is malloc'ed, free with tg_host_free. */
 int tg_read_fvecs(const char* path, float** rows, uint32_t* N, uint32_t* D);
+/* .bvecs (u32 dim + dim bytes per vector) and .fbin (u32 N, u32 D, N*D f32) readers, the
+ * other vector formats of the ANN benchmarks the reference's loader family reads
+ * (common.cu:6-22); rows as f32, malloc'ed like tg_read_fvecs. */
+int tg_read_bvecs(const char* path, float** rows, uint32_t* N, uint32_t* D);
+int tg_read_fbin(const char* path, float** rows, uint32_t* N, uint32_t* D);
+/* Reader of the index layouts StoreNSG / StoreVamana / StoreNSSG / StoreDPG / StoreCAGRA
+ * write (common.cu:151-248; tg_store_index writes the same bytes).  N: the vectors the
+ * index covers.  *width: row capacity (NSG / NSSG header, Vamana max degree, DPG largest
+ * degree, CAGRA d; at most 128); *deg [N] and *ids [N * width] (INV-padded) are malloc'ed
+ * (tg_host_free); eps[10] receives the entry points the file names, *n_eps their number. */
+int tg_read_index(const char* path, const char* index_type, uint32_t N, uint32_t* width,
+                  uint32_t** deg, uint32_t** ids, uint32_t* eps, uint32_t* n_eps);
+/* Installs a pruned graph (e.g. one read with tg_read_index) as the handle's current one,
+ * for tg_search / tg_store_index. */
+int tg_set_pruned(tg_state* s, uint32_t width, const uint32_t* deg, const uint32_t* ids,
+                  uint32_t ep);
 void tg_host_free(void* p);
 
 /* kNN graph.  tg_knn_build runs GNN-Descent (GNN_descent, pybind.cu:19-142); tg_set_knn

@@ -136,6 +136,11 @@ def lib():
     L.tg_gen_data.argtypes = [vp, c.c_int32, c.c_uint32, c.c_uint64, c.c_float]
     L.tg_get_data.argtypes = [vp, vp, c.c_uint32, c.c_uint32]
     L.tg_read_fvecs.argtypes = [c.c_char_p, P(vp), P(c.c_uint32), P(c.c_uint32)]
+    L.tg_read_bvecs.argtypes = [c.c_char_p, P(vp), P(c.c_uint32), P(c.c_uint32)]
+    L.tg_read_fbin.argtypes = [c.c_char_p, P(vp), P(c.c_uint32), P(c.c_uint32)]
+    L.tg_read_index.argtypes = [c.c_char_p, c.c_char_p, c.c_uint32, P(c.c_uint32), P(vp),
+                                P(vp), vp, P(c.c_uint32)]
+    L.tg_set_pruned.argtypes = [vp, c.c_uint32, vp, vp, c.c_uint32]
     L.tg_host_free.argtypes = [vp]
     L.tg_host_free.restype = None
     L.tg_knn_build.argtypes = [vp, P(KnnParams)]
@@ -298,6 +303,19 @@ class State:
     def store_index(self, path: str, index_type: str, nssg_seed: int = 0x5EED):
         _check(lib().tg_store_index(self._h, path.encode(), index_type.encode(), nssg_seed))
 
+    def set_pruned(self, deg, ids, ep: int = 0):
+        """Installs a pruned graph (deg [N], ids [N, width]) as the current one."""
+        deg = np.ascontiguousarray(deg, np.uint32)
+        ids = np.ascontiguousarray(ids, np.uint32)
+        _check(lib().tg_set_pruned(self._h, ids.shape[1], _ptr(deg), _ptr(ids), ep))
+        self.FD = ids.shape[1]
+
+    def load_index(self, path: str, index_type: str):
+        """Reads an index file (tg_read_index) and installs it; returns its entry points."""
+        deg, ids, eps = read_index(path, index_type, self.N)
+        self.set_pruned(deg, ids, int(eps[0]) if eps.size else 0)
+        return eps
+
     def search(self, queries, k: int = 10, L: int = 64, seed: int = 0x5EA4C4, ep=None):
         """Beam search of host queries [nq, D] over the last pruned graph.  Returns
         (ids [nq, k], dists [nq, k], dict(ms, evals, expansions))."""
@@ -414,6 +432,51 @@ def selftest_sort(keys, npl: int, mode: int = 0, device: int = 0):
     out = np.empty_like(k)
     _check(lib().tg_selftest_sort(device, npl, k.shape[0], mode, _ptr(k), _ptr(out)))
     return out
+
+
+def _read_rows(fn, path: str):
+    rows = ctypes.c_void_p()
+    n, d = ctypes.c_uint32(), ctypes.c_uint32()
+    _check(fn(path.encode(), ctypes.byref(rows), ctypes.byref(n), ctypes.byref(d)))
+    try:
+        buf = (ctypes.c_float * (n.value * d.value)).from_address(rows.value)
+        return np.array(buf, np.float32).reshape(n.value, d.value)
+    finally:
+        lib().tg_host_free(rows)
+
+
+def read_fvecs(path: str):
+    """.fvecs vectors as float32 [N, D]."""
+    return _read_rows(lib().tg_read_fvecs, path)
+
+
+def read_bvecs(path: str):
+    """.bvecs (u8) vectors as float32 [N, D]."""
+    return _read_rows(lib().tg_read_bvecs, path)
+
+
+def read_fbin(path: str):
+    """.fbin (u32 N, u32 D, f32 rows) vectors as float32 [N, D]."""
+    return _read_rows(lib().tg_read_fbin, path)
+
+
+def read_index(path: str, index_type: str, N: int):
+    """An index file in one of the five layouts (NSG, Vamana, NSSG, DPG, CAGRA).  Returns
+    (deg [N], ids [N, width] INV-padded, entry points)."""
+    w = ctypes.c_uint32()
+    dp, ip = ctypes.c_void_p(), ctypes.c_void_p()
+    eps = np.zeros(10, np.uint32)
+    ne = ctypes.c_uint32()
+    _check(lib().tg_read_index(path.encode(), index_type.encode(), N, ctypes.byref(w),
+                               ctypes.byref(dp), ctypes.byref(ip), _ptr(eps), ctypes.byref(ne)))
+    try:
+        deg = np.array((ctypes.c_uint32 * N).from_address(dp.value), np.uint32)
+        ids = np.array((ctypes.c_uint32 * (N * w.value)).from_address(ip.value),
+                       np.uint32).reshape(N, w.value)
+    finally:
+        lib().tg_host_free(dp)
+        lib().tg_host_free(ip)
+    return deg, ids, eps[:ne.value].copy()
 
 
 def nssg_entry_points(N: int, seed: int):

@@ -538,6 +538,248 @@ int tg_read_fvecs(const char* path, float** rows, uint32_t* N, uint32_t* D) {
   return TG_OK;
 }
 
+// Host readers of the other vector formats.  bvecs: per vector u32 dim then dim bytes;
+// fbin: u32 N, u32 D, then N*D f32 (the big-ann layout); both give f32 rows.
+int tg_read_bvecs(const char* path, float** rows, uint32_t* N, uint32_t* D) {
+  if (!path || !rows || !N || !D) return fail(TG_ERR_ARG, "null argument");
+  *rows = nullptr;
+  FILE* f = fopen(path, "rb");
+  if (!f) return fail(TG_ERR_IO, std::string("cannot open ") + path);
+  uint32_t dim = 0;
+  if (fread(&dim, 4, 1, f) != 1 || dim == 0) {
+    fclose(f);
+    return fail(TG_ERR_IO, std::string("short or empty bvecs file ") + path);
+  }
+  fseek(f, 0, SEEK_END);
+  const long long size = ftell(f);
+  const long long rec = 4ll + dim;
+  if (size % rec != 0) {
+    fclose(f);
+    return fail(TG_ERR_IO, std::string("bvecs size is not a multiple of the record: ") + path);
+  }
+  const uint64_t n = uint64_t(size / rec);
+  float* out = static_cast<float*>(malloc(n * dim * 4 + 4));
+  std::vector<uint8_t> buf(dim);
+  if (!out) {
+    fclose(f);
+    return fail(TG_ERR_IO, "out of host memory");
+  }
+  fseek(f, 0, SEEK_SET);
+  for (uint64_t i = 0; i < n; ++i) {
+    uint32_t d2 = 0;
+    if (fread(&d2, 4, 1, f) != 1 || d2 != dim || fread(buf.data(), 1, dim, f) != dim) {
+      fclose(f);
+      free(out);
+      return fail(TG_ERR_IO, std::string("inconsistent bvecs record in ") + path);
+    }
+    for (uint32_t j = 0; j < dim; ++j) out[i * dim + j] = float(buf[j]);
+  }
+  fclose(f);
+  *rows = out;
+  *N = uint32_t(n);
+  *D = dim;
+  return TG_OK;
+}
+
+int tg_read_fbin(const char* path, float** rows, uint32_t* N, uint32_t* D) {
+  if (!path || !rows || !N || !D) return fail(TG_ERR_ARG, "null argument");
+  *rows = nullptr;
+  FILE* f = fopen(path, "rb");
+  if (!f) return fail(TG_ERR_IO, std::string("cannot open ") + path);
+  uint32_t hd[2] = {0, 0};
+  if (fread(hd, 4, 2, f) != 2 || hd[0] == 0 || hd[1] == 0) {
+    fclose(f);
+    return fail(TG_ERR_IO, std::string("short or empty fbin header in ") + path);
+  }
+  fseek(f, 0, SEEK_END);
+  const long long size = ftell(f);
+  const uint64_t want = 8ull + 4ull * uint64_t(hd[0]) * hd[1];
+  if (uint64_t(size) != want) {
+    fclose(f);
+    return fail(TG_ERR_IO, std::string("fbin size does not match its header: ") + path);
+  }
+  const uint64_t n = uint64_t(hd[0]) * hd[1];
+  float* out = static_cast<float*>(malloc(n * 4 + 4));
+  if (!out) {
+    fclose(f);
+    return fail(TG_ERR_IO, "out of host memory");
+  }
+  fseek(f, 8, SEEK_SET);
+  if (fread(out, 4, n, f) != n) {
+    fclose(f);
+    free(out);
+    return fail(TG_ERR_IO, std::string("short fbin file ") + path);
+  }
+  fclose(f);
+  *rows = out;
+  *N = hd[0];
+  *D = hd[1];
+  return TG_OK;
+}
+
+// Reader of the five index layouts tg_store_index writes (the reference's StoreNSG ...
+// StoreCAGRA, common.cu:151-248).  N rows; width = the row capacity (NSG / NSSG header, the
+// Vamana max degree, the DPG largest degree, the CAGRA d); deg [N] and ids [N*width]
+// (malloc'ed, padded with INV); eps: the entry points the file names (NSG / Vamana: 1,
+// NSSG: 10, DPG / CAGRA: 0).
+int tg_read_index(const char* path, const char* index_type, uint32_t N, uint32_t* width,
+                  uint32_t** deg, uint32_t** ids, uint32_t* eps, uint32_t* n_eps) {
+  if (!path || !index_type || !width || !deg || !ids || !eps || !n_eps)
+    return fail(TG_ERR_ARG, "null argument");
+  *deg = nullptr;
+  *ids = nullptr;
+  *n_eps = 0;
+  const std::string t(index_type);
+  int kind = -1;
+  if (t == "NSG") kind = 0;
+  else if (t == "Vamana") kind = 1;
+  else if (t == "NSSG") kind = 2;
+  else if (t == "DPG") kind = 3;
+  else if (t == "CAGRA") kind = 4;
+  else return fail(TG_ERR_ARG, "unknown index type " + t);
+  if (N == 0) return fail(TG_ERR_ARG, "N must be >= 1");
+  FILE* f = fopen(path, "rb");
+  if (!f) return fail(TG_ERR_IO, std::string("cannot open ") + path);
+  std::vector<uint32_t> words;
+  {
+    fseek(f, 0, SEEK_END);
+    const long long size = ftell(f);
+    fseek(f, 0, SEEK_SET);
+    if (size < 0 || size % 4 != 0) {
+      fclose(f);
+      return fail(TG_ERR_IO, std::string("index size is not a multiple of 4: ") + path);
+    }
+    words.resize(size_t(size / 4));
+    if (!words.empty() && fread(words.data(), 4, words.size(), f) != words.size()) {
+      fclose(f);
+      return fail(TG_ERR_IO, std::string("short read of ") + path);
+    }
+  }
+  fclose(f);
+  size_t pos = 0;
+  auto bad = [&](const char* what) {
+    return fail(TG_ERR_IO, std::string("malformed ") + t + " index (" + what + "): " + path);
+  };
+  auto take = [&](uint32_t& v) {
+    if (pos >= words.size()) return false;
+    v = words[pos++];
+    return true;
+  };
+  uint32_t w = 0;
+  if (kind == 0 || kind == 2) {
+    if (!take(w)) return bad("header");
+    if (kind == 0) {
+      if (!take(eps[0])) return bad("header");
+      *n_eps = 1;
+    } else {
+      uint32_t ne = 0;
+      if (!take(ne) || ne > 10) return bad("entry points");
+      for (uint32_t i = 0; i < ne; ++i)
+        if (!take(eps[i])) return bad("entry points");
+      *n_eps = ne;
+    }
+  } else if (kind == 1) {
+    uint32_t lo = 0, hi = 0, f0 = 0, f1 = 0;
+    if (!take(lo) || !take(hi) || !take(w) || !take(eps[0]) || !take(f0) || !take(f1))
+      return bad("header");
+    if ((uint64_t(hi) << 32 | lo) != uint64_t(words.size()) * 4) return bad("index_size");
+    *n_eps = 1;
+  } else if (kind == 3) {
+    uint32_t m0 = 0, m1 = 0, vmaj = 0, vmin = 0, n = 0;
+    if (!take(m0) || !take(m1) || !take(vmaj) || !take(vmin) || !take(n)) return bad("header");
+    if (std::memcmp(&m0, "KNNG", 4) != 0 || std::memcmp(&m1, "RAPH", 4) != 0 || n != N)
+      return bad("magic or N");
+  }
+  // rows: kind 3 repeats the degree; kind 4 rows have no separate degree (width d each)
+  const size_t body = pos;
+  if (kind == 3 || kind == 4) {
+    w = 0;
+    for (uint32_t u = 0; u < N; ++u) {
+      uint32_t d = 0;
+      if (!take(d)) return bad("rows");
+      if (kind == 3) {
+        uint32_t d2 = 0;
+        if (!take(d2) || d2 != d) return bad("degree pair");
+      }
+      if (pos + d > words.size()) return bad("rows");
+      pos += d;
+      w = std::max(w, d);
+    }
+    pos = body;
+  }
+  if (w == 0 || w > 128) return bad("row width must be in [1, 128]");
+  uint32_t* dg = static_cast<uint32_t*>(malloc(size_t(N) * 4));
+  uint32_t* id = static_cast<uint32_t*>(malloc(size_t(N) * w * 4));
+  if (!dg || !id) {
+    free(dg);
+    free(id);
+    return fail(TG_ERR_IO, "out of host memory");
+  }
+  int rc = TG_OK;
+  for (uint32_t u = 0; u < N && rc == TG_OK; ++u) {
+    uint32_t d = 0;
+    if (!take(d)) rc = bad("rows");
+    if (rc == TG_OK && kind == 3) pos += 1;
+    if (rc == TG_OK && (d > w || pos + d > words.size())) rc = bad("row degree");
+    if (rc != TG_OK) break;
+    dg[u] = d;
+    for (uint32_t j = 0; j < w; ++j) {
+      const uint32_t v = j < d ? words[pos + j] : TG_INV;
+      if (j < d && v >= N) rc = bad("id out of range");
+      id[size_t(u) * w + j] = v;
+    }
+    pos += d;
+  }
+  if (rc == TG_OK && pos != words.size()) rc = bad("trailing data or wrong N");
+  for (uint32_t i = 0; rc == TG_OK && i < *n_eps; ++i)
+    if (eps[i] >= N) rc = bad("entry point out of range");
+  if (rc != TG_OK) {
+    free(dg);
+    free(id);
+    return rc;
+  }
+  *width = w;
+  *deg = dg;
+  *ids = id;
+  return TG_OK;
+}
+
+// Installs a pruned graph (deg [N], ids [N*width], entry point) as the handle's current one,
+// e.g. an index read with tg_read_index, so that tg_search and tg_store_index use it.
+int tg_set_pruned(tg_state* s, uint32_t width, const uint32_t* deg, const uint32_t* ids,
+                  uint32_t ep) {
+  if (!s || !deg || !ids) return fail(TG_ERR_ARG, "null argument");
+  if (s->nseg) return fail(TG_ERR_STATE, "a pruned graph is set on a handle without segments");
+  if (width < 1 || width > 128) return fail(TG_ERR_ARG, "width must be in [1, 128]");
+  if (ep >= s->N) return fail(TG_ERR_ARG, "entry point out of range");
+  for (uint32_t u = 0; u < s->N; ++u) {
+    if (deg[u] > width) return fail(TG_ERR_ARG, "degree above width");
+    for (uint32_t j = 0; j < deg[u]; ++j)
+      if (ids[size_t(u) * width + j] >= s->N) return fail(TG_ERR_ARG, "id out of range");
+  }
+  HIPX(hipSetDevice(s->device));
+  const uint32_t N = s->N;
+  if (s->FD_alloc < width) {
+    dfree(s->fwd_deg);
+    dfree(s->fwd_ids);
+    dfree(s->fwd_d);
+    dfree(s->out_deg);
+    dfree(s->out_ids);
+    if (dalloc(&s->fwd_deg, N) || dalloc(&s->fwd_ids, size_t(N) * width) ||
+        dalloc(&s->fwd_d, size_t(N) * width) || dalloc(&s->out_deg, N) ||
+        dalloc(&s->out_ids, size_t(N) * width))
+      return TG_ERR_HIP;
+    s->FD_alloc = width;
+  }
+  HIPX(hipMemcpy(s->out_deg, deg, size_t(N) * 4, hipMemcpyHostToDevice));
+  HIPX(hipMemcpy(s->out_ids, ids, size_t(N) * width * 4, hipMemcpyHostToDevice));
+  s->last_prune = tg_prune_params{};
+  s->last_prune.FD = width;
+  s->ep = ep;
+  s->has_prune = true;
+  return TG_OK;
+}
+
 void tg_host_free(void* p) { free(p); }
 
 int tg_set_profiling(tg_state* s, int on) {

@@ -129,3 +129,30 @@ def test_search_rejects_bad_arguments():
             s.search(Q, k=5, L=10, ep=500)
         with pytest.raises(TagoreError):
             s.search_recall(Q, np.zeros((4, 20), np.uint32), 20)
+
+
+@pytest.mark.parametrize("index_type,kind,fr,rr,th", [
+    ("NSG", 0, 0, 0, 1.0), ("Vamana", 0, 0, 0, 1.2), ("NSSG", 1, 1, 1, 0.5),
+    ("CAGRA", 2, 0, 2, 0.0), ("DPG", 3, 0, 2, 0.0),
+])
+def test_stored_index_loads_back_and_searches_alike(tmp_path, index_type, kind, fr, rr, th):
+    """tg_store_index -> tg_read_index -> tg_set_pruned gives the same graph, and the search
+    over the loaded index equals the search over the pruned one (same entry point)."""
+    Xall = gmm_int(2200, 64, 16, 31)
+    X, Q = Xall[:2000], Xall[2000:]
+    path = str(tmp_path / f"idx.{index_type}")
+    s, deg, ids, ep = _pruned(X, 32, kind, fr, rr, th, FD=24)
+    with s:
+        s.store_index(path, index_type)
+        a = s.search(Q, k=10, L=48, seed=2, ep=ep)
+    with State(*X.shape) as t:
+        t.set_data(X)
+        eps = t.load_index(path, index_type)
+        d2, i2, ep2 = t.get_pruned()
+        assert np.array_equal(d2, deg)
+        for u in range(X.shape[0]):
+            assert np.array_equal(i2[u, :deg[u]], ids[u, :deg[u]])
+        if index_type in ("NSG", "Vamana"):
+            assert eps.tolist() == [ep] and ep2 == ep
+        b = t.search(Q, k=10, L=48, seed=2, ep=ep)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
