@@ -67,15 +67,19 @@ __global__ __launch_bounds__(64) void k_init_ids(KnnArgs a) {
 template <int MC>
 __global__ __launch_bounds__(64) void k_init_sort(KnnArgs a) {
   __shared__ uint64_t kk[128];
+  __shared__ uint32_t row[128];
   const uint32_t u = row_of_block(a.perm, a.N);
   if (u >= a.N) return;
-  const int lane = threadIdx.x, g = lane >> 4, t = lane & 15;
+  const int lane = threadIdx.x, t = lane & 15;
   const int K = a.K, D = a.D, n = pow2_at_least(K);
   Row16<MC> xu;
   load_row16<MC>(xu, a.X + size_t(u) * D, D, t);
-  for (int j = lane; j < n; j += TG_WAVE) kk[j] = TG_KEY_INF;
+  // the list's ids in LDS first, so the distance batches wait for one load, not two
+  for (int j = lane; j < n; j += TG_WAVE) {
+    kk[j] = TG_KEY_INF;
+    if (j < K) row[j] = a.G[size_t(u) * K + j];
+  }
   __syncthreads();
-  const uint32_t* row = a.G + size_t(u) * K;
   eval_batch<MC>(a.X, D, xu, K, [&](int q) { return row[q]; },
                  [&](int q, float d) { kk[q] = mk_key(d, row[q]); });
   __syncthreads();

@@ -341,7 +341,9 @@ __global__ __launch_bounds__(256) void k_km_assign(const float* __restrict__ Z,
                                                    const float* __restrict__ cent,
                                                    const float* __restrict__ cn, uint32_t C,
                                                    uint32_t* lab, uint32_t* lab2) {
-  __shared__ float sc[64][MLP_CODE + 1];
+  // every lane reads the same centroid at a time (broadcast), so rows need no padding and
+  // are read as float4
+  __shared__ float4 sc[64][MLP_CODE / 4];
   __shared__ float sn[64];
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   float z[MLP_CODE];
@@ -357,9 +359,11 @@ __global__ __launch_bounds__(256) void k_km_assign(const float* __restrict__ Z,
   uint32_t c1 = 0, c2 = 0;
   for (uint32_t c0 = 0; c0 < C; c0 += 64) {
     __syncthreads();
-    for (uint32_t e = threadIdx.x; e < 64 * MLP_CODE; e += 256) {
-      const uint32_t c = c0 + e / MLP_CODE;
-      sc[e / MLP_CODE][e % MLP_CODE] = c < C ? cent[size_t(c) * MLP_CODE + e % MLP_CODE] : 0.f;
+    for (uint32_t e = threadIdx.x; e < 64 * MLP_CODE / 4; e += 256) {
+      const uint32_t c = c0 + e / (MLP_CODE / 4), k4 = e % (MLP_CODE / 4);
+      sc[e / (MLP_CODE / 4)][k4] =
+          c < C ? *reinterpret_cast<const float4*>(cent + size_t(c) * MLP_CODE + 4 * k4)
+                : make_float4(0.f, 0.f, 0.f, 0.f);
     }
     if (threadIdx.x < 64) sn[threadIdx.x] = c0 + threadIdx.x < C ? cn[c0 + threadIdx.x] : INFINITY;
     __syncthreads();
@@ -367,7 +371,13 @@ __global__ __launch_bounds__(256) void k_km_assign(const float* __restrict__ Z,
     for (uint32_t q = 0; q < cend; ++q) {
       float dot = 0.f;
 #pragma unroll
-      for (int k = 0; k < int(MLP_CODE); ++k) dot = fmaf(z[k], sc[q][k], dot);
+      for (int k4 = 0; k4 < int(MLP_CODE) / 4; ++k4) {
+        const float4 w = sc[q][k4];
+        dot = fmaf(z[4 * k4], w.x, dot);
+        dot = fmaf(z[4 * k4 + 1], w.y, dot);
+        dot = fmaf(z[4 * k4 + 2], w.z, dot);
+        dot = fmaf(z[4 * k4 + 3], w.w, dot);
+      }
       const float s = sn[q] - 2.f * dot;
       const uint32_t c = c0 + q;
       if (s < s1) {
