@@ -281,12 +281,14 @@ __global__ __launch_bounds__(256) void k_localjoin_screen(KnnArgs a) {
   auto divA = [&](int x) { return int((uint32_t(x) * mA) >> 16); };
   const int nj0 = nA * tB, nj1 = nB * tA, njobs = nj0 + nj1 + nA * tA;
   uint32_t n_exact = 0;
-  for (int base = 0; base < njobs; base += QJ) {
-    // ---- band masks (threads 0..QJ-1 = wave 0)
+  // exact screens need no queue: every thread of the workgroup takes a job; otherwise the
+  // band masks of QJ jobs at a time are formed by wave 0
+  const int JSTEP = a.exact_screen ? 256 : QJ;
+  for (int base = 0; base < njobs; base += JSTEP) {
     int kind = 0, r = 0, lo = 0;
     uint32_t mask = 0;
     const int J = base + tid;
-    if (tid < QJ && J < njobs) {
+    if (tid < JSTEP && J < njobs) {
       if (J < nj0) {
         kind = 0; r = divB(J); lo = (J - r * tB) * LJ_TILE;
       } else if (J < nj0 + nj1) {
