@@ -450,6 +450,78 @@ __device__ __forceinline__ float dot_f8(const RowF8<MF>& a, const uint8_t* y, in
   return s;
 }
 
+// Four candidates at a time (their loads issued before any arithmetic); each result is
+// the same operation sequence as dot_bf16 / dot_f8 of that candidate.
+template <int MB>
+__device__ __forceinline__ void dot_bf16x4(const RowBf<MB>& a, const uint16_t* const (&y)[4],
+                                           int D, int t, float (&out)[4]) {
+  uint4 q[4][MB];
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int m = 0; m < MB; ++m) {
+      const int i = 128 * m + 8 * t;
+      q[b][m] = i < D ? *reinterpret_cast<const uint4*>(y[b] + i) : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    float s = 0.f;
+#pragma unroll
+    for (int m = 0; m < MB; ++m) {
+      if (128 * m + 8 * t < D) {
+        const uint32_t w[4] = {q[b][m].x, q[b][m].y, q[b][m].z, q[b][m].w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          s = __builtin_fmaf(a.v[m][2 * k], bfbits2f(w[k] & 0xFFFFu), s);
+          s = __builtin_fmaf(a.v[m][2 * k + 1], bfbits2f(w[k] >> 16), s);
+        }
+      }
+    }
+    out[b] = s;
+  }
+#pragma unroll
+  for (int o = 8; o >= 1; o >>= 1)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) out[b] += __shfl_xor(out[b], o, 16);
+}
+
+template <int MF>
+__device__ __forceinline__ void dot_f8x4(const RowF8<MF>& a, const uint8_t* const (&y)[4], int D,
+                                         int t, float (&out)[4]) {
+  uint4 q[4][MF];
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int m = 0; m < MF; ++m) {
+      const int i = 256 * m + 16 * t;
+      q[b][m] = i < D ? *reinterpret_cast<const uint4*>(y[b] + i) : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    float s = 0.f;
+#pragma unroll
+    for (int m = 0; m < MF; ++m) {
+      if (256 * m + 16 * t < D) {
+        const int w[4] = {int(q[b][m].x), int(q[b][m].y), int(q[b][m].z), int(q[b][m].w)};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const f32x2 lo = __builtin_amdgcn_cvt_pk_f32_fp8(w[k], false);
+          const f32x2 hi = __builtin_amdgcn_cvt_pk_f32_fp8(w[k], true);
+          s = __builtin_fmaf(a.v[m][4 * k], lo.x, s);
+          s = __builtin_fmaf(a.v[m][4 * k + 1], lo.y, s);
+          s = __builtin_fmaf(a.v[m][4 * k + 2], hi.x, s);
+          s = __builtin_fmaf(a.v[m][4 * k + 3], hi.y, s);
+        }
+      }
+    }
+    out[b] = s;
+  }
+#pragma unroll
+  for (int o = 8; o >= 1; o >>= 1)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) out[b] += __shfl_xor(out[b], o, 16);
+}
+
 __device__ __forceinline__ float screen_bound2(float na, float qa, float ea, float nb, float qb,
                                                float eb, float eta) {
   return 2.0f * (ea * qb + qa * eb + ea * eb) + eta * (na + nb + 2.0f * qa * qb);
@@ -481,15 +553,21 @@ __global__ __launch_bounds__(64) void k_phase2(KnnArgs a) {
     nl = min(LEADERS, nl + int(__popcll(m)));
   }
   __syncthreads();
-  // gather candidate ids
-  int tot = 0;
-  for (int l = 0; l < nl; ++l) {
+  // gather candidate ids: the leaders' counts first, then every leader's loads back to back
+  const int nf = min(TAKE, K);
+  const int cnt_l = lane < nl ? nf + min(TAKE, int(a.Rn0[lead[lane]])) : 0;
+  int off[LEADERS + 1];
+  off[0] = 0;
+#pragma unroll
+  for (int l = 0; l < LEADERS; ++l) off[l + 1] = off[l] + __shfl(cnt_l, l, TG_WAVE);
+  const int tot = off[LEADERS];
+#pragma unroll
+  for (int l = 0; l < LEADERS; ++l) {
+    if (l >= nl) break;
     const uint32_t v = lead[l];
-    const int nf = min(TAKE, K);
-    const int nr = min(TAKE, int(a.Rn0[v]));
-    for (int i = lane; i < nf + nr; i += TG_WAVE)
-      ids[tot + i] = i < nf ? a.G[size_t(v) * K + i] : key_id(a.R0[size_t(v) * K + (i - nf)]);
-    tot += nf + nr;
+    for (int i = lane; i < off[l + 1] - off[l]; i += TG_WAVE)
+      ids[off[l] + i] =
+          i < nf ? a.G[size_t(v) * K + i] : key_id(a.R0[size_t(v) * K + (i - nf)]);
   }
   const int n = pow2_at_least(max(tot, 1));
   for (int i = tot + lane; i < n; i += TG_WAVE) ids[i] = TG_INV;
@@ -535,18 +613,41 @@ __global__ __launch_bounds__(64) void k_phase2(KnnArgs a) {
   int nre = cnt;
   if (SCREEN) {
     __shared__ uint16_t keep[512];
-    for (int q = g; q < cnt; q += 4) {
-      const uint32_t y = ids[cl[q]];
-      const float dot = PREC == KP_FP8
-                            ? (su * a.sc8[y]) * dot_f8(uf, a.Xq8 + size_t(y) * D, D, t)
-                            : dot_bf16(ub, a.Xb + size_t(y) * D, D, t);
-      const float s = (nu + a.nrm2[y]) - 2.0f * dot;
-      if (a.exact_screen) {
-        // integer data: s is the canonical distance itself
-        if (t == 0 && s < worst) kv[q] = mk_key(s, y);
+    // item q0 + g + 4b in group g, four candidates per group in flight
+    for (int q0 = 0; q0 < cnt; q0 += 16) {
+      uint32_t y[4];
+      float dp[4], nb[4];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int q = q0 + g + 4 * b;
+        y[b] = q < cnt ? ids[cl[q]] : u;  // unused slots read row u
+        nb[b] = a.nrm2[y[b]];
+      }
+      if (PREC == KP_FP8) {
+        const uint8_t* r8[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) r8[b] = a.Xq8 + size_t(y[b]) * D;
+        dot_f8x4(uf, r8, D, t, dp);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) dp[b] = (su * a.sc8[y[b]]) * dp[b];
       } else {
-        const float E = screen_bound2(nu, qu, eu, a.nrm2[y], a.nq[y], a.ne[y], a.eta);
-        if (t == 0) keep[q] = (s - E < worst) ? 1 : 0;
+        const uint16_t* rb[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) rb[b] = a.Xb + size_t(y[b]) * D;
+        dot_bf16x4(ub, rb, D, t, dp);
+      }
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int q = q0 + g + 4 * b;
+        if (q >= cnt) continue;
+        const float s = (nu + nb[b]) - 2.0f * dp[b];
+        if (a.exact_screen) {
+          // integer data: s is the canonical distance itself
+          if (t == 0 && s < worst) kv[q] = mk_key(s, y[b]);
+        } else {
+          const float E = screen_bound2(nu, qu, eu, nb[b], a.nq[y[b]], a.ne[y[b]], a.eta);
+          if (t == 0) keep[q] = (s - E < worst) ? 1 : 0;
+        }
       }
     }
     __syncthreads();
