@@ -119,7 +119,7 @@ struct tg_state {
   bool cert_int = false;           // integer data whose screen distances are exact
   uint32_t* fill = nullptr;
   uint32_t* scan_tmp = nullptr;
-  unsigned long long* counters = nullptr;  // [16]
+  unsigned long long* counters = nullptr;  // [CNT_SLOTS * CNT_N], summed over the slots
   // prune
   uint32_t FD_alloc = 0;
   uint32_t *fwd_deg = nullptr, *fwd_ids = nullptr, *out_deg = nullptr, *out_ids = nullptr;
@@ -196,7 +196,7 @@ void clear_errors(tg_state* s, std::initializer_list<PB*> pbs) {
 int ensure_common_scratch(tg_state* s) {
   if (!s->fill) {
     if (dalloc(&s->fill, s->N) || dalloc(&s->scan_tmp, scan_tmp_words(s->N) + 16) ||
-        dalloc(&s->counters, 16))
+        dalloc(&s->counters, size_t(CNT_SLOTS) * CNT_N))
       return TG_ERR_HIP;
   }
   return TG_OK;
@@ -347,6 +347,17 @@ struct Sections {
 int check_launch(const char* what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(TG_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  return TG_OK;
+}
+
+// the evaluation counters, summed over their slots
+int read_counters(tg_state* s, unsigned long long (&cnt)[CNT_N]) {
+  std::vector<unsigned long long> all(size_t(CNT_SLOTS) * CNT_N);
+  HIPX(hipMemcpy(all.data(), s->counters, all.size() * 8, hipMemcpyDeviceToHost));
+  for (int i = 0; i < CNT_N; ++i) {
+    cnt[i] = 0;
+    for (int b = 0; b < CNT_SLOTS; ++b) cnt[i] += all[size_t(b) * CNT_N + i];
+  }
   return TG_OK;
 }
 
@@ -1012,7 +1023,7 @@ int tg_knn_build(tg_state* s, const tg_knn_params* p) {
   HIPX(hipEventCreate(&w1));
   Sections sec{s};
   sec.next = 0;
-  fill_u32(reinterpret_cast<uint32_t*>(s->counters), 0, 32, st);
+  fill_u32(reinterpret_cast<uint32_t*>(s->counters), 0, 2 * CNT_SLOTS * CNT_N, st);
   clear_errors(s, {&s->pn, &s->po, &s->pc, &s->pr});
   HIPX(hipEventRecord(w0, st));
   // precision-dependent parts of the argument block (re-applied after every swap)
@@ -1118,8 +1129,8 @@ int tg_knn_build(tg_state* s, const tg_knn_params* p) {
   s->stats.knn_ms = ms;
   for (double& v : s->stats.knn_kernel_ms) v = 0.0;
   if (sec.collect(s->stats.knn_kernel_ms)) return TG_ERR_HIP;
-  unsigned long long cnt[8];
-  HIPX(hipMemcpy(cnt, s->counters, sizeof(cnt), hipMemcpyDeviceToHost));
+  unsigned long long cnt[CNT_N];
+  if (int rc = read_counters(s, cnt)) return rc;
   s->stats.knn_evals = cnt[0] + extra_evals;
   s->stats.knn_rescored = cnt[1];
   if (check_pb_err(s, s->pn, "reverse sampling") || check_pb_err(s, s->po, "reverse sampling") ||
@@ -1238,7 +1249,7 @@ int tg_prune(tg_state* s, const tg_prune_params* p) {
   HIPX(hipEventCreate(&w0));
   HIPX(hipEventCreate(&w1));
   Sections sec{s};
-  fill_u32(reinterpret_cast<uint32_t*>(s->counters), 0, 32, st);
+  fill_u32(reinterpret_cast<uint32_t*>(s->counters), 0, 2 * CNT_SLOTS * CNT_N, st);
   fill_u32(s->ep_dev, 0, 1, st);
   clear_errors(s, {&s->rev});
   HIPX(hipEventRecord(w0, st));
@@ -1296,8 +1307,8 @@ int tg_prune(tg_state* s, const tg_prune_params* p) {
   s->stats.prune_ms = ms;
   for (double& v : s->stats.prune_kernel_ms) v = 0.0;
   if (sec.collect(s->stats.prune_kernel_ms)) return TG_ERR_HIP;
-  unsigned long long cnt[15];
-  HIPX(hipMemcpy(cnt, s->counters, sizeof(cnt), hipMemcpyDeviceToHost));
+  unsigned long long cnt[CNT_N];
+  if (int rc = read_counters(s, cnt)) return rc;
   for (int i = 0; i < 15; ++i) s->stats.prune_counters[i] = cnt[i];
   if (s->nseg) {
     s->seg_ep_host.resize(s->nseg);

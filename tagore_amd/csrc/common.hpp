@@ -41,6 +41,16 @@ __device__ __forceinline__ int mask_rank(uint64_t m) {
 }
 
 // ---------------------------------------------------------------------------------------
+// Evaluation counters: CNT_N counters in each of CNT_SLOTS 128-byte slots; workgroup b adds
+// to slot b % CNT_SLOTS, so that a million workgroups do not serialise their atomics on one
+// cache line.  The host sums the slots.
+// ---------------------------------------------------------------------------------------
+constexpr int CNT_N = 16, CNT_SLOTS = 64;
+__device__ __forceinline__ unsigned long long* cnt_slot(unsigned long long* c) {
+  return c + (blockIdx.x % CNT_SLOTS) * CNT_N;
+}
+
+// ---------------------------------------------------------------------------------------
 // Canonical distance.  `MC` = number of 64-element chunks held (ceil(D/64) <= MC).
 // Row a is preloaded by the group (ra[]), row b is read from memory.  The value is valid in
 // lane 0 of the group.  Requires D % 4 == 0 (host checks).

@@ -64,7 +64,9 @@ __global__ __launch_bounds__(64) void k_init_ids(KnnArgs a) {
   for (int j = lane; j < K; j += TG_WAVE) a.G[size_t(u) * K + j] = acc[j];
 }
 
-template <int MC>
+// Distances of the initial lists and their (d, id) sort.  T: FP32 rows, or the BF16 copy
+// (Xr16) when every value is exact in BF16 (the same values, half the bytes).
+template <int MC, class T>
 __global__ __launch_bounds__(64) void k_init_sort(KnnArgs a) {
   __shared__ uint64_t kk[128];
   __shared__ uint32_t row[128];
@@ -72,15 +74,16 @@ __global__ __launch_bounds__(64) void k_init_sort(KnnArgs a) {
   if (u >= a.N) return;
   const int lane = threadIdx.x, t = lane & 15;
   const int K = a.K, D = a.D, n = pow2_at_least(K);
+  const T* X = a.Xr16 ? reinterpret_cast<const T*>(a.Xr16) : reinterpret_cast<const T*>(a.X);
   Row16<MC> xu;
-  load_row16<MC>(xu, a.X + size_t(u) * D, D, t);
+  load_row16<MC>(xu, X + size_t(u) * D, D, t);
   // the list's ids in LDS first, so the distance batches wait for one load, not two
   for (int j = lane; j < n; j += TG_WAVE) {
     kk[j] = TG_KEY_INF;
     if (j < K) row[j] = a.G[size_t(u) * K + j];
   }
   __syncthreads();
-  eval_batch<MC>(a.X, D, xu, K, [&](int q) { return row[q]; },
+  eval_batch<MC>(X, D, xu, K, [&](int q) { return row[q]; },
                  [&](int q, float d) { kk[q] = mk_key(d, row[q]); });
   __syncthreads();
   wave_sort<128>(kk, n);
@@ -89,7 +92,7 @@ __global__ __launch_bounds__(64) void k_init_sort(KnnArgs a) {
     a.GD[size_t(u) * K + j] = key_d(kk[j]);
     a.V[size_t(u) * K + j] = 0;
   }
-  if (a.counters && lane == 0) atomicAdd(&a.counters[0], (unsigned long long)K);
+  if (a.counters && lane == 0) atomicAdd(&cnt_slot(a.counters)[0], (unsigned long long)K);
 }
 
 // ------------------------------------------------------------------------- phase 1
@@ -253,7 +256,7 @@ __global__ __launch_bounds__(256) void k_localjoin_exact(KnnArgs a) {
       pb_push(a.pc, A[r], best);
   }
   if (a.counters && tid == 0)
-    atomicAdd(&a.counters[0], (unsigned long long)(nA * nB + nA * (nA - 1) / 2));
+    atomicAdd(&cnt_slot(a.counters)[0], (unsigned long long)(nA * nB + nA * (nA - 1) / 2));
 }
 
 // ------------------------------------------------------------------ row merge helper
@@ -670,7 +673,7 @@ __global__ __launch_bounds__(64) void k_phase2(KnnArgs a) {
                  [&](int q, float d) {
                    if (d < worst) kv[q] = mk_key(d, ids[cl[q]]);
                  });
-  if (a.counters && lane == 0 && nre) atomicAdd(&a.counters[1], (unsigned long long)nre);
+  if (a.counters && lane == 0 && nre) atomicAdd(&cnt_slot(a.counters)[1], (unsigned long long)nre);
   __syncthreads();
   wave_sort<512>(kv, nk);
   int nc = 0;
@@ -687,7 +690,7 @@ __global__ __launch_bounds__(64) void k_phase2(KnnArgs a) {
     const int pos = b + lower_bound_lds(row.gk, K, k);
     if (pos < K) pb_push(a.pr, key_id(k), mk_key(key_d(k), u));
   }
-  if (a.counters && lane == 0) atomicAdd(&a.counters[0], (unsigned long long)cnt);
+  if (a.counters && lane == 0) atomicAdd(&cnt_slot(a.counters)[0], (unsigned long long)cnt);
 }
 
 // R1[w] = K smallest of R0[w] and this iteration's pushes
@@ -744,12 +747,23 @@ __global__ __launch_bounds__(64) void k_final(KnnArgs a) {
     else KERNEL<16><<<GRID, BLOCK, 0, S>>> ARGS;                     \
   } while (0)
 
-void launch_knn_init_sort(const KnnArgs& a, hipStream_t s) {
-  TG_MC_DISPATCH(k_init_sort, grid_rows(a.perm, a.N), 64, s, (a));
+static void launch_init_sort(const KnnArgs& a, hipStream_t s) {
+  const int mc = (int(a.D) + 63) / 64;
+  const uint32_t grid = grid_rows(a.perm, a.N);
+#define IS(M, T) k_init_sort<M, T><<<grid, 64, 0, s>>>(a)
+#define IS_MC(T)                                                     \
+  do {                                                               \
+    if (mc <= 2) IS(2, T); else if (mc <= 4) IS(4, T); else IS(16, T); \
+  } while (0)
+  if (a.Xr16) IS_MC(uint16_t);
+  else IS_MC(float);
+#undef IS_MC
+#undef IS
 }
+void launch_knn_init_sort(const KnnArgs& a, hipStream_t s) { launch_init_sort(a, s); }
 void launch_knn_init(const KnnArgs& a, hipStream_t s) {
   k_init_ids<<<a.N, 64, 0, s>>>(a);
-  TG_MC_DISPATCH(k_init_sort, grid_rows(a.perm, a.N), 64, s, (a));
+  launch_init_sort(a, s);
 }
 void launch_knn_sample(const KnnArgs& a, hipStream_t s) { k_sample<<<a.N, 64, 0, s>>>(a); }
 void launch_knn_rsample(const KnnArgs& a, hipStream_t s) { k_rsample<<<a.N, 64, 0, s>>>(a); }

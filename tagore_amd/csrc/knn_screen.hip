@@ -398,8 +398,8 @@ __global__ __launch_bounds__(256) void k_localjoin_screen(KnnArgs a) {
   }
   if (a.counters && tid == 0) {
     // [0] pair evaluations (screen), [1] exact re-scores
-    atomicAdd(&a.counters[0], (unsigned long long)(nA * nB + nA * (nA - 1) / 2));
-    atomicAdd(&a.counters[1], (unsigned long long)n_exact);
+    atomicAdd(&cnt_slot(a.counters)[0], (unsigned long long)(nA * nB + nA * (nA - 1) / 2));
+    atomicAdd(&cnt_slot(a.counters)[1], (unsigned long long)n_exact);
   }
 }
 

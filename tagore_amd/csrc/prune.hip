@@ -336,7 +336,7 @@ __device__ __forceinline__ int filter_pool(const PruneArgs& a, uint64_t* P, int 
 }
 
 __device__ __forceinline__ void count_add(unsigned long long* c, int idx, uint32_t v) {
-  if (c && threadIdx.x == 0 && v) atomicAdd(&c[idx], (unsigned long long)v);
+  if (c && threadIdx.x == 0 && v) atomicAdd(&cnt_slot(c)[idx], (unsigned long long)v);
 }
 
 // -------------------------------------------------------------------------------------
@@ -457,12 +457,12 @@ __global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
   __syncthreads();
   if (a.phase_clock && lane == 0 && a.counters) {
     const long long c3 = clock64();
-    atomicAdd(&a.counters[6], (unsigned long long)(c1 - c0));
-    atomicAdd(&a.counters[7], (unsigned long long)(c3 - c2));
-    atomicAdd(&a.counters[8], (unsigned long long)(c2 - c1));
-    atomicAdd(&a.counters[9], (unsigned long long)ph.ev);
-    for (int i = 0; i < 4; ++i) atomicAdd(&a.counters[10 + i], (unsigned long long)ph.sub[i]);
-    atomicAdd(&a.counters[14], (unsigned long long)(c1s - c1));
+    atomicAdd(&cnt_slot(a.counters)[6], (unsigned long long)(c1 - c0));
+    atomicAdd(&cnt_slot(a.counters)[7], (unsigned long long)(c3 - c2));
+    atomicAdd(&cnt_slot(a.counters)[8], (unsigned long long)(c2 - c1));
+    atomicAdd(&cnt_slot(a.counters)[9], (unsigned long long)ph.ev);
+    for (int i = 0; i < 4; ++i) atomicAdd(&cnt_slot(a.counters)[10 + i], (unsigned long long)ph.sub[i]);
+    atomicAdd(&cnt_slot(a.counters)[14], (unsigned long long)(c1s - c1));
   }
   for (int k = lane; k < deg; k += TG_WAVE) {
     const uint64_t s = sel[k];

@@ -156,8 +156,8 @@ __global__ __launch_bounds__(256) void k_dpg_screen(PruneArgs a, DpgScreen q) {
   }
   __syncthreads();
   if (tid == 0 && a.counters) {
-    atomicAdd(&a.counters[1], (unsigned long long)npairs);
-    atomicAdd(&a.counters[5], (unsigned long long)nqd);
+    atomicAdd(&cnt_slot(a.counters)[1], (unsigned long long)npairs);
+    atomicAdd(&cnt_slot(a.counters)[5], (unsigned long long)nqd);
   }
   // select the d entries of smallest (detour count, id) and push the reverse pairs
   if (tid < 64) srt[tid] = tid < K ? ((uint64_t(det[tid]) << 32) | nei[tid]) : TG_KEY_INF;
