@@ -541,7 +541,12 @@ __device__ __forceinline__ void pb_push(const PushBuf& pb, uint32_t v, uint64_t 
   if (s < pb.cap) {
     pb.slots[size_t(v) * pb.cap + s] = key;
   } else {
-    const uint32_t o = atomicAdd(pb.ov_used, 1u);
+    // overflow entries of the active lanes take one allocation from the shared counter
+    const uint64_t m = __ballot(1);
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane_id() == leader) base = atomicAdd(pb.ov_used, uint32_t(__popcll(m)));
+    const uint32_t o = uint32_t(__shfl(int(base), leader, TG_WAVE)) + uint32_t(mask_rank(m));
     if (o < pb.ov_cap) {
       pb.ov_dest[o] = v;
       pb.ov_key[o] = key;
