@@ -55,7 +55,7 @@ struct Learner {
 };
 
 struct LearnConfig {
-  uint32_t B = 1024;      // anchors per batch
+  uint32_t B = 2048;      // anchors per batch
   uint32_t P = 4;         // positives per anchor
   uint32_t epochs = 2;
   uint32_t kmeans_iters = 5;

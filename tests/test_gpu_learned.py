@@ -95,7 +95,7 @@ def test_learned_lists_and_parity(kind, N, D, K):
         G0, GD0 = s.get_knn()
         st = s.stats()
         _say(f"  loss {st['learn_loss']}, buckets {st['learn_buckets']}, steps {st['learn_steps']}")
-        assert st["learn_steps"] == 2 * (5000 // 1024)  # 2 epochs by default
+        assert st["learn_steps"] == 2 * (5000 // 2048)  # 2 epochs of 2048-anchor batches
         assert st["learn_loss"][1] < st["learn_loss"][0]
         _check_lists(X, G0, GD0)
         codes = s.codes()
