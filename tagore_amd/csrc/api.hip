@@ -108,6 +108,7 @@ struct tg_state {
   uint64_t *R0 = nullptr, *R1 = nullptr;
   uint32_t *Rn0 = nullptr, *Rn1 = nullptr;
   PB pn, po, pc, pr;
+  uint32_t *rv_hist = nullptr, *rv_scan = nullptr;  // reverse-run histogram and its scan
   uint8_t* Xq8 = nullptr;  // FP8 e4m3 copy of X with per-vector scales (screen)
   float *sc8 = nullptr, *nq8 = nullptr, *ne8 = nullptr;
   bool prep8_valid = false;
@@ -211,8 +212,12 @@ int ensure_knn_scratch(tg_state* s, uint32_t K) {
     dfree(s->R1);
     dfree(s->Rn0);
     dfree(s->Rn1);
+    dfree(s->rv_hist);
+    dfree(s->rv_scan);
+    const size_t hw = rev_hist_words(uint32_t(N), K);
     if (dalloc(&s->G1, N * K) || dalloc(&s->GD1, N * K) || dalloc(&s->R0, N * K) ||
-        dalloc(&s->R1, N * K) || dalloc(&s->Rn0, N) || dalloc(&s->Rn1, N))
+        dalloc(&s->R1, N * K) || dalloc(&s->Rn0, N) || dalloc(&s->Rn1, N) ||
+        dalloc(&s->rv_hist, hw) || dalloc(&s->rv_scan, scan_tmp_words(uint32_t(hw - 1)) + 16))
       return TG_ERR_HIP;
     s->K_scratch = K;
   }
@@ -434,6 +439,8 @@ int tg_free(tg_state* s) {
   dfree(s->R1);
   dfree(s->Rn0);
   dfree(s->Rn1);
+  dfree(s->rv_hist);
+  dfree(s->rv_scan);
   s->pn.release();
   s->po.release();
   s->pc.release();
@@ -1096,10 +1103,11 @@ int tg_knn_build(tg_state* s, const tg_knn_params* p) {
   }
   if (p->iters > 0) {  // iters == 0 returns the initial lists
     if (sec.start(5)) return TG_ERR_HIP;
-    pb_reset(a.pr, N, s->fill, st);
-    launch_knn_rev_build(a, st);
-    pb_group_overflow(a.pr, N, s->fill, s->scan_tmp, st);
-    launch_knn_rev_reduce(a, st);
+    if (launch_knn_rev_build(a, s->rv_hist, s->rv_scan, st))
+      return fail(TG_ERR_ARG, "reverse lists: N and K beyond 2^26 ids or 2^32 edges");
+    KnnArgs ar = a;
+    ar.pr.cap = 0;  // the runs are the grouped overflow area
+    launch_knn_rev_reduce(ar, st);
     if (sec.stop()) return TG_ERR_HIP;
     for (uint32_t it = 0; it < it2; ++it) {
       if (sec.start(6)) return TG_ERR_HIP;

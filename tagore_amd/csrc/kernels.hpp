@@ -130,7 +130,11 @@ void launch_knn_rsample(const KnnArgs& a, hipStream_t s);
 // tile: staging depth of the MFMA screen (0: 256-byte rows per round, 1: 64-byte rows)
 void launch_knn_localjoin(const KnnArgs& a, int precision, int tile, hipStream_t s);
 void launch_knn_merge(const KnnArgs& a, hipStream_t s);
-void launch_knn_rev_build(const KnnArgs& a, hipStream_t s);
+// Reverse lists as exact runs per destination in a.pr (cnt, ov_off, ov_sorted; read them with
+// cap 0).  hist holds rev_hist_words(N, K) words, scan_tmp scan_tmp_words(that - 1).
+// Returns nonzero, launching nothing, beyond 2^26 ids or the edge capacity of a.pr.
+size_t rev_hist_words(uint32_t N, uint32_t K);
+int launch_knn_rev_build(const KnnArgs& a, uint32_t* hist, uint32_t* scan_tmp, hipStream_t s);
 void launch_knn_rev_reduce(const KnnArgs& a, hipStream_t s);
 void launch_knn_phase2(const KnnArgs& a, int precision, hipStream_t s);
 void launch_knn_rupdate(const KnnArgs& a, hipStream_t s);
@@ -142,6 +146,8 @@ void pb_reset(const PushBuf& pb, uint32_t N, uint32_t* fill, hipStream_t s);
 void pb_group_overflow(const PushBuf& pb, uint32_t N, uint32_t* fill, uint32_t* scan_tmp,
                        hipStream_t s);
 size_t scan_tmp_words(uint32_t N);
+// exclusive scan of v[0..n) in place, v[n] = total
+void excl_scan_u32(uint32_t* v, uint32_t n, uint32_t* tmp, hipStream_t s);
 
 void fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 // Beam search over a pruned graph (search.hip): queries Q [nq*D], pool L <= 256, k <= L;

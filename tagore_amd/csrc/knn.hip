@@ -334,12 +334,123 @@ __global__ __launch_bounds__(64) void k_merge(KnnArgs a) {
 }
 
 // ---------------------------------------------------------------- reverse build
-// one thread per list entry (u, j): push (d, u) to G[u][j]; a grid-stride loop over all
-// N*K entries keeps many pushes in flight per wave
-__global__ __launch_bounds__(256) void k_rev_build(KnnArgs a) {
-  const size_t n = size_t(a.N) * a.K;
-  for (size_t e = size_t(blockIdx.x) * 256 + threadIdx.x; e < n; e += size_t(gridDim.x) * 256)
-    pb_push(a.pr, a.G[e], mk_key(a.GD[e], uint32_t(e / a.K)));
+// The reverse lists of do_reverse_graph (Tagore_src.cu:1187-1206) as exact runs per
+// destination, built without a global atomic per edge (in-degrees of a kNN graph are heavy
+// tailed: a fixed number of slots per node overflows for a large share of the edges):
+//   1. k_rev_hist: each tile of edges counts its destinations per block of 2^lgR ids (LDS);
+//   2. an exclusive scan of those counts, block-major, gives every (block, tile) pair its
+//      output range;
+//   3. k_rev_part: each tile scatters its edges -- key (d, u) and destination v -- into its
+//      ranges, the rank inside a range from an LDS cursor;
+//   4. k_rev_runs: per block, LDS counts and a scan give every destination its run
+//      [off[v], off[v] + cnt[v]); LDS cursors place the keys.
+// The runs land in the reverse PushBuf as its grouped overflow with cap 0 (cnt, ov_off,
+// ov_sorted), the form pb_top reads; partitioned keys and destinations use its slot area and
+// ov_dest as scratch.  The order inside a run is unspecified: pb_top selects by (d, id).
+constexpr int REV_THREADS = 512;
+constexpr uint32_t REV_MAXP = 8192;   // destination blocks (LDS counters of hist and part)
+constexpr uint32_t REV_MAXR = 8192;   // ids per block (LDS counters of runs)
+constexpr uint32_t REV_TILES = 256;
+
+struct RevPlan {
+  uint32_t lgR, P, tiles;
+  uint64_t n, T;       // edges, edges per tile
+  uint32_t* H;         // [P * tiles + 1]
+};
+
+RevPlan rev_plan(uint32_t N, uint32_t K, uint32_t* H) {
+  RevPlan r;
+  r.lgR = 9;  // at least 512 ids per block, so every thread of k_rev_runs owns one
+  while (r.lgR < 13 && (size_t(N) >> r.lgR) > 256) ++r.lgR;
+  r.P = uint32_t((size_t(N) + (size_t(1) << r.lgR) - 1) >> r.lgR);
+  r.n = uint64_t(N) * K;
+  const uint64_t t = std::max<uint64_t>(1, std::min<uint64_t>(REV_TILES, (r.n + 4095) / 4096));
+  r.T = (r.n + t - 1) / t;
+  r.tiles = uint32_t((r.n + r.T - 1) / r.T);
+  r.H = H;
+  return r;
+}
+
+__global__ __launch_bounds__(REV_THREADS) void k_rev_hist(KnnArgs a, RevPlan r) {
+  __shared__ uint32_t h[REV_MAXP];
+  const uint32_t t = blockIdx.x;
+  for (uint32_t b = threadIdx.x; b < r.P; b += REV_THREADS) h[b] = 0;
+  __syncthreads();
+  const uint64_t e0 = uint64_t(t) * r.T, e1 = min(r.n, e0 + r.T);
+  for (uint64_t e = e0 + threadIdx.x; e < e1; e += REV_THREADS) {
+    const uint32_t v = a.G[e];
+    if (v < a.N) atomicAdd(&h[v >> r.lgR], 1u);
+  }
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < r.P; b += REV_THREADS) r.H[size_t(b) * r.tiles + t] = h[b];
+}
+
+__global__ __launch_bounds__(REV_THREADS) void k_rev_part(KnnArgs a, RevPlan r) {
+  __shared__ uint32_t cur[REV_MAXP];
+  const uint32_t t = blockIdx.x;
+  for (uint32_t b = threadIdx.x; b < r.P; b += REV_THREADS) cur[b] = r.H[size_t(b) * r.tiles + t];
+  __syncthreads();
+  uint64_t* pkey = a.pr.slots;
+  uint32_t* pv = a.pr.ov_dest;
+  const uint64_t e0 = uint64_t(t) * r.T, e1 = min(r.n, e0 + r.T);
+  for (uint64_t e = e0 + threadIdx.x; e < e1; e += REV_THREADS) {
+    const uint32_t v = a.G[e];
+    if (v < a.N) {
+      const uint32_t p = atomicAdd(&cur[v >> r.lgR], 1u);
+      if (p < r.n) {  // always, for the counts of k_rev_hist
+        pkey[p] = mk_key(a.GD[e], uint32_t(e / a.K));
+        pv[p] = v;
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(REV_THREADS) void k_rev_runs(KnnArgs a, RevPlan r) {
+  __shared__ uint32_t c[REV_MAXR];
+  __shared__ uint32_t wsum[REV_THREADS / TG_WAVE];
+  const uint32_t b = blockIdx.x, R = 1u << r.lgR, v0 = b << r.lgR;
+  const uint32_t end = r.H[size_t(r.P) * r.tiles];
+  uint32_t hi = b + 1 < r.P ? r.H[size_t(b + 1) * r.tiles] : end;
+  uint32_t lo = r.H[size_t(b) * r.tiles];
+  hi = min(hi, uint32_t(r.n));
+  lo = min(lo, hi);
+  const uint32_t* pv = a.pr.ov_dest;
+  const uint64_t* pkey = a.pr.slots;
+  for (uint32_t i = threadIdx.x; i < R; i += REV_THREADS) c[i] = 0;
+  __syncthreads();
+  for (uint32_t p = lo + threadIdx.x; p < hi; p += REV_THREADS) {
+    const uint32_t l = pv[p] - v0;
+    if (l < R) atomicAdd(&c[l], 1u);
+  }
+  __syncthreads();
+  // exclusive scan of c[0..R): thread i owns c[i*per .. (i+1)*per)
+  const uint32_t per = R / REV_THREADS, i0 = threadIdx.x * per;
+  uint32_t own = 0;
+  for (uint32_t i = 0; i < per; ++i) own += c[i0 + i];
+  const uint32_t incl = uint32_t(wave_incl_scan(int(own)));
+  const int w = threadIdx.x / TG_WAVE;
+  if (lane_id() == TG_WAVE - 1) wsum[w] = incl;
+  __syncthreads();
+  uint32_t run = incl - own;
+  for (int q = 0; q < w; ++q) run += wsum[q];
+  for (uint32_t i = 0; i < per; ++i) {
+    const uint32_t ci = c[i0 + i], v = v0 + i0 + i;
+    if (v < a.N) {
+      a.pr.cnt[v] = ci;
+      a.pr.ov_off[v] = lo + run;
+    }
+    c[i0 + i] = run;
+    run += ci;
+  }
+  if (b + 1 == r.P && threadIdx.x == 0) a.pr.ov_off[a.N] = hi;
+  __syncthreads();
+  for (uint32_t p = lo + threadIdx.x; p < hi; p += REV_THREADS) {
+    const uint32_t l = pv[p] - v0;
+    if (l < R) {
+      const uint32_t q = atomicAdd(&c[l], 1u);
+      if (lo + q < hi) a.pr.ov_sorted[lo + q] = pkey[p];
+    }
+  }
 }
 
 __global__ __launch_bounds__(64) void k_rev_reduce(KnnArgs a) {
@@ -776,10 +887,19 @@ void launch_knn_localjoin(const KnnArgs& a, int precision, int tile, hipStream_t
     launch_knn_localjoin_screen(a, precision, tile, s);
 }
 void launch_knn_merge(const KnnArgs& a, hipStream_t s) { k_merge<<<a.N, 64, 0, s>>>(a); }
-void launch_knn_rev_build(const KnnArgs& a, hipStream_t s) {
-  const size_t n = size_t(a.N) * a.K;
-  const uint32_t grid = uint32_t(std::min<size_t>((n + 255) / 256, 256 * 64));
-  k_rev_build<<<grid, 256, 0, s>>>(a);
+size_t rev_hist_words(uint32_t N, uint32_t K) {
+  const RevPlan r = rev_plan(N, K, nullptr);
+  return size_t(r.P) * r.tiles + 1;
+}
+int launch_knn_rev_build(const KnnArgs& a, uint32_t* hist, uint32_t* scan_tmp, hipStream_t s) {
+  const RevPlan r = rev_plan(a.N, a.K, hist);
+  if (r.P > REV_MAXP || r.n >= a.pr.ov_cap + uint64_t(1) || r.n > uint64_t(a.N) * a.pr.cap)
+    return -1;  // beyond 2^26 ids or 2^32 edges: the sizes the state allocates cannot hold it
+  k_rev_hist<<<r.tiles, REV_THREADS, 0, s>>>(a, r);
+  excl_scan_u32(hist, r.P * r.tiles, scan_tmp, s);
+  k_rev_part<<<r.tiles, REV_THREADS, 0, s>>>(a, r);
+  k_rev_runs<<<r.P, REV_THREADS, 0, s>>>(a, r);
+  return 0;
 }
 void launch_knn_rev_reduce(const KnnArgs& a, hipStream_t s) { k_rev_reduce<<<a.N, 64, 0, s>>>(a); }
 void launch_knn_phase2(const KnnArgs& a, int precision, hipStream_t s) {

@@ -98,6 +98,13 @@ __global__ void k_scan3(uint32_t* off, const uint32_t* bsum, uint32_t N) {
 
 size_t scan_tmp_words(uint32_t N) { return (size_t(N) + SCAN_BLOCK - 1) / SCAN_BLOCK + 1; }
 
+void excl_scan_u32(uint32_t* v, uint32_t n, uint32_t* tmp, hipStream_t s) {
+  const uint32_t nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
+  k_scan1<<<nb, 256, 0, s>>>(v, 0, n, v, tmp);  // each thread reads its elements before writing
+  k_scan2<<<1, 256, 0, s>>>(tmp, nb, v, n);
+  k_scan3<<<nb, 256, 0, s>>>(v, tmp, n);
+}
+
 __global__ void k_ov_scatter(PushBuf pb, uint32_t* fill) {
   const uint32_t used = min(*pb.ov_used, pb.ov_cap);
   for (uint32_t o = blockIdx.x * blockDim.x + threadIdx.x; o < used; o += gridDim.x * blockDim.x) {
