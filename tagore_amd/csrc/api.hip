@@ -114,10 +114,12 @@ struct tg_state {
   bool prep8_valid = false;
   uint16_t* Xb = nullptr;  // BF16 copy of X (screen)
   float *nrm2 = nullptr, *nq = nullptr, *ne = nullptr;
-  uint32_t* cert_flags = nullptr;  // [4], see k_screen_prep
+  uint32_t* cert_flags = nullptr;  // [5], see k_screen_prep
   bool prep_valid = false;         // Xb / norms / certificates match the current X
   bool cert_bf16 = false;          // every value of X is exactly representable in BF16
   bool cert_int = false;           // integer data whose screen distances are exact
+  bool cert_u8 = false;            // and every value in [0, 255]: X8 holds the u8 copy
+  uint8_t* X8 = nullptr;
   uint32_t* fill = nullptr;
   uint32_t* scan_tmp = nullptr;
   unsigned long long* counters = nullptr;  // [CNT_SLOTS * CNT_N], summed over the slots
@@ -240,12 +242,12 @@ int ensure_prep(tg_state* s) {
   const size_t N = s->N;
   if (!s->nrm2) {
     if (dalloc(&s->nrm2, N) || dalloc(&s->nq, N) || dalloc(&s->ne, N) ||
-        dalloc(&s->cert_flags, 4) || dalloc(&s->Xb, N * s->D))
+        dalloc(&s->cert_flags, 5) || dalloc(&s->Xb, N * s->D))
       return TG_ERR_HIP;
   }
-  fill_u32(s->cert_flags, 0, 4, s->stream);
+  fill_u32(s->cert_flags, 0, 5, s->stream);
   launch_screen_prep(s->X, s->N, s->D, s->Xb, s->nrm2, s->nq, s->ne, s->cert_flags, s->stream);
-  uint32_t f[4];
+  uint32_t f[5];
   HIPX(hipMemcpyAsync(f, s->cert_flags, sizeof(f), hipMemcpyDeviceToHost, s->stream));
   HIPX(hipStreamSynchronize(s->stream));
   s->cert_bf16 = f[0] == 0;
@@ -254,6 +256,13 @@ int ensure_prep(tg_state* s) {
   // every MFMA partial sum, |a|^2+|b|^2 and |a-b|^2 stay integers below 2^24
   const double lim = 16777216.0;
   s->cert_int = f[1] == 0 && (f[2] == 0 ? 2.0 * qmax < lim : 4.0 * qmax < lim);
+  // non-negative integers up to 255 with every |a-b|^2 below 2^24: the exact u8 dot4
+  // distance equals the canonical FP32 fold
+  s->cert_u8 = s->cert_int && f[2] == 0 && f[4] == 0 && s->D % 4 == 0;
+  if (s->cert_u8) {
+    if (!s->X8 && dalloc(&s->X8, N * s->D)) return TG_ERR_HIP;
+    launch_make_u8(s->X, N * s->D, s->X8, s->stream);
+  }
   s->prep_valid = true;
   return TG_OK;
 }
@@ -455,6 +464,7 @@ int tg_free(tg_state* s) {
   dfree(s->nq);
   dfree(s->ne);
   dfree(s->cert_flags);
+  dfree(s->X8);
   dfree(s->fill);
   dfree(s->scan_tmp);
   dfree(s->counters);
@@ -1240,13 +1250,12 @@ int tg_prune(tg_state* s, const tg_prune_params* p) {
   a.phase_clock = s->profiling >= 2 ? 1 : 0;
   a.rows = s->X;
   a.rows_bf16 = 0;
-  if (p->precision == TG_PREC_BF16 || p->precision == TG_PREC_FP8) {
-    if (int rc = ensure_prep(s)) return rc;
-    if (p->precision == TG_PREC_BF16 && s->cert_bf16) {
-      a.rows = s->Xb;
-      a.rows_bf16 = 1;
-    }
+  if (int rc = ensure_prep(s)) return rc;
+  if (p->precision == TG_PREC_BF16 && s->cert_bf16) {
+    a.rows = s->Xb;
+    a.rows_bf16 = 1;
   }
+  a.rows8 = s->cert_u8 ? s->X8 : nullptr;
   // DPG's pair distances on an MFMA Gram (K <= 64) with exact re-scores near the threshold
   const bool dpg_screen = p->kind == TG_KIND_DPG && K <= 64 &&
                           ((p->precision == TG_PREC_BF16 && s->D % 8 == 0) ||

@@ -11,6 +11,7 @@
 //   area; every reduction sorts, so results do not depend on arrival order.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include <stdint.h>
 
 #define TG_WAVE 64
@@ -58,6 +59,7 @@ __device__ __forceinline__ unsigned long long* cnt_slot(unsigned long long* c) {
 template <int MC>
 struct Row16 {
   float4 v[MC];
+  int n2;  // u8 rows only: |x|^2 of the row, exact, in lane t == 0 of the group
 };
 
 // Four consecutive elements as FP32: rows are stored as FP32, or as BF16 bits when every
@@ -118,15 +120,42 @@ __device__ __forceinline__ float4 load4(const uint16_t* p) {
                      __uint_as_float(q.y << 16), __uint_as_float(q.y & 0xFFFF0000u));
 }
 
+// Reduction of an integer over the 16 lanes of a DPP row (valid in lane t == 0).
+__device__ __forceinline__ int row_sum16(int q) {
+  q += __builtin_amdgcn_update_dpp(0, q, 0x108, 0xF, 0xF, false);
+  q += __builtin_amdgcn_update_dpp(0, q, 0x104, 0xF, 0xF, false);
+  q += __builtin_amdgcn_update_dpp(0, q, 0x102, 0xF, 0xF, false);
+  return q + __builtin_amdgcn_update_dpp(0, q, 0x101, 0xF, 0xF, false);
+}
+
+// Four bytes m*64 + 4t .. +3 of a u8 row as one word, zero past D (D % 4 == 0).
+__device__ __forceinline__ uint32_t load_u8x4(const uint8_t* x, int m, int D, int t) {
+  const int i = m * 64 + 4 * t;
+  return i < D ? *reinterpret_cast<const uint32_t*>(x + i) : 0u;
+}
+
 template <int MC, class T>
 __device__ __forceinline__ void load_row16(Row16<MC>& r, const T* __restrict__ x, int D, int t) {
+  if constexpr (std::is_same<T, uint8_t>::value) {
+    // u8 rows: word m holds the lane's four bytes of chunk m (in v[m].x), n2 = |x|^2
+    uint32_t sq = 0;
 #pragma unroll
-  for (int m = 0; m < MC; ++m) {
-    const int i = m * 64 + 4 * t;
-    if (i < D)
-      r.v[m] = load4(x + i);
-    else
-      r.v[m] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int m = 0; m < MC; ++m) {
+      const uint32_t w = load_u8x4(x, m, D, t);
+      r.v[m] = make_float4(__uint_as_float(w), 0.f, 0.f, 0.f);
+      sq = __builtin_amdgcn_udot4(w, w, sq, false);
+    }
+    r.n2 = row_sum16(int(sq));
+  } else {
+#pragma unroll
+    for (int m = 0; m < MC; ++m) {
+      const int i = m * 64 + 4 * t;
+      if (i < D)
+        r.v[m] = load4(x + i);
+      else
+        r.v[m] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    r.n2 = 0;
   }
 }
 
@@ -214,6 +243,32 @@ __device__ __forceinline__ void dist16_batch(const Row16<MC>& a, const T* const 
   }
 #pragma unroll
   for (int k = 0; k < NB; ++k) out[k] = tree16(p[k][0], p[k][1], p[k][2], p[k][3]);
+}
+
+// NB distances from a u8 row a (registers, with |a|^2) to u8 rows b[0..NB): the integer
+// |a|^2 + |b|^2 - 2 a.b with v_dot4_u32_u8, exact.  Used only when every value of the data set
+// is an integer in [0, 255] and every squared distance is below 2^24: then every partial sum
+// of the canonical FP32 fold is an exact integer too, so both give the same float.  Every b[k]
+// must be a readable row; results valid in lane t == 0.
+template <int MC, int NB>
+__device__ __forceinline__ void dist16_batch_u8(const Row16<MC>& a, const uint8_t* const (&b)[NB],
+                                                int D, int t, float (&out)[NB]) {
+  uint32_t y[NB][MC];
+#pragma unroll
+  for (int k = 0; k < NB; ++k)
+#pragma unroll
+    for (int m = 0; m < MC; ++m) y[k][m] = m * 64 < D ? load_u8x4(b[k], m, D, t) : 0u;
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    uint32_t ab = 0, bb = 0;
+#pragma unroll
+    for (int m = 0; m < MC; ++m) {
+      if (m * 64 >= D) break;
+      ab = __builtin_amdgcn_udot4(__float_as_uint(a.v[m].x), y[k][m], ab, false);
+      bb = __builtin_amdgcn_udot4(y[k][m], y[k][m], bb, false);
+    }
+    out[k] = float(a.n2 + row_sum16(int(bb) - 2 * int(ab)));
+  }
 }
 
 // NB distances between the row pairs (a[k], b[k]), both read from memory; every pointer
@@ -676,7 +731,10 @@ __device__ __forceinline__ void eval_batch(const T* X, int D, const Row16<MC>& t
       rows[b] = q < n ? X + size_t(id_of(q)) * D : X;  // unused slots read row 0
     }
     float dd[4];
-    dist16_batch<MC, 4>(tgt, rows, D, t, dd);
+    if constexpr (std::is_same<T, uint8_t>::value)
+      dist16_batch_u8<MC, 4>(tgt, rows, D, t, dd);
+    else
+      dist16_batch<MC, 4>(tgt, rows, D, t, dd);
     float v = 0.f;
 #pragma unroll
     for (int b = 0; b < 4; ++b) {

@@ -38,6 +38,9 @@ struct PruneArgs {
   // data set is exactly representable in BF16 (identical FP32 values, half the bytes)
   const void* rows;
   int rows_bf16;
+  // u8 copy of X (or null) when every value is an integer in [0, 255] and every squared
+  // distance is below 2^24: the forward and reverse passes then take exact dot4 distances
+  const uint8_t* rows8;
   // segments: rows [seg_lo[c], seg_lo[c+1]) form independent data sets (null: one segment)
   const uint32_t* seg_of;  // [N] segment of each row
   const uint32_t* seg_lo;  // [nseg + 1]
@@ -114,11 +117,13 @@ struct KnnArgs {
   const uint32_t* perm;     // processing order of the gather-heavy kernels (null: 0..N-1)
 };
 
-// screen-mode preparation: BF16 copy (if Xb), norms, quantisation-error norms and the
-// data-set certificates in flags[4] (zeroed by the caller)
 // FP8 copy: q = e4m3(x / s), s = max|x| / 440, with |s q| and |x - s q| (rounded up)
 void launch_screen_prep8(const float* X, uint32_t N, uint32_t D, uint8_t* Xq, float* sc,
                          float* nq, float* ne, hipStream_t s);
+// u8 copy of the n values of X (n % 4 == 0; every value an integer in [0, 255])
+void launch_make_u8(const float* X, size_t n, uint8_t* X8, hipStream_t s);
+// screen-mode preparation: BF16 copy (if Xb), norms, quantisation-error norms and the
+// data-set certificates in flags[5] (zeroed by the caller)
 void launch_screen_prep(const float* X, uint32_t N, uint32_t D, uint16_t* Xb, float* nrm2,
                         float* nq, float* ne, uint32_t* flags, hipStream_t s);
 

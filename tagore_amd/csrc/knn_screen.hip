@@ -24,14 +24,15 @@ constexpr int HROW_S = 128;
 // Per-vector screen statistics (one wave per row, FP64 sums) and data-set certificates:
 // flags[0] |= some value is not exactly representable in BF16,
 // flags[1] |= some value is not an integer in [-256, 256], flags[2] |= some value is
-// negative, flags[3] = max |x|^2 (FP32 bits; exact for integer data below 2^24).
+// negative, flags[3] = max |x|^2 (FP32 bits; exact for integer data below 2^24),
+// flags[4] |= some value is above 255.
 __global__ void k_screen_prep(const float* X, uint32_t N, uint32_t D, uint16_t* Xb, float* nrm2,
                               float* nq, float* ne, uint32_t* flags) {
   const uint32_t row = blockIdx.x;
   if (row >= N) return;
   const float* x = X + size_t(row) * D;
   double s2 = 0.0, q2 = 0.0, e2 = 0.0;
-  uint32_t not_bf16 = 0, not_int = 0, neg = 0;
+  uint32_t not_bf16 = 0, not_int = 0, neg = 0, big = 0;
   for (uint32_t j = threadIdx.x; j < D; j += TG_WAVE) {
     const float v = x[j];
     const uint16_t b = f2bf(v);
@@ -40,6 +41,7 @@ __global__ void k_screen_prep(const float* X, uint32_t N, uint32_t D, uint16_t* 
     not_bf16 |= w != v;
     not_int |= !(rintf(v) == v && fabsf(v) <= 256.f);
     neg |= v < 0.f;
+    big |= v > 255.f;
     s2 += double(v) * double(v);
     q2 += double(w) * double(w);
     e2 += (double(w) - double(v)) * (double(w) - double(v));
@@ -49,7 +51,8 @@ __global__ void k_screen_prep(const float* X, uint32_t N, uint32_t D, uint16_t* 
     q2 += __shfl_xor(q2, o, TG_WAVE);
     e2 += __shfl_xor(e2, o, TG_WAVE);
   }
-  const uint64_t fb = __ballot(not_bf16 != 0), fi = __ballot(not_int != 0), fn = __ballot(neg != 0);
+  const uint64_t fb = __ballot(not_bf16 != 0), fi = __ballot(not_int != 0), fn = __ballot(neg != 0),
+                 fg = __ballot(big != 0);
   if (threadIdx.x == 0) {
     nrm2[row] = float(s2);
     // norms rounded up, so the bound stays an upper bound
@@ -59,7 +62,24 @@ __global__ void k_screen_prep(const float* X, uint32_t N, uint32_t D, uint16_t* 
     if (fi) atomicOr(&flags[1], 1u);
     if (fn) atomicOr(&flags[2], 1u);
     atomicMax(&flags[3], __float_as_uint(float(s2)));
+    if (fg) atomicOr(&flags[4], 1u);
   }
+}
+
+// u8 copy of X, four values per thread (every value an integer in [0, 255], D % 4 == 0)
+__global__ void k_make_u8(const float4* X, size_t n4, uint32_t* X8) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n4;
+       i += size_t(gridDim.x) * blockDim.x) {
+    const float4 v = X[i];
+    X8[i] = uint32_t(v.x) | uint32_t(v.y) << 8 | uint32_t(v.z) << 16 | uint32_t(v.w) << 24;
+  }
+}
+
+void launch_make_u8(const float* X, size_t n, uint8_t* X8, hipStream_t s) {
+  const size_t n4 = n / 4;
+  const uint32_t grid = uint32_t(std::min<size_t>((n4 + 255) / 256, 8192));
+  k_make_u8<<<grid, 256, 0, s>>>(reinterpret_cast<const float4*>(X), n4,
+                                 reinterpret_cast<uint32_t*>(X8));
 }
 
 // FP8 e4m3 copy with a per-vector scale: q = e4m3(x / s), s = max|x| / 440 (below the

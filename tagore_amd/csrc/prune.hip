@@ -737,8 +737,33 @@ static void launch_prune_t(const PruneArgs& a, int phase, const float* centroid,
   else launch_prune_mc<16, T>(a, phase, centroid, ep, s, nseg);
 }
 
+// The passes whose targets are rows of X, on the u8 copy.
+template <int MC>
+static void launch_prune_u8_mc(const PruneArgs& a, int phase, hipStream_t s) {
+  const dim3 grid(grid_rows(a.perm, a.N)), block(64);
+  const FwdLayout f = fwd_layout(a.K, a.TOPM, a.FD);
+  switch (phase) {
+    case PH_FWD_PATH:
+      if (f.W == 1) k_forward<MC, uint8_t, 0, 1><<<grid, block, f.bytes, s>>>(a);
+      else k_forward<MC, uint8_t, 0, 2><<<grid, block, f.bytes, s>>>(a);
+      break;
+    case PH_FWD_2HOP: k_forward<MC, uint8_t, 1, 1><<<grid, block, f.bytes, s>>>(a); break;
+    case PH_REV: k_reverse<MC, uint8_t><<<grid, block, 0, s>>>(a); break;
+    default: break;
+  }
+}
+
 void launch_prune(const PruneArgs& a, int phase, const float* centroid, uint32_t* ep,
                   hipStream_t s, uint32_t nseg) {
+  // (rows up to 256 values: wider u8 batches would not stay in registers)
+  if (a.rows8 && a.D <= 256 && (phase == PH_FWD_PATH || phase == PH_FWD_2HOP || phase == PH_REV)) {
+    PruneArgs b = a;
+    b.rows = a.rows8;
+    b.rows_bf16 = 0;
+    if (a.D <= 128) launch_prune_u8_mc<2>(b, phase, s);
+    else launch_prune_u8_mc<4>(b, phase, s);
+    return;
+  }
   if (a.rows_bf16) launch_prune_t<uint16_t>(a, phase, centroid, ep, s, nseg);
   else launch_prune_t<float>(a, phase, centroid, ep, s, nseg);
 }

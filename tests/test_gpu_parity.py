@@ -107,10 +107,11 @@ def test_prune_parity_unit(unit_case, name, kind, fr, rr, th):
     ("DPG", 3, 0, 2, 0.0),
 ])
 def test_prune_parity_bf16_rows(int_case, name, kind, fr, rr, th, prec):
-    """BF16 rows are read only for data sets that BF16 holds exactly: integer data in
-    [0,255] and BF16-rounded unit vectors take the BF16 path, plain unit vectors fall back
-    to FP32 rows.  DPG screens its pair distances on an MFMA Gram (BF16 or FP8).  All must
-    equal the oracle bit for bit."""
+    """BF16 rows are read only for data sets that BF16 holds exactly: BF16-rounded unit
+    vectors take the BF16 path, integer data in [0,255] the u8 copy in the forward and
+    reverse passes (BF16 in the others), plain unit vectors fall back to FP32 rows.  DPG
+    screens its pair distances on an MFMA Gram (BF16 or FP8).  All must equal the oracle
+    bit for bit."""
     X, G, GD = int_case
     _compare(X, G, GD, kind, fr, rr, 32, 64, th, prec)
     U = gmm_unit(2000, 96, 16, 303)
@@ -120,12 +121,13 @@ def test_prune_parity_bf16_rows(int_case, name, kind, fr, rr, th, prec):
         _compare(Y, GY, GDY, kind, fr, rr, 32, 64, th, prec)
 
 
-@pytest.mark.parametrize("D,lo,hi", [(128, 0, 255), (96, 0, 255), (200, -60, 60), (64, -256, 256)])
+@pytest.mark.parametrize("D,lo,hi", [(128, 0, 255), (96, 0, 255), (200, 0, 200), (200, -60, 60),
+                                    (64, -256, 256)])
 def test_prune_parity_integer_bf16_rows(D, lo, hi):
-    """Integer data under precision "bf16" (the rows are read from the BF16 copy, which holds
-    these values exactly): unsigned and signed values, D not a multiple of 32 or 64, values
-    at +-256.  Every rule, and a beam of two register slots (TOPM > 64), must equal the
-    oracle bit for bit."""
+    """Integer data under precision "bf16": values in [0, 255] take the u8 copy (exact dot4
+    distances, one or two 64-byte chunks per row), signed values the BF16 copy, which holds
+    them exactly; D not a multiple of 32 or 64, values at +-256.  Every rule, and a beam of
+    two register slots (TOPM > 64), must equal the oracle bit for bit."""
     rng = np.random.default_rng(D + hi)
     C = rng.integers(lo, hi + 1, (24, D))
     X = np.clip(C[rng.integers(0, 24, 2500)] + rng.integers(-12, 13, (2500, D)), lo, hi)
