@@ -136,63 +136,20 @@ __device__ __forceinline__ float screen_bound(float na, float qa, float ea, floa
   return 2.0f * (ea * qb + qa * eb + ea * eb) + eta * (na + nb + 2.0f * qa * qb);
 }
 
-// -------------------------------------------------------------------------------------
-// Screened local join.  PREC 1: FP32 MFMA (16x16x4), PREC 2: BF16 MFMA (16x16x32),
-// PREC 3: FP8 e4m3 MFMA (16x16x32) with per-vector scales applied to the product.
-// 256 threads: wave w owns row block w (16 rows of A) against the 4 column blocks of B and
-// of A.  Staging chunk: DC elements per row (128 BF16 or 64 FP32 = 256 B).
-// -------------------------------------------------------------------------------------
-template <int PREC, int MC, class T, int DC>
-__global__ __launch_bounds__(256) void k_localjoin_screen(KnnArgs a) {
-  // DC: elements per staged chunk (the tile's k depth per LDS round)
+// Products of the screen copies of the node's sample rows, staged through LDS DC elements
+// at a time: wave w gets accB[c] = A rows 16w.. x B rows 16c.. and accA[c] = A rows 16w.. x
+// A rows 16c.. (mfma layout: lane l, register r holds row 4(l/16)+r, column l%16 of the
+// 16x16 block).  Rows past nA / nB are zero.
+template <int PREC, int DC>
+__device__ __forceinline__ void screen_products(const KnnArgs& a, const uint32_t* A,
+                                                const uint32_t* B, int nA, int nB,
+                                                unsigned char* sA, unsigned char* sB,
+                                                f32x4 (&accB)[4], f32x4 (&accA)[4]) {
   constexpr int ESZ = PREC == 3 ? 1 : (PREC == 2 ? 2 : 4);  // element bytes
   constexpr int CPR = DC * ESZ / 16;                // 16-byte loads per staged row
   constexpr int RS = DC * ESZ + 16;                 // padded row stride in bytes
-  constexpr int STAGE = 64 * RS;                    // one operand
-  constexpr int MROW = 65;                          // padded distance-matrix row (floats)
-  constexpr int SMEM = 2 * STAGE > 2 * 64 * MROW * 4 ? 2 * STAGE : 2 * 64 * MROW * 4;
-  static_assert(DC % (PREC >= 2 ? 32 : 4) == 0 && CPR >= 1, "tile depth");
-  // the distance matrices alias the staging area once the products are done
-  __shared__ __attribute__((aligned(16))) unsigned char smem[SMEM];
-  __shared__ uint32_t A[64], B[64];
-  __shared__ float naA[64], qA[64], eA[64], wA[64];
-  __shared__ float naB[64], qB[64], eB[64], wB[64];
-  __shared__ float scA[64], scB[64];
-  unsigned char* sA = smem;
-  unsigned char* sB = smem + STAGE;
-  float* MAB = reinterpret_cast<float*>(smem);
-  float* MAA = MAB + 64 * MROW;
-
-  const uint32_t u = row_of_block(a.perm, a.N);
-  if (u >= a.N) return;
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
-  const int K = a.K, D = a.D;
-  const int nA = a.na[u], nB = a.nb[u];
-  if (tid < 64) {
-    if (tid < nA) {
-      const uint32_t v = a.H[size_t(u) * HROW_S + tid];
-      A[tid] = v;
-      naA[tid] = a.nrm2[v];
-      qA[tid] = a.nq[v];
-      eA[tid] = a.ne[v];
-      wA[tid] = a.GD[size_t(v) * K + K - 1];
-      if (PREC == 3) scA[tid] = a.sc8[v];
-    }
-  } else if (tid < 128) {
-    const int j = tid - 64;
-    if (j < nB) {
-      const uint32_t v = a.H[size_t(u) * HROW_S + 64 + j];
-      B[j] = v;
-      naB[j] = a.nrm2[v];
-      qB[j] = a.nq[v];
-      eB[j] = a.ne[v];
-      wB[j] = a.GD[size_t(v) * K + K - 1];
-      if (PREC == 3) scB[j] = a.sc8[v];
-    }
-  }
-  __syncthreads();
-
-  f32x4 accB[4], accA[4];
+  const int D = a.D;
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     accB[c] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -257,6 +214,66 @@ __global__ __launch_bounds__(256) void k_localjoin_screen(KnnArgs a) {
     }
     __syncthreads();
   }
+}
+
+// -------------------------------------------------------------------------------------
+// Screened local join.  PREC 1: FP32 MFMA (16x16x4), PREC 2: BF16 MFMA (16x16x32),
+// PREC 3: FP8 e4m3 MFMA (16x16x32) with per-vector scales applied to the product.
+// 256 threads: wave w owns row block w (16 rows of A) against the 4 column blocks of B and
+// of A.  Staging chunk: DC elements per row (128 BF16 or 64 FP32 = 256 B).
+// -------------------------------------------------------------------------------------
+template <int PREC, int MC, class T, int DC>
+__global__ __launch_bounds__(256) void k_localjoin_screen(KnnArgs a) {
+  // DC: elements per staged chunk (the tile's k depth per LDS round)
+  constexpr int ESZ = PREC == 3 ? 1 : (PREC == 2 ? 2 : 4);  // element bytes
+  constexpr int CPR = DC * ESZ / 16;                // 16-byte loads per staged row
+  constexpr int RS = DC * ESZ + 16;                 // padded row stride in bytes
+  constexpr int STAGE = 64 * RS;                    // one operand
+  constexpr int MROW = 65;                          // padded distance-matrix row (floats)
+  constexpr int SMEM = 2 * STAGE > 2 * 64 * MROW * 4 ? 2 * STAGE : 2 * 64 * MROW * 4;
+  static_assert(DC % (PREC >= 2 ? 32 : 4) == 0 && CPR >= 1, "tile depth");
+  // the distance matrices alias the staging area once the products are done
+  __shared__ __attribute__((aligned(16))) unsigned char smem[SMEM];
+  __shared__ uint32_t A[64], B[64];
+  __shared__ float naA[64], qA[64], eA[64], wA[64];
+  __shared__ float naB[64], qB[64], eB[64], wB[64];
+  __shared__ float scA[64], scB[64];
+  unsigned char* sA = smem;
+  unsigned char* sB = smem + STAGE;
+  float* MAB = reinterpret_cast<float*>(smem);
+  float* MAA = MAB + 64 * MROW;
+
+  const uint32_t u = row_of_block(a.perm, a.N);
+  if (u >= a.N) return;
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+  const int K = a.K, D = a.D;
+  const int nA = a.na[u], nB = a.nb[u];
+  if (tid < 64) {
+    if (tid < nA) {
+      const uint32_t v = a.H[size_t(u) * HROW_S + tid];
+      A[tid] = v;
+      naA[tid] = a.nrm2[v];
+      qA[tid] = a.nq[v];
+      eA[tid] = a.ne[v];
+      wA[tid] = a.GD[size_t(v) * K + K - 1];
+      if (PREC == 3) scA[tid] = a.sc8[v];
+    }
+  } else if (tid < 128) {
+    const int j = tid - 64;
+    if (j < nB) {
+      const uint32_t v = a.H[size_t(u) * HROW_S + 64 + j];
+      B[j] = v;
+      naB[j] = a.nrm2[v];
+      qB[j] = a.nq[v];
+      eB[j] = a.ne[v];
+      wB[j] = a.GD[size_t(v) * K + K - 1];
+      if (PREC == 3) scB[j] = a.sc8[v];
+    }
+  }
+  __syncthreads();
+
+  f32x4 accB[4], accA[4];
+  screen_products<PREC, DC>(a, A, B, nA, nB, sA, sB, accB, accA);
   // screen distances into the (aliased) matrices: C[i][j], i = 4*(l/16)+r, j = l%16
   const float INF = __uint_as_float(0x7F800000u);
 #pragma unroll
@@ -423,10 +440,119 @@ __global__ __launch_bounds__(256) void k_localjoin_screen(KnnArgs a) {
   }
 }
 
+// Local join when the screen distances are the canonical distances (integer data, FP32 or
+// BF16 products): each job's answer is the (s, id) minimum of its 16-wide tile, taken from
+// the accumulators by cross-lane minima instead of distance matrices in LDS, so the kernel
+// holds only the staging area (and runs more workgroups per CU).  Jobs, as in the screened
+// kernel: A rows x B tiles, B columns x A tiles, A rows x A tiles.  Lane t of group g owns
+// row job (row 16w + 4g + (t & 3), tile t >> 2) of both row kinds, and lane l owns column
+// job (column l, tile w).
+// (d, id) minimum over the lanes l ^ x for the masks x in [x0, x1) (powers of two): the
+// smallest d, then the smallest id among the lanes holding it (distances are >= 0 or +inf,
+// so this is the minimum of the keys mk_key(d, id)).
+template <int X0, int X1>
+__device__ __forceinline__ uint64_t key_min_lanes(float d, uint32_t id) {
+  float m = d;
+#pragma unroll
+  for (int x = X0; x < X1; x <<= 1) m = fminf(m, __uint_as_float(lane_xor32(__float_as_uint(m), x)));
+  uint32_t k = d == m ? id : 0xFFFFFFFFu;
+#pragma unroll
+  for (int x = X0; x < X1; x <<= 1) k = min(k, lane_xor32(k, x));
+  return mk_key(m, k);
+}
+
+template <int PREC, int DC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+void k_localjoin_exact_screen(KnnArgs a) {
+  constexpr int ESZ = PREC == 2 ? 2 : 4;
+  constexpr int STAGE = 64 * (DC * ESZ + 16);
+  static_assert(PREC == 1 || PREC == 2, "exact screens are FP32 or BF16 products");
+  __shared__ __attribute__((aligned(16))) unsigned char smem[2 * STAGE];
+  __shared__ uint32_t A[64], B[64];
+  __shared__ float naA[64], wA[64], naB[64], wB[64];
+  const uint32_t u = row_of_block(a.perm, a.N);
+  if (u >= a.N) return;
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, g = l >> 4, t = l & 15;
+  const int K = a.K;
+  const int nA = a.na[u], nB = a.nb[u];
+  const float INF = __uint_as_float(0x7F800000u);
+  if (tid < 64) {
+    const bool in = tid < nA;
+    const uint32_t v = in ? a.H[size_t(u) * HROW_S + tid] : 0u;
+    A[tid] = v;
+    naA[tid] = in ? a.nrm2[v] : 0.f;
+    wA[tid] = in ? a.GD[size_t(v) * K + K - 1] : 0.f;
+  } else if (tid < 128) {
+    const int j = tid - 64;
+    const bool in = j < nB;
+    const uint32_t v = in ? a.H[size_t(u) * HROW_S + 64 + j] : 0u;
+    B[j] = v;
+    naB[j] = in ? a.nrm2[v] : 0.f;
+    wB[j] = in ? a.GD[size_t(v) * K + K - 1] : 0.f;
+  }
+  __syncthreads();
+  f32x4 accB[4], accA[4];
+  screen_products<PREC, DC>(a, A, B, nA, nB, smem, smem + STAGE, accB, accA);
+  uint64_t row_b = TG_KEY_INF, row_a = TG_KEY_INF, col_b = TG_KEY_INF;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int j = 16 * c + t;
+    float cd = INF;  // column job: this lane's 4 rows, then across the groups
+    uint32_t cid = 0xFFFFFFFFu;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = 16 * w + 4 * g + r;
+      const float sb = (i < nA && j < nB && A[i] != B[j]) ? (naA[i] + naB[j]) - 2.0f * accB[c][r] : INF;
+      const float sa = (i < nA && j < nA && i != j) ? (naA[i] + naA[j]) - 2.0f * accA[c][r] : INF;
+      const uint64_t kb = key_min_lanes<1, 16>(sb, B[j]);
+      const uint64_t ka = key_min_lanes<1, 16>(sa, A[j]);
+      if (t == 4 * c + r) {
+        row_b = kb;
+        row_a = ka;
+      }
+      if (sb < cd || (sb == cd && A[i] < cid)) {
+        cd = sb;
+        cid = A[i];
+      }
+    }
+    const uint64_t kc = key_min_lanes<16, 64>(cd, cid);
+    if (g == c) col_b = kc;
+  }
+  // pushes: a job's minimum enters when it beats the destination's worst entry and is not
+  // already listed (the merge would drop it)
+  {
+    const int i = 16 * w + 4 * g + (t & 3);
+    if (i < nA) {
+      const uint32_t dest = A[i];
+      const float worst = wA[i];
+      if (key_d(row_b) < worst && !row_has_key(a.G, a.GD, K, dest, row_b)) pb_push(a.pc, dest, row_b);
+      if (key_d(row_a) < worst && !row_has_key(a.G, a.GD, K, dest, row_a)) pb_push(a.pc, dest, row_a);
+    }
+    if (l < nB) {
+      const uint32_t dest = B[l];
+      if (key_d(col_b) < wB[l] && !row_has_key(a.G, a.GD, K, dest, col_b)) pb_push(a.pc, dest, col_b);
+    }
+  }
+  if (a.counters && tid == 0)
+    atomicAdd(&cnt_slot(a.counters)[0], (unsigned long long)(nA * nB + nA * (nA - 1) / 2));
+}
+
 // tile 0: 256-byte staged rows (256 FP8 / 128 BF16 / 64 FP32 elements per round); tile 1:
 // 64-byte rows (64 / 32 / 16 elements per round): less zero padding at D not a multiple of
 // the depth, more LDS rounds
 void launch_knn_localjoin_screen(const KnnArgs& a, int precision, int tile, hipStream_t s) {
+  if (a.exact_screen && (precision == KP_BF16 || precision == KP_FP32)) {
+    // 256-byte staged rows (128 BF16 / 64 FP32 elements), 64-byte rows with tile 1
+    const dim3 grid(grid_rows(a.perm, a.N));
+    if (precision == KP_BF16) {
+      if (tile == 1) k_localjoin_exact_screen<2, 32><<<grid, 256, 0, s>>>(a);
+      else k_localjoin_exact_screen<2, 128><<<grid, 256, 0, s>>>(a);
+    } else {
+      if (tile == 1) k_localjoin_exact_screen<1, 16><<<grid, 256, 0, s>>>(a);
+      else k_localjoin_exact_screen<1, 64><<<grid, 256, 0, s>>>(a);
+    }
+    return;
+  }
   const int mc = (int(a.D) + 63) / 64;
 #define LJS(P, M, T, DC) \
   k_localjoin_screen<P, M, T, DC><<<grid_rows(a.perm, a.N), 256, 0, s>>>(a)
