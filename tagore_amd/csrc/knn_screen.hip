@@ -318,14 +318,13 @@ __global__ __launch_bounds__(256) void k_localjoin_screen(KnnArgs a) {
   auto divA = [&](int x) { return int((uint32_t(x) * mA) >> 16); };
   const int nj0 = nA * tB, nj1 = nB * tA, njobs = nj0 + nj1 + nA * tA;
   uint32_t n_exact = 0;
-  // exact screens need no queue: every thread of the workgroup takes a job; otherwise the
-  // band masks of QJ jobs at a time are formed by wave 0
-  const int JSTEP = a.exact_screen ? 256 : QJ;
-  for (int base = 0; base < njobs; base += JSTEP) {
+  // the band masks of QJ jobs at a time are formed by wave 0 (exact screens take
+  // k_localjoin_exact_screen)
+  for (int base = 0; base < njobs; base += QJ) {
     int kind = 0, r = 0, lo = 0;
     uint32_t mask = 0;
     const int J = base + tid;
-    if (tid < JSTEP && J < njobs) {
+    if (tid < QJ && J < njobs) {
       if (J < nj0) {
         kind = 0; r = divB(J); lo = (J - r * tB) * LJ_TILE;
       } else if (J < nj0 + nj1) {
@@ -355,23 +354,10 @@ __global__ __launch_bounds__(256) void k_localjoin_screen(KnnArgs a) {
           }
         }
       }
-      if (a.exact_screen) {
-        // the screen distances are the canonical distances: the job's answer is the
-        // (s, id) minimum of the tile, no bound and no re-score
-        uint64_t best = TG_KEY_INF;
 #pragma unroll
-        for (int q = 0; q < LJ_TILE; ++q)
-          if (sv[q] < INF) best = min(best, mk_key(sv[q], cb ? B[lo + q] : A[lo + q]));
-        const uint32_t dest = col ? B[r] : A[r];
-        if (key_d(best) < worst && !row_has_key(a.G, a.GD, K, dest, best))
-          pb_push(a.pc, dest, best);
-      } else {
-#pragma unroll
-        for (int q = 0; q < LJ_TILE; ++q)
-          if (sv[q] < INF && !(sv[q] - ev[q] > tau)) mask |= 1u << q;
-      }
+      for (int q = 0; q < LJ_TILE; ++q)
+        if (sv[q] < INF && !(sv[q] - ev[q] > tau)) mask |= 1u << q;
     }
-    if (a.exact_screen) continue;  // uniform across the workgroup
     // ---- queue offsets (wave 0 scan)
     if (tid < QJ) {
       const int c = __popc(mask);
