@@ -786,13 +786,22 @@ __global__ __launch_bounds__(64) void k_phase2(KnnArgs a) {
                  });
   if (a.counters && lane == 0 && nre) atomicAdd(&cnt_slot(a.counters)[1], (unsigned long long)nre);
   __syncthreads();
-  wave_sort<512>(kv, nk);
-  int nc = 0;
+  // the keys that beat the worst entry (usually few) compacted to the front, in place (a
+  // chunk is read before any of its positions is written), then sorted
+  int nv = 0;
   for (int base = 0; base < nk; base += TG_WAVE) {
     const int i = base + lane;
-    nc += __popcll(ballot(i < nk && kv[i] != TG_KEY_INF));
+    const uint64_t k = i < nk ? kv[i] : TG_KEY_INF;
+    const bool ok = k != TG_KEY_INF;
+    const uint64_t m = ballot(ok);
+    if (ok) kv[nv + mask_rank(m)] = k;
+    nv += int(__popcll(m));
   }
-  nc = min(nc, K);
+  const int ns = pow2_at_least(max(nv, 1));
+  for (int i = nv + lane; i < ns; i += TG_WAVE) kv[i] = TG_KEY_INF;
+  __syncthreads();
+  wave_sort<512>(kv, ns);
+  const int nc = min(nv, K);
   merge_row(K, row.gk, row.vf, kv, nc, a.G1 + size_t(u) * K, a.GD1 + size_t(u) * K,
             a.V + size_t(u) * K);
   // reverse pushes for entries that entered the list
