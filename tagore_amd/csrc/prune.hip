@@ -34,29 +34,34 @@ constexpr int POOL_MAX = 1024;  // K + 480 <= 608, padded to a power of two
 constexpr int EXPL_MAX = 32;
 
 // -------------------------------------------------------------------------------------
-// Id set in LDS: open addressing over 512 slots (<= 256 members), INV marks an empty slot.
+// Id set in LDS: open addressing over 2^bits slots, INV marks an empty slot.  Callers size
+// the table above the most members it can hold, so a probe always ends.
 // -------------------------------------------------------------------------------------
-constexpr int HSET = 512;
-__device__ __forceinline__ uint32_t hslot(uint32_t id) { return (id * 0x9E3779B1u) >> 23; }
-__device__ void hset_clear(uint32_t* tab) {
-  for (int i = threadIdx.x; i < HSET; i += TG_WAVE) tab[i] = TG_INV;
+constexpr int HSET = 512;  // k_cal_ep: up to 256 members
+__device__ __forceinline__ uint32_t hslot(uint32_t id, int bits) {
+  return (id * 0x9E3779B1u) >> (32 - bits);
 }
-__device__ __forceinline__ void hset_insert(uint32_t* tab, uint32_t id) {
+__device__ void hset_clear(uint32_t* tab, int bits) {
+  for (int i = threadIdx.x; i < (1 << bits); i += TG_WAVE) tab[i] = TG_INV;
+}
+__device__ __forceinline__ void hset_insert(uint32_t* tab, uint32_t id, int bits) {
   if (id == TG_INV) return;
-  uint32_t h = hslot(id);
+  const uint32_t mask = (1u << bits) - 1;
+  uint32_t h = hslot(id, bits);
   for (;;) {
     const uint32_t old = atomicCAS(&tab[h], TG_INV, id);
     if (old == TG_INV || old == id) return;
-    h = (h + 1) & (HSET - 1);
+    h = (h + 1) & mask;
   }
 }
-__device__ __forceinline__ bool hset_has(const uint32_t* tab, uint32_t id) {
-  uint32_t h = hslot(id);
+__device__ __forceinline__ bool hset_has(const uint32_t* tab, uint32_t id, int bits) {
+  const uint32_t mask = (1u << bits) - 1;
+  uint32_t h = hslot(id, bits);
   for (;;) {
     const uint32_t v = tab[h];
     if (v == id) return true;
     if (v == TG_INV) return false;
-    h = (h + 1) & (HSET - 1);
+    h = (h + 1) & mask;
   }
 }
 
@@ -76,7 +81,7 @@ __device__ __forceinline__ bool hset_has(const uint32_t* tab, uint32_t id) {
 template <int MC, class T, int W>
 __device__ __forceinline__ void beam_search(const PruneArgs& a, const Row16<MC>& tgt,
                                             uint32_t start,
-                            int max_exp, uint64_t* S, uint32_t* tab, uint32_t* expl,
+                            int max_exp, uint64_t* S, uint32_t* tab, int hbits, uint32_t* expl,
                             uint16_t* cl, uint64_t* ring, uint64_t* out, uint32_t& ring_c,
                             uint32_t& n_eval, uint32_t& n_adj, PhaseAcc& ph) {
   const int lane = threadIdx.x;
@@ -97,13 +102,13 @@ __device__ __forceinline__ void beam_search(const PruneArgs& a, const Row16<MC>&
   n_eval += K;
   n_adj += 1;
   ring_c = K;
-  hset_clear(tab);
+  hset_clear(tab, hbits);
   __syncthreads();
   uint64_t v[W], bk[W];
 #pragma unroll
   for (int r = 0; r < W; ++r) {
     v[r] = S[r * 64 + lane];
-    hset_insert(tab, key_id(v[r]));  // the first dedup set is ids(S); padding ids are INV
+    hset_insert(tab, key_id(v[r]), hbits);  // the first dedup set is ids(S); padding ids are INV
   }
   reg_bitonic<W>(v);
 #pragma unroll
@@ -144,7 +149,7 @@ __device__ __forceinline__ void beam_search(const PruneArgs& a, const Row16<MC>&
       bool fresh = false;
       if (j < K) {
         const uint32_t y = a.G[size_t(x) * K + j];
-        fresh = !hset_has(tab, y);
+        fresh = !hset_has(tab, y, hbits);
         S[j] = INFKEY | y;
       } else {
         S[j] = TG_KEY_INF;
@@ -172,12 +177,12 @@ __device__ __forceinline__ void beam_search(const PruneArgs& a, const Row16<MC>&
 #pragma unroll
     for (int q = 0; q < W; ++q) v[q] = S[q * 64 + lane];
     // next dedup set: ids of this round's beam and candidates
-    hset_clear(tab);
+    hset_clear(tab, hbits);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < W; ++q) {
-      hset_insert(tab, key_id(bk[q]));
-      hset_insert(tab, key_id(v[q]));
+      hset_insert(tab, key_id(bk[q]), hbits);
+      hset_insert(tab, key_id(v[q]), hbits);
     }
     const long long s2 = pclock(a);
     reg_bitonic<W>(v);
@@ -353,15 +358,18 @@ __device__ __forceinline__ void count_add(unsigned long long* c, int idx, uint32
 constexpr int RING_P = 512;
 
 struct FwdLayout {
-  int W, NS, L, offB, bytes;
+  int W, NS, L, HB, offB, bytes;
 };
 __host__ __device__ inline FwdLayout fwd_layout(int K, int TOPM, int FD) {
   FwdLayout l;
   l.W = (K <= 64 && TOPM <= 64) ? 1 : 2;
   l.NS = 64 * l.W;
   l.L = K + RING;
+  // the beam's id set holds at most TOPM + K ids: 256 slots up to 192 of them (three
+  // quarters full at most), else 512
+  l.HB = TOPM + K <= 192 ? 8 : 9;
   l.offB = ((l.L > RING_P ? l.L : RING_P) * 8 + 15) / 16 * 16;
-  const int beam = l.NS * 8 + HSET * 4 + EXPL_MAX * 4 + l.NS * 2;
+  const int beam = l.NS * 8 + (4 << l.HB) + EXPL_MAX * 4 + l.NS * 2;
   const int filt = (FD + 1) * 8 + l.L * 2;
   l.bytes = l.offB + ((beam > filt ? beam : filt) + 15) / 16 * 16;
   return l;
@@ -379,7 +387,7 @@ __global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
   uint64_t* P = R;  // after the pool is formed
   uint64_t* S = reinterpret_cast<uint64_t*>(dsm + ly.offB);
   uint32_t* tab = reinterpret_cast<uint32_t*>(S + ly.NS);
-  uint32_t* expl = tab + HSET;
+  uint32_t* expl = tab + (1 << ly.HB);
   uint16_t* clb = reinterpret_cast<uint16_t*>(expl + EXPL_MAX);
   uint64_t* sel = S;  // after the beam
   uint16_t* cl = reinterpret_cast<uint16_t*>(sel + a.FD + 1);
@@ -395,8 +403,8 @@ __global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
   if (KIND == 0) {
     uint32_t ring_c = 0;
     const uint32_t ep = a.ep[a.seg_of ? a.seg_of[u] : 0];
-    beam_search<MC, T, W>(a, xu, ep, 10, S, tab, expl, clb, R, nullptr, ring_c, n_col, n_adj,
-                          ph);
+    beam_search<MC, T, W>(a, xu, ep, 10, S, tab, ly.HB, expl, clb, R, nullptr, ring_c, n_col,
+                          n_adj, ph);
   } else {
     // two-hop ids y_i = G[G[u][i/K]][i%K], i < 480
     for (int i = lane; i < RING; i += TG_WAVE) {
@@ -679,9 +687,9 @@ __global__ __launch_bounds__(64) void k_cal_ep(PruneArgs a, const float* centroi
   uint32_t rc = 0, ne = 0, na = 0;
   PhaseAcc ph;  // not reported for the entry point
   if (a.K <= 64 && a.TOPM <= 64)
-    beam_search<MC, T, 1>(a, c, start, 30, S, tab, expl, cl, nullptr, M0, rc, ne, na, ph);
+    beam_search<MC, T, 1>(a, c, start, 30, S, tab, 9, expl, cl, nullptr, M0, rc, ne, na, ph);
   else
-    beam_search<MC, T, 2>(a, c, start, 30, S, tab, expl, cl, nullptr, M0, rc, ne, na, ph);
+    beam_search<MC, T, 2>(a, c, start, 30, S, tab, 9, expl, cl, nullptr, M0, rc, ne, na, ph);
   if (threadIdx.x == 0) ep[seg] = key_id(M0[0]);
   count_add(a.counters, 4, ne);
 }
