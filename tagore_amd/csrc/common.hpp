@@ -629,24 +629,28 @@ __device__ int pb_top(const PushBuf& pb, uint32_t v, int R, bool uniq, Pred pred
                       const uint64_t* seed = nullptr, int nseed = 0) {
   const int lane = threadIdx.x;
   const uint32_t n = pb_count(pb, v);
-  if (nseed == 0 && n <= 128) {
-    // one chunk and no seed (most nodes): sort only the next power of two above n, then
-    // keep the first R (unique if requested) -- what the general path selects
-    const int m = pow2_at_least(max(int(n), 1));
+  if (uint32_t(nseed) + n <= 256) {
+    // seed and pushes fit buf together (most nodes): one sort of the next power of two
+    // above their count, then the first R (unique if requested) -- what the chunked path
+    // selects
+    const int tot = nseed + int(n);
+    const int m = pow2_at_least(max(tot, 1));
     for (int i = lane; i < m; i += TG_WAVE) {
       uint64_t k = TG_KEY_INF;
-      if (i < int(n)) {
-        k = pb_get(pb, v, uint32_t(i));
+      if (i < nseed) {
+        k = seed[i];
+      } else if (i < tot) {
+        k = pb_get(pb, v, uint32_t(i - nseed));
         if (!pred(k)) k = TG_KEY_INF;
       }
       buf[i] = k;
     }
     __syncthreads();
-    wave_sort<128>(buf, m);
-    uint64_t kk[2];
-    bool kp[2];
+    wave_sort<256>(buf, m);
+    uint64_t kk[4];
+    bool kp[4];
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {  // every read before any write
+    for (int r = 0; r < 4; ++r) {  // every read before any write
       const int i = r * TG_WAVE + lane;
       kk[r] = i < m ? buf[i] : TG_KEY_INF;
       kp[r] = kk[r] != TG_KEY_INF && (!uniq || i == 0 || buf[i - 1] != kk[r]);
@@ -654,7 +658,7 @@ __device__ int pb_top(const PushBuf& pb, uint32_t v, int R, bool uniq, Pred pred
     __syncthreads();
     int pos = 0;
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
+    for (int r = 0; r < 4; ++r) {
       const uint64_t bm = ballot(kp[r]);
       const int p = pos + mask_rank(bm);
       if (kp[r] && p < R) buf[p] = kk[r];
