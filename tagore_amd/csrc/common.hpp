@@ -271,6 +271,17 @@ __device__ __forceinline__ void dist16_batch_u8(const Row16<MC>& a, const uint8_
   }
 }
 
+// Distance between two u8 rows held in registers (exact, as dist16_batch_u8), in every lane.
+template <int MC>
+__device__ __forceinline__ float dist_rows_u8(const Row16<MC>& a, const Row16<MC>& b, int D) {
+  uint32_t ab = 0;
+#pragma unroll
+  for (int m = 0; m < MC; ++m)
+    if (m * 64 < D) ab = __builtin_amdgcn_udot4(__float_as_uint(a.v[m].x), __float_as_uint(b.v[m].x), ab, false);
+  const float d = float(a.n2 + b.n2 - 2 * row_sum16(int(ab)));  // valid in lane t == 0
+  return __shfl(d, 0, TG_WAVE);
+}
+
 // NB distances between the row pairs (a[k], b[k]), both read from memory; every pointer
 // must be a readable row, as for dist16_batch.
 template <int MC, int NB, class T>
@@ -759,6 +770,50 @@ __device__ void merge_lds(const uint64_t* A, int na, const uint64_t* B, int nb, 
 // register row `tgt`, four candidates per 16-lane group at a time (item q0 + g + 4b in group
 // g).  The 16 results are then moved to lanes 0..15 (lane j holds item q0 + j) and emit(q, d)
 // runs once in those 16 lanes instead of four times in one lane per group.
+// Two u8 targets at once: each candidate row is read once and gives d(ta, c) and d(tb, c);
+// emit(q, da, db) runs in lanes 0..15 for the items q0 + lane, as in eval_batch.
+template <int MC, class IdOf, class Emit>
+__device__ __forceinline__ void eval_batch2_u8(const uint8_t* X, int D, const Row16<MC>& ta,
+                                               const Row16<MC>& tb, int n, IdOf id_of,
+                                               Emit emit) {
+  const int lane = threadIdx.x, g = lane >> 4, t = lane & 15;
+  const int src = 16 * (lane & 3), mine = (lane >> 2) & 3;
+  for (int q0 = 0; q0 < n; q0 += 16) {
+    uint32_t y[4][MC];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int q = q0 + g + 4 * b;
+      const uint8_t* r = q < n ? X + size_t(id_of(q)) * D : X;  // unused slots read row 0
+#pragma unroll
+      for (int m = 0; m < MC; ++m) y[b][m] = m * 64 < D ? load_u8x4(r, m, D, t) : 0u;
+    }
+    float da[4], db[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      uint32_t aa = 0, ab = 0, bb = 0;
+#pragma unroll
+      for (int m = 0; m < MC; ++m) {
+        if (m * 64 >= D) break;
+        aa = __builtin_amdgcn_udot4(__float_as_uint(ta.v[m].x), y[b][m], aa, false);
+        ab = __builtin_amdgcn_udot4(__float_as_uint(tb.v[m].x), y[b][m], ab, false);
+        bb = __builtin_amdgcn_udot4(y[b][m], y[b][m], bb, false);
+      }
+      da[b] = float(ta.n2 + row_sum16(int(bb) - 2 * int(aa)));
+      db[b] = float(tb.n2 + row_sum16(int(bb) - 2 * int(ab)));
+    }
+    float va = 0.f, vb = 0.f;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const float xa = __shfl(da[b], src, TG_WAVE), xb = __shfl(db[b], src, TG_WAVE);
+      if (b == mine) {
+        va = xa;
+        vb = xb;
+      }
+    }
+    if (lane < 16 && q0 + lane < n) emit(q0 + lane, va, vb);
+  }
+}
+
 template <int MC, class T, class IdOf, class Emit>
 __device__ __forceinline__ void eval_batch(const T* X, int D, const Row16<MC>& tgt, int n,
                                            IdOf id_of, Emit emit) {

@@ -309,6 +309,67 @@ __device__ __forceinline__ int filter_pool(const PruneArgs& a, uint64_t* P, int 
       return id != TG_INV && id != pid;
     });
     const long long e0 = pclock(a);
+    if constexpr (std::is_same<T, uint8_t>::value) {
+      // Two rounds in one pass: the first live candidate c1 is the next selection exactly
+      // when this round keeps it, which d(p, c1) -- two rows in registers -- decides.  Then
+      // each candidate row is read once for d(p, c) and d(c1, c), and every candidate
+      // meets the two rounds' rules in their order: one invalidated by p is not an
+      // evaluation of the second round, an entry repeating c1's id is dropped uncounted
+      // (as at a round's start), the others count for the second round.
+      if (cnt > 1 && nsel + 1 < FD) {
+        const int i1 = cl[0];
+        const uint64_t k1 = P[i1];
+        const uint32_t id1 = key_id(k1);
+        const float cur1 = key_d(k1);
+        Row16<MC> qrow;
+        load_row16<MC>(qrow, reinterpret_cast<const T*>(a.rows) + size_t(id1) * D, D, t);
+        const float d1 = dist_rows_u8<MC>(prow, qrow, D);
+        if (!(rule_eval(rule, d1, cur1, cur, a.th) < 0.0f) && i1 + 1 < L) {
+          int n2 = 0;
+          eval_batch2_u8<MC>(reinterpret_cast<const uint8_t*>(a.rows), D, prow, qrow, cnt,
+                             [&](int q) { return key_id(P[cl[q]]); },
+                             [&](int q, float da, float db) {
+                               if (q == 0) return;  // c1: kept by this round
+                               const int i = cl[q];
+                               const uint64_t k = P[i];
+                               if (rule_eval(rule, da, key_d(k), cur, a.th) < 0.0f) {
+                                 P[i] = key_with_id(k, TG_INV);
+                               } else if (key_id(k) == id1) {
+                                 P[i] = key_with_id(k, TG_INV);
+                               } else {
+                                 ++n2;
+                                 if (rule_eval(rule, db, key_d(k), cur1, a.th) < 0.0f)
+                                   P[i] = key_with_id(k, TG_INV);
+                               }
+                             });
+#pragma unroll
+          for (int o = 32; o > 0; o >>= 1) n2 += __shfl_xor(n2, o, TG_WAVE);
+          ph.ev += pclock(a) - e0;
+          n_eval += cnt + n2;
+          __syncthreads();
+          // the first round's selection is c1; the second's is its first survivor
+          if (lane == 0) sel[nsel] = k1;
+          ++nsel;
+          nl = compact_cl(cl, 1, cnt - 1, [&](int i) { return key_id(P[i]) != TG_INV; });
+          if (nl > 0) {
+            const int pi = cl[0];
+            if (lane == 0) sel[nsel] = P[pi];
+            cur = key_d(P[pi]);
+            pid = key_id(P[pi]);
+            tix = pi;
+            from = 1;
+            --nl;
+          } else {
+            tix = L;
+          }
+          ++nsel;
+          ++tix;
+          __syncthreads();
+          if (tix >= L) break;
+          continue;
+        }
+      }
+    }
     eval_batch<MC>(reinterpret_cast<const T*>(a.rows), D, prow, cnt, [&](int q) { return key_id(P[cl[q]]); },
                    [&](int q, float res) {
                      const int i = cl[q];
