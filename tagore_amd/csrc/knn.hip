@@ -683,18 +683,31 @@ __global__ __launch_bounds__(64) void k_phase2(KnnArgs a) {
       ids[off[l] + i] =
           i < nf ? a.G[size_t(v) * K + i] : key_id(a.R0[size_t(v) * K + (i - nf)]);
   }
-  const int n = pow2_at_least(max(tot, 1));
-  for (int i = tot + lane; i < n; i += TG_WAVE) ids[i] = TG_INV;
+  // unique (an LDS id set in kv's space, 1024 slots for at most 512 ids: the insert that
+  // finds the slot empty keeps the id), not u, not listed; gather order (the keys are sorted
+  // afterwards, so the order of evaluation does not matter)
+  uint32_t* hs = reinterpret_cast<uint32_t*>(kv);
+  for (int i = lane; i < 1024; i += TG_WAVE) hs[i] = TG_INV;
   __syncthreads();
-  wave_sort<512>(ids, n);
-  // unique, not u, not listed
   int cnt = 0;
   for (int base = 0; base < tot; base += TG_WAVE) {
     const int i = base + lane;
     bool ok = false;
     if (i < tot) {
       const uint32_t id = ids[i];
-      ok = id != TG_INV && id != u && (i == 0 || ids[i - 1] != id) && !listed(row, K, id);
+      if (id != TG_INV && id != u) {
+        uint32_t h = (id * 0x9E3779B1u) >> 22;
+        for (;;) {
+          const uint32_t old = atomicCAS(&hs[h], TG_INV, id);
+          if (old == TG_INV) {
+            ok = true;
+            break;
+          }
+          if (old == id) break;
+          h = (h + 1) & 1023u;
+        }
+        ok = ok && !listed(row, K, id);
+      }
     }
     const uint64_t m = ballot(ok);
     if (ok) cl[cnt + mask_rank(m)] = uint16_t(i);
