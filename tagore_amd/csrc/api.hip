@@ -1062,6 +1062,7 @@ int tg_knn_build(tg_state* s, const tg_knn_params* p) {
     }
     k.exact_screen = exact_screen ? 1 : 0;
     k.Xr16 = rows16;
+    k.X8 = (p->precision != TG_PREC_EXACT && s->cert_u8 && s->D <= 256) ? s->X8 : nullptr;
     if (p->precision == TG_PREC_FP8) {
       k.Xq8 = s->Xq8;
       k.sc8 = s->sc8;

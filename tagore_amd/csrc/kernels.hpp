@@ -106,6 +106,7 @@ struct KnnArgs {
   int exact_screen;
   // rows used for canonical re-scores: BF16 copy when the data set is BF16-exact, else X
   const uint16_t* Xr16;
+  const uint8_t* X8;     // u8 copy of X (every value an integer in [0, 255], D <= 256) or null
   // FP8 e4m3 copy with a per-vector scale (FP8 screen: nq/ne then hold its norms)
   const uint8_t* Xq8;
   const float* sc8;

@@ -74,7 +74,9 @@ __global__ __launch_bounds__(64) void k_init_sort(KnnArgs a) {
   if (u >= a.N) return;
   const int lane = threadIdx.x, t = lane & 15;
   const int K = a.K, D = a.D, n = pow2_at_least(K);
-  const T* X = a.Xr16 ? reinterpret_cast<const T*>(a.Xr16) : reinterpret_cast<const T*>(a.X);
+  const T* X = std::is_same<T, uint8_t>::value
+                   ? reinterpret_cast<const T*>(a.X8)
+                   : (a.Xr16 ? reinterpret_cast<const T*>(a.Xr16) : reinterpret_cast<const T*>(a.X));
   Row16<MC> xu;
   load_row16<MC>(xu, X + size_t(u) * D, D, t);
   // the list's ids in LDS first, so the distance batches wait for one load, not two
@@ -888,8 +890,14 @@ static void launch_init_sort(const KnnArgs& a, hipStream_t s) {
   do {                                                               \
     if (mc <= 2) IS(2, T); else if (mc <= 4) IS(4, T); else IS(16, T); \
   } while (0)
-  if (a.Xr16) IS_MC(uint16_t);
-  else IS_MC(float);
+  if (a.X8 && a.D <= 256) {  // exact dot4 distances on the u8 copy
+    if (mc <= 2) IS(2, uint8_t);
+    else IS(4, uint8_t);
+  } else if (a.Xr16) {
+    IS_MC(uint16_t);
+  } else {
+    IS_MC(float);
+  }
 #undef IS_MC
 #undef IS
 }
