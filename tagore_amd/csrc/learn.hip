@@ -408,12 +408,23 @@ __global__ __launch_bounds__(64) void k_km_update(const float* __restrict__ Z,
 #pragma unroll
   for (int k = 0; k < int(MLP_CODE); ++k) acc[k] = 0.f;
   float cnt = 0.f;
-  for (uint32_t j = lane; j < S; j += TG_WAVE) {
-    if (slab[j] != c) continue;
-    const float* zp = Z + size_t(samp[j]) * MLP_CODE;
+  // each lane visits its members j = lane, lane + 64, ... in ascending order; the labels
+  // are loaded eight at a time so that the scan is not one dependent load per step
+  for (uint32_t j0 = lane; j0 < S; j0 += 8 * TG_WAVE) {
+    uint32_t lb[8];
 #pragma unroll
-    for (int k = 0; k < int(MLP_CODE); ++k) acc[k] += zp[k];
-    cnt += 1.f;
+    for (int r = 0; r < 8; ++r) {
+      const uint32_t j = j0 + uint32_t(r) * TG_WAVE;
+      lb[r] = j < S ? slab[j] : 0xFFFFFFFFu;
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      if (lb[r] != c) continue;
+      const float* zp = Z + size_t(samp[j0 + uint32_t(r) * TG_WAVE]) * MLP_CODE;
+#pragma unroll
+      for (int k = 0; k < int(MLP_CODE); ++k) acc[k] += zp[k];
+      cnt += 1.f;
+    }
   }
   cnt = wave_sum(cnt);
   float mine = 0.f;
