@@ -158,60 +158,79 @@ __device__ __forceinline__ float wave_sum(float v) {
 }
 
 // Contrastive loss and its gradient with respect to the codes: one workgroup of four waves
-// per anchor b; each wave takes every fourth negative, the wave partials are added in wave
-// order (deterministic).
+// per pair of anchors (B is even), so each negative's code is read once for both; thread tid
+// takes the negatives c = tid, tid + 256, ... for each anchor in that order, the wave
+// partials are added in wave order, and wave a finishes anchor b0 + a (deterministic, and
+// the same sums as one anchor per workgroup).
+constexpr int LOSS_AP = 2;
 __global__ __launch_bounds__(256) void k_mlp_loss(const float* __restrict__ Z, uint32_t B,
                                                   uint32_t P, float margin, uint32_t M,
                                                   uint16_t* dZb, uint16_t* dZT, float* lossp) {
-  __shared__ float red[4][MLP_CODE + 2];
-  const uint32_t b = blockIdx.x;
-  if (b >= B) return;
+  __shared__ float red[LOSS_AP][4][MLP_CODE + 2];
+  __shared__ float zas[LOSS_AP][MLP_CODE];
+  const uint32_t b0 = blockIdx.x * LOSS_AP;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  float za[MLP_CODE], acc[MLP_CODE];
-#pragma unroll
-  for (int k = 0; k < int(MLP_CODE); ++k) {
-    za[k] = Z[size_t(b) * MLP_CODE + k];
-    acc[k] = 0.f;
-  }
-  float nact = 0.f, lneg = 0.f;
-  for (uint32_t c = tid; c < B; c += 256) {
-    if (c == b) continue;
-    const float* zc = Z + size_t(c) * MLP_CODE;
-    float d2 = 0.f;
-#pragma unroll
-    for (int k = 0; k < int(MLP_CODE); ++k) {
-      const float e = za[k] - zc[k];
-      d2 = fmaf(e, e, d2);
-    }
-    if (d2 < margin) {
-      nact += 1.f;
-      lneg += margin - d2;
-#pragma unroll
-      for (int k = 0; k < int(MLP_CODE); ++k) acc[k] += zc[k];
-    }
-  }
-  nact = wave_sum(nact);
-  lneg = wave_sum(lneg);
-#pragma unroll
-  for (int k = 0; k < int(MLP_CODE); ++k) {
-    const float v = wave_sum(acc[k]);
-    if (lane == k) red[wave][2 + k] = v;
-  }
-  if (lane == 0) {
-    red[wave][0] = nact;
-    red[wave][1] = lneg;
+  if (tid < LOSS_AP * int(MLP_CODE)) {
+    const uint32_t b = b0 + uint32_t(tid) / MLP_CODE;
+    zas[tid / MLP_CODE][tid % MLP_CODE] = b < B ? Z[size_t(b) * MLP_CODE + tid % MLP_CODE] : 0.f;
   }
   __syncthreads();
-  if (wave != 0) return;
-  const int k = lane & 31;
-  nact = ((red[0][0] + red[1][0]) + red[2][0]) + red[3][0];
-  lneg = ((red[0][1] + red[1][1]) + red[2][1]) + red[3][1];
-  const float accs = ((red[0][2 + k] + red[1][2 + k]) + red[2][2 + k]) + red[3][2 + k];
-  const float cpos = 2.f / float(B * P), cneg = 4.f / (float(B) * float(B - 1));
-  float zak = 0.f;
+  float acc[LOSS_AP][MLP_CODE];
+  float nact[LOSS_AP], lneg[LOSS_AP];
 #pragma unroll
-  for (int q = 0; q < int(MLP_CODE); ++q)
-    if (q == k) zak = za[q];
+  for (int q = 0; q < LOSS_AP; ++q) {
+    nact[q] = lneg[q] = 0.f;
+#pragma unroll
+    for (int k = 0; k < int(MLP_CODE); ++k) acc[q][k] = 0.f;
+  }
+  for (uint32_t c = tid; c < B; c += 256) {
+    const float* zc = Z + size_t(c) * MLP_CODE;
+    float zv[MLP_CODE];
+#pragma unroll
+    for (int k = 0; k < int(MLP_CODE); ++k) zv[k] = zc[k];
+#pragma unroll
+    for (int q = 0; q < LOSS_AP; ++q) {
+      const uint32_t b = b0 + uint32_t(q);
+      if (c == b || b >= B) continue;
+      float d2 = 0.f;
+#pragma unroll
+      for (int k = 0; k < int(MLP_CODE); ++k) {
+        const float e = zas[q][k] - zv[k];
+        d2 = fmaf(e, e, d2);
+      }
+      if (d2 < margin) {
+        nact[q] += 1.f;
+        lneg[q] += margin - d2;
+#pragma unroll
+        for (int k = 0; k < int(MLP_CODE); ++k) acc[q][k] += zv[k];
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < LOSS_AP; ++q) {
+    const float n = wave_sum(nact[q]), l = wave_sum(lneg[q]);
+#pragma unroll
+    for (int k = 0; k < int(MLP_CODE); ++k) {
+      const float v = wave_sum(acc[q][k]);
+      if (lane == k) red[q][wave][2 + k] = v;
+    }
+    if (lane == 0) {
+      red[q][wave][0] = n;
+      red[q][wave][1] = l;
+    }
+  }
+  __syncthreads();
+  if (wave >= LOSS_AP) return;
+  const int qa = wave;
+  const uint32_t b = b0 + uint32_t(qa);
+  if (b >= B) return;
+  const int k = lane & 31;
+  const float na = ((red[qa][0][0] + red[qa][1][0]) + red[qa][2][0]) + red[qa][3][0];
+  const float ln = ((red[qa][0][1] + red[qa][1][1]) + red[qa][2][1]) + red[qa][3][1];
+  const float accs =
+      ((red[qa][0][2 + k] + red[qa][1][2 + k]) + red[qa][2][2 + k]) + red[qa][3][2 + k];
+  const float cpos = 2.f / float(B * P), cneg = 4.f / (float(B) * float(B - 1));
+  const float zak = zas[qa][k];
   float sp = 0.f, lpos = 0.f;
   for (uint32_t j = 0; j < P; ++j) {
     const uint32_t row = B + b * P + j;
@@ -226,12 +245,12 @@ __global__ __launch_bounds__(256) void k_mlp_loss(const float* __restrict__ Z, u
     }
   }
   if (lane < int(MLP_CODE)) {
-    const uint16_t g = f2bf(cpos * sp - cneg * (nact * zak - accs));
+    const uint16_t g = f2bf(cpos * sp - cneg * (na * zak - accs));
     dZb[size_t(b) * MLP_CODE + k] = g;
     dZT[size_t(k) * M + b] = g;
   }
   lpos = wave_sum(lane < int(MLP_CODE) ? lpos : 0.f);
-  if (lane == 0) lossp[b] = lpos / float(B * P) + lneg / (float(B) * float(B - 1));
+  if (lane == 0) lossp[b] = lpos / float(B * P) + ln / (float(B) * float(B - 1));
 }
 
 // Adam over the flat parameter block (and the BF16 weight copies)
@@ -803,7 +822,8 @@ void mlp_forward_backward(Learner& L, const float* X, const uint32_t* rows, uint
   g.Kd = MLP_HID; g.Cf = L.Z; g.ldc = MLP_CODE; g.bias = L.prm + mlp_n1(D) + MLP_HID;
   g.bias_stride = MLP_HID + 1;
   gemm<EPI_Z>(g, 1, st);
-  k_mlp_loss<<<B, 256, 0, st>>>(L.Z, B, P, margin, M, L.dZb, L.dZT, L.lossp);
+  k_mlp_loss<<<(B + LOSS_AP - 1) / LOSS_AP, 256, 0, st>>>(L.Z, B, P, margin, M, L.dZb, L.dZT,
+                                                         L.lossp);
   const uint32_t kt = M / GK;
   const uint32_t splits = std::min(L.splits, std::max(1u, kt));
   // unused split slots must read as zero in the reduction
