@@ -546,6 +546,7 @@ __global__ __launch_bounds__(64) void k_seed_lists(const uint32_t* __restrict__ 
                                                    const uint32_t* __restrict__ order, uint32_t N,
                                                    uint32_t K, uint64_t seed, uint32_t* G) {
   __shared__ uint32_t acc[128];
+  __shared__ uint32_t first[TG_WAVE];
   const uint32_t u = blockIdx.x;
   if (u >= N) return;
   const int lane = threadIdx.x;
@@ -566,11 +567,14 @@ __global__ __launch_bounds__(64) void k_seed_lists(const uint32_t* __restrict__ 
         uint64_t s = st0 + (base + lane) * 0x9E3779B97F4A7C15ull;
         const uint32_t ix = sm_bounded(sm_next(s), n);
         const uint32_t c = mem ? mem[ix] : ix;
-        bool ok = c != u && !chosen(c);
-        for (int j = 0; j < TG_WAVE; ++j) {
-          const uint32_t cj = __shfl(c, j, TG_WAVE);
-          if (j < lane && cj == c) ok = false;
-        }
+        // a draw repeating one in a lower lane is rejected: sort (id, lane) and keep the
+        // first of each id
+        uint64_t kv[1] = {(uint64_t(c) << 32) | uint32_t(lane)};
+        reg_bitonic<1>(kv);
+        const uint32_t prev = uint32_t(__shfl(int(kv[0] >> 32), (lane + 63) & 63, TG_WAVE));
+        first[uint32_t(kv[0])] = (lane == 0 || prev != uint32_t(kv[0] >> 32)) ? 1u : 0u;
+        __syncthreads();
+        const bool ok = c != u && first[lane] != 0 && !chosen(c);
         const uint64_t m = ballot(ok);
         const int rk = cnt + mask_rank(m);
         __syncthreads();
