@@ -30,6 +30,7 @@ constexpr int TAKE = 64;     // forward / reverse entries taken per leader
 // equal draw is either accepted or rejected for a reason that applies to this one too).
 __global__ __launch_bounds__(64) void k_init_ids(KnnArgs a) {
   __shared__ uint32_t acc[128];
+  __shared__ uint32_t first[TG_WAVE];
   const uint32_t u = blockIdx.x;
   if (u >= a.N) return;
   const int lane = threadIdx.x, K = a.K;
@@ -50,10 +51,13 @@ __global__ __launch_bounds__(64) void k_init_ids(KnnArgs a) {
     const uint32_t c = lo + sm_bounded(sm_next(s), n);
     bool ok = c != u;
     for (int j = 0; j < cnt && ok; ++j) ok = acc[j] != c;
-    for (int j = 0; j < TG_WAVE; ++j) {
-      const uint32_t cj = __shfl(c, j, TG_WAVE);
-      if (j < lane && cj == c) ok = false;
-    }
+    // a draw repeating one in a lower lane is rejected: sort (id, lane), keep each id's first
+    uint64_t kv[1] = {(uint64_t(c) << 32) | uint32_t(lane)};
+    reg_bitonic<1>(kv);
+    const uint32_t prev = uint32_t(__shfl(int(kv[0] >> 32), (lane + 63) & 63, TG_WAVE));
+    first[uint32_t(kv[0])] = (lane == 0 || prev != uint32_t(kv[0] >> 32)) ? 1u : 0u;
+    __syncthreads();
+    ok = ok && first[lane] != 0;
     const uint64_t m = ballot(ok);
     const int rk = cnt + mask_rank(m);
     __syncthreads();
