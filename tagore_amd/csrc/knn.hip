@@ -650,8 +650,7 @@ __device__ __forceinline__ float screen_bound2(float na, float qa, float ea, flo
 template <int MC, int PREC, class T>
 __global__ __launch_bounds__(64) void k_phase2(KnnArgs a) {
   __shared__ RowLds row;
-  __shared__ uint32_t ids[512];
-  __shared__ uint16_t cl[512];
+  __shared__ uint32_t ids[512];  // gathered ids, then the candidates compacted in place
   __shared__ uint64_t kv[512];
   __shared__ uint32_t lead[LEADERS];
   const uint32_t u = row_of_block(a.perm, a.N);
@@ -699,8 +698,8 @@ __global__ __launch_bounds__(64) void k_phase2(KnnArgs a) {
   for (int base = 0; base < tot; base += TG_WAVE) {
     const int i = base + lane;
     bool ok = false;
-    if (i < tot) {
-      const uint32_t id = ids[i];
+    const uint32_t id = i < tot ? ids[i] : TG_INV;
+    {
       if (id != TG_INV && id != u) {
         uint32_t h = (id * 0x9E3779B1u) >> 22;
         for (;;) {
@@ -715,8 +714,9 @@ __global__ __launch_bounds__(64) void k_phase2(KnnArgs a) {
         ok = ok && !listed(row, K, id);
       }
     }
+    // in place: this chunk was read above, and writes land at or below the read positions
     const uint64_t m = ballot(ok);
-    if (ok) cl[cnt + mask_rank(m)] = uint16_t(i);
+    if (ok) ids[cnt + mask_rank(m)] = id;
     cnt += __popcll(m);
   }
   __syncthreads();
@@ -741,11 +741,12 @@ __global__ __launch_bounds__(64) void k_phase2(KnnArgs a) {
     eu = a.ne[u];
   }
   for (int q = lane; q < nk; q += TG_WAVE) kv[q] = TG_KEY_INF;
-  // BF16 screen: keep (in cl order) only the candidates whose bound interval reaches below
-  // the worst distance; everything kept is re-scored exactly
+  // BF16 screen: keep (in order) only the candidates whose bound interval reaches below the
+  // worst distance; everything kept is re-scored exactly.  The keep flags use kv's space
+  // (the keys are cleared again after the compaction).
   int nre = cnt;
   if (SCREEN) {
-    __shared__ uint16_t keep[512];
+    uint16_t* keep = reinterpret_cast<uint16_t*>(kv);
     // item q0 + g + 4b in group g, four candidates per group in flight
     for (int q0 = 0; q0 < cnt; q0 += 16) {
       uint32_t y[4];
@@ -753,7 +754,7 @@ __global__ __launch_bounds__(64) void k_phase2(KnnArgs a) {
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
         const int q = q0 + g + 4 * b;
-        y[b] = q < cnt ? ids[cl[q]] : u;  // unused slots read row u
+        y[b] = q < cnt ? ids[q] : u;  // unused slots read row u
         nb[b] = a.nrm2[y[b]];
       }
       if (PREC == KP_FP8) {
@@ -785,23 +786,25 @@ __global__ __launch_bounds__(64) void k_phase2(KnnArgs a) {
     }
     __syncthreads();
     nre = 0;
-    if (!a.exact_screen)
-    for (int base = 0; base < cnt; base += TG_WAVE) {
-      const int q = base + lane;
-      const bool k = q < cnt && keep[q];
-      const uint16_t c = q < cnt ? cl[q] : 0;
-      const uint64_t m = ballot(k);
-      __syncthreads();
-      if (k) cl[nre + mask_rank(m)] = c;
-      nre += __popcll(m);
-      __syncthreads();
+    if (!a.exact_screen) {
+      for (int base = 0; base < cnt; base += TG_WAVE) {
+        const int q = base + lane;
+        const bool k = q < cnt && keep[q];
+        const uint32_t c = q < cnt ? ids[q] : 0u;
+        const uint64_t m = ballot(k);
+        __syncthreads();
+        if (k) ids[nre + mask_rank(m)] = c;
+        nre += __popcll(m);
+        __syncthreads();
+      }
+      for (int q = lane; q < nk; q += TG_WAVE) kv[q] = TG_KEY_INF;
     }
   }
   __syncthreads();
   const T* Xr = a.Xr16 ? reinterpret_cast<const T*>(a.Xr16) : reinterpret_cast<const T*>(a.X);
-  eval_batch<MC>(Xr, D, xu, nre, [&](int q) { return ids[cl[q]]; },
+  eval_batch<MC>(Xr, D, xu, nre, [&](int q) { return ids[q]; },
                  [&](int q, float d) {
-                   if (d < worst) kv[q] = mk_key(d, ids[cl[q]]);
+                   if (d < worst) kv[q] = mk_key(d, ids[q]);
                  });
   if (a.counters && lane == 0 && nre) atomicAdd(&cnt_slot(a.counters)[1], (unsigned long long)nre);
   __syncthreads();
