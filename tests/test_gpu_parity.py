@@ -212,6 +212,35 @@ def test_knn_bit_identical(N, D, K, iters, kind, precision):
     assert st["knn_evals"] == ev
 
 
+@pytest.mark.parametrize("precision", ["exact", "bf16"])
+def test_knn_bit_identical_hub(precision):
+    """A hub: 1500 integer points within +-3 of one centre, plus 500 points of distant
+    clusters.  The centre enters nearly every list near it, so its reverse run and its push
+    list hold far more than 256 keys (the chunked top-R path) -- still the oracle's graph."""
+    rng = np.random.default_rng(404)
+    N, D, K = 2000, 64, 32
+    X = np.empty((N, D), np.float32)
+    X[0] = 128.0
+    X[1:1500] = 128.0 + rng.integers(-3, 4, (1499, D))
+    X[1500:] = gmm_int(500, D, 8, 404)
+    seed = 0x4B
+    Gr, GDr, ev = oracle.gnn_descent(X, K, 4, seed)
+    indeg = np.bincount(Gr.ravel(), minlength=N)
+    _say(f"hub {precision}: centre in-degree {indeg[0]}, max {indeg.max()}")
+    assert indeg.max() > 256
+    with State(N, D) as s:
+        s.set_data(X)
+        s.knn_build(K, 4, seed, precision)
+        G, GD = s.get_knn()
+        st = s.stats()
+    bad = np.nonzero((G != Gr).any(1))[0]
+    assert bad.size == 0, f"{bad.size} rows differ, first {bad[:5]}"
+    assert np.array_equal(GD, GDr)
+    assert st["knn_evals"] == ev
+    if precision == "bf16":  # the prune's reverse pass sees the hub too
+        _compare(X, Gr, GDr, 0, 0, 0, 32, 64, 1.0, "bf16")
+
+
 def test_recall_judge_matches_oracle():
     X = gmm_unit(3000, 96, 16, 9)
     with State(*X.shape) as s:
