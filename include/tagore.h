@@ -121,7 +121,9 @@ int tg_set_pruned(tg_state* s, uint32_t width, const uint32_t* deg, const uint32
 void tg_host_free(void* p);
 
 /* kNN graph.  tg_knn_build runs GNN-Descent (GNN_descent, pybind.cu:19-142); tg_set_knn
- * installs a given graph (rows sorted by (distance, id)); tg_get_knn copies it back. */
+ * installs a given graph (rows sorted by (distance, id)); tg_get_knn copies it back.
+ * tg_knn_build takes handles of at most 2^26 vectors with N*K below 2^32 (TG_ERR_ARG
+ * otherwise); largeIndex batches larger sets. */
 int tg_knn_build(tg_state* s, const tg_knn_params* p);
 int tg_set_knn(tg_state* s, uint32_t K, const uint32_t* ids, const float* dists);
 int tg_get_knn(tg_state* s, uint32_t* ids, float* dists);

@@ -1003,6 +1003,10 @@ int tg_knn_build(tg_state* s, const tg_knn_params* p) {
   const uint32_t K = p->K;
   if (K < 2 || K > 128) return fail(TG_ERR_ARG, "K must be in [2, 128]");
   if (K >= s->N) return fail(TG_ERR_ARG, "K must be smaller than N");
+  // the reverse lists index destinations in blocks of at most 2^13 ids (2^13 blocks) and
+  // edges with 32 bits
+  if (s->N > (1u << 26) || uint64_t(s->N) * K >= 0xFFFFFFF0ull)
+    return fail(TG_ERR_ARG, "kNN build: N must be at most 2^26 and N*K below 2^32");
   if (p->precision < TG_PREC_EXACT || p->precision > TG_PREC_FP8)
     return fail(TG_ERR_ARG, "unknown precision");
   if (p->precision == TG_PREC_BF16 && s->D % 8 != 0)
