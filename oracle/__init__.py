@@ -94,6 +94,13 @@ def lib():
         L.or_collect_path.argtypes = [f32p, ctypes.c_int, u32p, f32p, ctypes.c_int,
                                       ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, u32p,
                                       f32p]
+        L.or_collect_2hop.restype = None
+        L.or_collect_2hop.argtypes = [f32p, ctypes.c_int, u32p, f32p, ctypes.c_int,
+                                      ctypes.c_uint32, u32p, f32p]
+        L.or_reverse.restype = ctypes.c_uint64
+        L.or_reverse.argtypes = [f32p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_int, ctypes.c_float, u32p, u32p, f32p,
+                                 ctypes.c_int64, u32p, u32p, f32p, u32p, u32p]
         L.or_search.restype = ctypes.c_int
         L.or_search.argtypes = [f32p, ctypes.c_int64, ctypes.c_int, u32p, u32p, ctypes.c_int,
                                 ctypes.c_uint32, f32p, ctypes.c_int64, ctypes.c_int,
@@ -285,6 +292,33 @@ def collect_path(X, G, GD, TOPM: int, ep: int, u: int):
     pd = np.zeros(K + 480, np.float32)
     c = lib().or_collect_path(X, X.shape[1], G, GD, K, TOPM, ep, u, pid, pd)
     return pid, pd, int(c)
+
+
+def collect_2hop(X, G, GD, u: int):
+    """Sorted, deduplicated two-hop pool of node u (length K + 480)."""
+    X, G, GD = _f32(X), _u32(G), _f32(GD)
+    K = G.shape[1]
+    pid = np.zeros(K + 480, np.uint32)
+    pd = np.zeros(K + 480, np.float32)
+    lib().or_collect_2hop(X, X.shape[1], G, GD, K, u, pid, pd)
+    return pid, pd
+
+
+def reverse(X, K: int, FD: int, rule: int, th: float, fwd_deg, fwd_ids, fwd_d, pushes):
+    """The reverse phase on given forward rows (fwd_deg [N], fwd_ids/fwd_d [N, FD]) and Store
+    pushes, a sequence of (dst, src, d): final rows (deg [N], ids [N, FD]) and the rule
+    evaluations."""
+    X = _f32(X)
+    N = X.shape[0]
+    P = list(pushes)
+    dst = _u32([p[0] for p in P]) if P else np.zeros(0, np.uint32)
+    src = _u32([p[1] for p in P]) if P else np.zeros(0, np.uint32)
+    dd = _f32([p[2] for p in P]) if P else np.zeros(0, np.float32)
+    deg = np.zeros(N, np.uint32)
+    ids = np.full((N, FD), INV, np.uint32)
+    ev = lib().or_reverse(X, N, X.shape[1], K, FD, rule, float(th), _u32(fwd_deg),
+                          _u32(fwd_ids), _f32(fwd_d), len(P), dst, src, dd, deg, ids)
+    return deg, ids, int(ev)
 
 
 MLP_HID, MLP_CODE = 256, 32

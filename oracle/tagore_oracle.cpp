@@ -288,6 +288,87 @@ struct PruneCounters {
   uint64_t ep_evals = 0;
 };
 
+// collect_2hop (Tagore_src.cu:1352-1396; spec A.6): appends the first 480 two-hop ids,
+// slot i holding G[G[u][i / K]][i % K] (rows of the first neighbours, whole rows in list
+// order), each with its canonical distance to u.
+template <class DistFn>
+void collect_2hop_pool(const uint32_t* G, int K, int64_t u, DistFn&& dfn, std::vector<Ent>& P) {
+  for (int i = 0; i < RING; ++i) {
+    uint32_t v = G[u * K + i / K];
+    uint32_t y = G[uint64_t(v) * K + (i % K)];
+    P.push_back({dfn(y), y});
+  }
+}
+
+// Reverse phase (filter_reverse, Tagore_src.cu:1537-1657; spec A.7).  For every node u:
+// C = its forward row (fdeg[u] entries, in order) followed by its reverse list, the K
+// smallest (d, src) pushes addressed to u in ascending order (cap K: the reverse_num cap of
+// the Store step, Tagore_src.cu:1465-1475).  If |C| > FD, C is sorted, deduplicated and
+// filtered again with the reverse rule (at most FD-1 kept, :1561-1626); otherwise the row is
+// the forward row followed by the reverse ids not already in it, in list order
+// (:1628-1649).  Consumes `pushes`; returns the filter's rule evaluations.
+uint64_t reverse_phase(const float* X, int64_t N, int D, int K, int FD, int rev_rule, float th,
+                       const uint32_t* fdeg, const uint32_t* fids, const float* fdd,
+                       PushList& pushes, uint32_t* out_deg, uint32_t* out_ids,
+                       uint32_t* rev_cnt, uint64_t* rev_keys) {
+  const int nthreads = omp_get_max_threads();
+  std::vector<uint64_t> cnt_f(nthreads, 0);
+  std::vector<uint32_t> roff;
+  std::vector<uint64_t> rkeys;
+  pushes.reduce(uint32_t(N), uint32_t(K), roff, rkeys);
+#pragma omp parallel
+  {
+    const int tid = omp_get_thread_num();
+    std::vector<Ent> C, out;
+#pragma omp for schedule(dynamic, 64)
+    for (int64_t u = 0; u < N; ++u) {
+      const uint32_t nr = roff[u + 1] - roff[u];
+      const uint32_t deg = fdeg[u];
+      C.clear();
+      for (uint32_t k = 0; k < deg; ++k) C.push_back({fdd[u * FD + k], fids[u * FD + k]});
+      for (uint32_t k = 0; k < nr; ++k) {
+        uint64_t key = rkeys[roff[u] + k];
+        C.push_back({key_d(key), key_id(key)});
+      }
+      uint32_t* row = out_ids + u * FD;
+      for (int k = 0; k < FD; ++k) row[k] = INV;
+      if (int(C.size()) > FD) {
+        sort_dedup_pool(C);
+        uint64_t nf = 0;
+        filter_pool(C, uint32_t(u), FD, rev_rule, th, X, D, out, &nf);
+        cnt_f[tid] += nf;
+        out_deg[u] = uint32_t(out.size());
+        for (size_t k = 0; k < out.size(); ++k) row[k] = out[k].id;
+      } else {
+        uint32_t n = 0;
+        for (uint32_t k = 0; k < deg; ++k) row[n++] = fids[u * FD + k];
+        for (uint32_t k = 0; k < nr; ++k) {
+          uint32_t w = key_id(rkeys[roff[u] + k]);
+          bool found = false;
+          for (uint32_t j = 0; j < deg; ++j)
+            if (fids[u * FD + j] == w) {
+              found = true;
+              break;
+            }
+          if (!found) row[n++] = w;
+        }
+        out_deg[u] = n;
+      }
+    }
+  }
+  if (rev_cnt || rev_keys) {
+    for (int64_t u = 0; u < N; ++u) {
+      uint32_t nr = roff[u + 1] - roff[u];
+      if (rev_cnt) rev_cnt[u] = nr;
+      if (rev_keys)
+        for (uint32_t k = 0; k < nr; ++k) rev_keys[u * K + k] = rkeys[roff[u] + k];
+    }
+  }
+  uint64_t total = 0;
+  for (int t = 0; t < nthreads; ++t) total += cnt_f[t];
+  return total;
+}
+
 }  // namespace
 
 extern "C" {
@@ -475,12 +556,7 @@ int or_prune(const float* X, int64_t N, int D, const uint32_t* G, const float* G
           cnt_a[tid] += na;
           for (auto& e : ring) P.push_back(e);
         } else {
-          // collect_2hop (Tagore_src.cu:1352-1396; spec A.6): first 480 two-hop ids.
-          for (int i = 0; i < RING; ++i) {
-            uint32_t v = G[u * K + i / K];
-            uint32_t y = G[uint64_t(v) * K + (i % K)];
-            P.push_back({dfn(y), y});
-          }
+          collect_2hop_pool(G, K, u, dfn, P);
           cnt_c[tid] += RING;
           cnt_a[tid] += (RING + K - 1) / K;
         }
@@ -500,61 +576,9 @@ int or_prune(const float* X, int64_t N, int D, const uint32_t* G, const float* G
       pc.collect_evals += cnt_c[t];
       pc.filter_evals += cnt_f[t];
       pc.adj_rows += cnt_a[t];
-      cnt_f[t] = 0;
     }
-    std::vector<uint32_t> roff;
-    std::vector<uint64_t> rkeys;
-    pushes.reduce(uint32_t(N), uint32_t(K), roff, rkeys);
-    // Reverse phase (filter_reverse, Tagore_src.cu:1537-1657; spec A.7).
-#pragma omp parallel
-    {
-      const int tid = omp_get_thread_num();
-      std::vector<Ent> C, out;
-#pragma omp for schedule(dynamic, 64)
-      for (int64_t u = 0; u < N; ++u) {
-        const uint32_t nr = roff[u + 1] - roff[u];
-        const uint32_t deg = fdeg[u];
-        C.clear();
-        for (uint32_t k = 0; k < deg; ++k) C.push_back({fdd[u * FD + k], fids[u * FD + k]});
-        for (uint32_t k = 0; k < nr; ++k) {
-          uint64_t key = rkeys[roff[u] + k];
-          C.push_back({key_d(key), key_id(key)});
-        }
-        uint32_t* row = out_ids + u * FD;
-        for (int k = 0; k < FD; ++k) row[k] = INV;
-        if (int(C.size()) > FD) {
-          sort_dedup_pool(C);
-          uint64_t nf = 0;
-          filter_pool(C, uint32_t(u), FD, rev_rule, th, X, D, out, &nf);
-          cnt_f[tid] += nf;
-          out_deg[u] = uint32_t(out.size());
-          for (size_t k = 0; k < out.size(); ++k) row[k] = out[k].id;
-        } else {
-          uint32_t n = 0;
-          for (uint32_t k = 0; k < deg; ++k) row[n++] = fids[u * FD + k];
-          for (uint32_t k = 0; k < nr; ++k) {
-            uint32_t w = key_id(rkeys[roff[u] + k]);
-            bool found = false;
-            for (uint32_t j = 0; j < deg; ++j)
-              if (fids[u * FD + j] == w) {
-                found = true;
-                break;
-              }
-            if (!found) row[n++] = w;
-          }
-          out_deg[u] = n;
-        }
-      }
-    }
-    for (int t = 0; t < nthreads; ++t) pc.reverse_evals += cnt_f[t];
-    if (rev_cnt || rev_keys) {
-      for (int64_t u = 0; u < N; ++u) {
-        uint32_t nr = roff[u + 1] - roff[u];
-        if (rev_cnt) rev_cnt[u] = nr;
-        if (rev_keys)
-          for (uint32_t k = 0; k < nr; ++k) rev_keys[u * K + k] = rkeys[roff[u] + k];
-      }
-    }
+    pc.reverse_evals = reverse_phase(X, N, D, K, FD, rev_rule, th, fdeg.data(), fids.data(),
+                                     fdd.data(), pushes, out_deg, out_ids, rev_cnt, rev_keys);
   } else {
     // One-hop rank (filter_cagra :1668-1710 / filter_dpg :1713-1785; specs A.8, A.9).
     const int d = FD;
@@ -1200,6 +1224,35 @@ uint64_t or_collect_path(const float* X, int D, const uint32_t* G, const float* 
     pool_d[i] = P[i].d;
   }
   return c;
+}
+
+// Direct access to the two-hop Collect for tests: the sorted, deduplicated pool of node u
+// (length K + 480): its own list (ids and distances from GD) and the 480 two-hop slots.
+void or_collect_2hop(const float* X, int D, const uint32_t* G, const float* GD, int K,
+                     uint32_t u, uint32_t* pool_ids, float* pool_d) {
+  const float* xu = X + uint64_t(u) * D;
+  auto dfn = [&](uint32_t y) { return cdist(xu, X + uint64_t(y) * D, D); };
+  std::vector<Ent> P;
+  for (int j = 0; j < K; ++j) P.push_back({GD[uint64_t(u) * K + j], G[uint64_t(u) * K + j]});
+  collect_2hop_pool(G, K, int64_t(u), dfn, P);
+  sort_dedup_pool(P);
+  for (size_t i = 0; i < P.size(); ++i) {
+    pool_ids[i] = P[i].id;
+    pool_d[i] = P[i].d;
+  }
+}
+
+// Direct access to the reverse phase for tests: forward rows (fdeg [N], fids/fdd [N x FD])
+// and a list of Store pushes (dst, src, d) -- src is pushed to dst with distance d -- give
+// the final rows exactly as or_prune's reverse phase does.  Returns the rule evaluations.
+uint64_t or_reverse(const float* X, int64_t N, int D, int K, int FD, int rev_rule, float th,
+                    const uint32_t* fdeg, const uint32_t* fids, const float* fdd,
+                    int64_t npush, const uint32_t* dst, const uint32_t* src, const float* d,
+                    uint32_t* out_deg, uint32_t* out_ids) {
+  PushList pushes(1);
+  for (int64_t i = 0; i < npush; ++i) pushes.push(0, dst[i], ent_key(d[i], src[i]));
+  return reverse_phase(X, N, D, K, FD, rev_rule, th, fdeg, fids, fdd, pushes, out_deg,
+                       out_ids, nullptr, nullptr);
 }
 
 // ---------------------------------------------------------------------------------------
