@@ -15,9 +15,12 @@ Three paths are timed in the same run:
   * ablation:   random init + BF16 screen at the fewest iterations that reach the recall
     target (--ablation-iters), to separate the two ingredients.
 `value` is the headline's vectors/s over the timed steps; the other paths' figures and the
-ratio are reported beside it.  Every graph is judged by FP64 recall@10 on a seeded sample.
-The roofline entry is the pruning forward kernel (HBM-bound gather), timed live with HIP
-events on the library's stream; the CPU oracle runs the same pipeline on a bounded sample.
+ratios are reported beside it, the plain path also at its own fewest iterations reaching the
+recall target.  Every kNN graph is judged by FP64 recall@10 on a seeded sample; the NSG index
+by a beam search with held-out queries.  The roofline entry is the pruning forward kernel,
+the step's largest, timed live with HIP events on the library's stream: its compulsory HBM
+bytes against the HBM peak, the measured HBM traffic, and its vector-ALU issue fraction from
+the committed counter run.  The CPU oracle runs the same pipeline on a bounded sample.
 
     python bench.py [--gpus 1] [--steps 2] [--warmup 1]
 """
@@ -95,22 +98,66 @@ def reachable_fraction(st):
     return float(seen.mean())
 
 
-def measured_traffic(N, D, K, FD, TOPM):
-    """HBM bytes per k_forward launch from the committed rocprofv3 counter runs of the same
-    workload (FETCH_SIZE and WRITE_SIZE, each scaled by its copy-kernel calibration), or
-    (None, None) when no run of this workload is recorded."""
-    path = os.path.join(ROOT, "profiles", "k_forward_traffic.json")
+def _profile_record(name, N, D, K, FD, TOPM):
+    """A committed counter record under profiles/ for this workload, or None."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
-        return None, None
+        return None
     with open(path) as f:
         rec = json.load(f)
     w = rec.get("workload", {})
     if (w.get("N"), w.get("dim"), w.get("K"), w.get("final_degree"), w.get("m")) != (
             N, D, K, FD, TOPM):
+        return None
+    return rec
+
+
+def measured_traffic(N, D, K, FD, TOPM):
+    """HBM bytes per k_forward launch from the committed rocprofv3 counter runs of the same
+    workload (FETCH_SIZE and WRITE_SIZE, each scaled by its copy-kernel calibration), or
+    (None, None) when no run of this workload is recorded."""
+    rec = _profile_record("k_forward_traffic.json", N, D, K, FD, TOPM)
+    if rec is None:
         return None, None
     # the calibration copy moved `scale` times the bytes the counter reported
     b = rec["fetch_kib"] * 1024 * rec["fetch_scale"] + rec["write_kib"] * 1024 * rec["write_scale"]
     return b, "profiles/k_forward_traffic.json"
+
+
+def issue_bound(N, D, K, FD, TOPM, kernel_ms):
+    """What bounds k_forward: its vector-ALU issue, from the committed SQ counter run of the
+    same workload.  valu_issue_frac = VALU wave-instructions x 4 cycles (one wave64 VALU
+    instruction occupies a SIMD for 4 cycles) / (SIMDs x cycles of the launch at the
+    measured clock)."""
+    rec = _profile_record("k_forward_counters.json", N, D, K, FD, TOPM)
+    if rec is None:
+        return None
+    c = rec["counters"]
+    simds = rec.get("simds", 1024)
+    clk_ghz = rec.get("clock_ghz", 2.4)
+    busy = simds * clk_ghz * 1e9 * kernel_ms / 1e3
+    out = {"source": "profiles/k_forward_counters.json", "simds": simds, "clock_ghz": clk_ghz,
+           "valu_insts": c["SQ_INSTS_VALU"], "valu_insts_per_node": c["SQ_INSTS_VALU"] / N,
+           "valu_issue_frac": c["SQ_INSTS_VALU"] * 4 / busy}
+    if "distance_valu_insts" in rec:
+        out["distance_valu_share"] = rec["distance_valu_insts"] / c["SQ_INSTS_VALU"]
+    return out
+
+
+def weak_components(deg, ids):
+    """Number of weakly connected components of a directed graph given as rows
+    (deg [N], ids [N, width]) and the fraction of vertices in the largest one."""
+    import numpy as np
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import connected_components
+
+    N = deg.shape[0]
+    valid = np.arange(ids.shape[1])[None, :] < deg[:, None]
+    rows = np.broadcast_to(np.arange(N, dtype=np.int32)[:, None], ids.shape)[valid]
+    cols = ids[valid].astype(np.int32)
+    A = csr_matrix((np.ones(rows.size, np.int8), (rows, cols)), shape=(N, N))
+    n, lab = connected_components(A, directed=True, connection="weak")
+    return int(n), float(np.bincount(lab).max() / N)
 
 
 def bench_clusters(args):
@@ -303,6 +350,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--search-queries", type=int, default=10000,
                     help="held-out queries for the beam-search check of the NSG index (0: off)")
+    ap.add_argument("--components", type=int, default=1,
+                    help="count the weakly connected components of the kNN and NSG graphs")
     ap.add_argument("--phase-clocks", action="store_true",
                     help="also report the prune forward kernel's per-phase shader cycles")
     ap.add_argument("--seed", type=int, default=0x7A600002)
@@ -336,6 +385,8 @@ def main():
         bench_wide(args)
         return
 
+    import numpy as np
+
     import tagore_amd
 
     N, D, K = args.n, args.dim, args.k
@@ -344,13 +395,29 @@ def main():
     st.gen_data(0, args.ncomp, args.seed, 20.0)
     st.set_profiling(True)
 
-    plain = ablation = None
+    plain = plain_target = ablation = None
     if not args.no_plain:
         plain = measure(st, args, args.plain_precision, args.plain_iters, "random", "plain")
+        # the plain path at the fewest iterations that reach the recall target
+        sweep = []
+        for it in range(2, args.plain_iters + 1):
+            st.knn_build(K, it, args.seed, args.plain_precision, init="random")
+            r = st.recall(min(args.recall_queries, N), 10, 0x5A11)
+            sweep.append({"iters": it, "knn_ms": st.stats()["knn_ms"], "recall_at_10": r})
+            log(f"plain sweep: {it} iterations, recall@10 {r:.4f}")
+            if r >= 0.95:
+                break
+        hit = [x for x in sweep if x["recall_at_10"] >= 0.95]
+        if hit:
+            plain_target = measure(st, args, args.plain_precision, hit[0]["iters"], "random",
+                                   "plain at target")
+            plain_target["sweep"] = sweep
         if args.ablation_iters > 0:
             ablation = measure(st, args, args.precision, args.ablation_iters, "random",
                                "ablation")
     head = measure(st, args, args.precision, args.iters, args.init, "headline")
+    fdeg, _, _ = st.get_forward()
+    fwd_edges = int(fdeg.astype(np.int64).sum())
     if args.phase_clocks:
         # one more prune of the last headline graph with the forward kernel's phase counters
         # (instrumented, so not part of any timing above)
@@ -387,20 +454,51 @@ def main():
             search["sweep"].append(row)
             log(f"search L={L}: {row['qps']:.0f} queries/s, recall@10 {rec:.4f}")
 
-    # roofline of the pruning forward kernel (Collect + Filter): algorithmic bytes = every
-    # canonical distance evaluation reads one FP32 row, plus adjacency rows, own lists, output
+    # roofline of the pruning forward kernel (Collect + Filter + Store), the step's largest.
+    # Algorithmic bytes are the compulsory ones: every input byte read once (the FP32 rows,
+    # the kNN lists' ids and distances) and every output byte written once (degrees, forward
+    # ids and distances, one 8-byte reverse push per forward edge).  The gathers the rule
+    # performs (one row per canonical distance evaluation) re-read rows that L2 and the
+    # Infinity Cache serve; they are reported beside it as row_reads_bytes.
     pc = head["prune_counters"]
     fwd_ms = head["prune_sections"]["forward"]
     evals = pc["collect"] + pc["filter"]
-    alg_bytes = evals * D * 4 + pc["adj_rows"] * K * 4 + N * K * 8 + N * args.fd * 8
+    alg_bytes = N * D * 4 + N * K * 8 + N * 4 + N * args.fd * 8 + fwd_edges * 8
     achieved = alg_bytes / (fwd_ms / 1e3) / 1e9 if fwd_ms > 0 else None
     traffic, traffic_src = measured_traffic(N, D, K, args.fd, args.topm)
-    roof = {"bound": "hbm", "kernel": "k_forward (NSG path collect + filter)",
+    roof = {"bound": "hbm", "kernel": "k_forward (NSG path collect + filter + store)",
             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
             "traffic_source": traffic_src,
-            "algorithmic_bytes": alg_bytes, "kernel_ms": fwd_ms,
-            "evaluations": evals}
+            "hbm_frac_measured": (traffic / (fwd_ms / 1e3) / 1e9 / HBM_PEAK_GBS)
+            if traffic and fwd_ms > 0 else None,
+            "algorithmic_bytes": alg_bytes, "kernel_ms": fwd_ms, "evaluations": evals,
+            "row_reads_bytes": evals * D * 4 + pc["adj_rows"] * K * 4,
+            "issue": issue_bound(N, D, K, args.fd, args.topm, fwd_ms)}
+
+    # graph structure: weakly connected components of the kNN graph and of the NSG index
+    comps = None
+    if args.components:
+        Gk, _ = st.get_knn()
+        kc = weak_components(np.full(N, K, np.uint32), Gk)
+        del Gk
+        d_, i_, _ = st.get_pruned()
+        nc = weak_components(d_, i_)
+        comps = {"knn_weak_components": kc[0], "knn_largest_component_frac": kc[1],
+                 "nsg_weak_components": nc[0], "nsg_largest_component_frac": nc[1]}
+        log(f"components: kNN {kc}, NSG {nc}")
+
+    # the other prunes of the workload on the headline graph (device windows)
+    other_prunes = {}
+    for name, (kind_, fr, rr, th) in {"Vamana": ("path", "dist", "dist", 1.2),
+                                      "CAGRA": ("cagra", "dist", "rank", 0.0)}.items():
+        ms = []
+        for _ in range(max(1, args.steps)):
+            st.prune(kind_, fr, rr, args.fd, args.topm, th,
+                     args.prune_rows if args.precision == "bf16" else "fp32")
+            ms.append(st.stats()["prune_ms"])
+        other_prunes[name] = {"prune_ms": min(ms), "step_ms": head["knn_ms"] + min(ms)}
+        log(f"prune {name}: {min(ms):.1f} ms")
 
     cpu = None
     if args.cpu_sample > 0:
@@ -429,12 +527,22 @@ def main():
                    "N": N, "dim": D, "K": K, "iters": args.iters, "final_degree": args.fd,
                    "m": args.topm, "precision": args.precision, "init": args.init},
         "recall_at_10": head["recall_at_10"],
+        "recall_gate": "recall@10 of the kNN graph against FP64 brute force on a seeded "
+                       "sample of the vectors; the NSG index's own beam-search recall is "
+                       "index_search_recall_at_10",
         "recall_target_met": bool(head["recall_at_10"] >= 0.95
                                   and (plain is None or plain["recall_at_10"] >= 0.95)),
+        "index_search_recall_at_10": next((r["recall_at_10"] for r in search["sweep"]
+                                           if r["L"] == 64), None) if search else None,
         "headline": head,
         "plain_fp32": plain,
+        "plain_fp32_at_target": plain_target,
         "ablation_random_bf16": ablation,
         "speedup_vs_plain": (plain["ms_per_step"] / head["ms_per_step"]) if plain else None,
+        "speedup_vs_plain_at_target": (plain_target["ms_per_step"] / head["ms_per_step"])
+        if plain_target else None,
+        "other_prunes": other_prunes,
+        "graph_structure": comps,
         "roofline": roof,
         "cpu_baseline": cpu,
         "search_nsg": search,

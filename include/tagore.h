@@ -133,6 +133,9 @@ int tg_get_knn(tg_state* s, uint32_t* ids, float* dists);
 int tg_prune(tg_state* s, const tg_prune_params* p);
 /* Pruned adjacency: deg[N], ids[N*FD] (row stride FD), entry point (path kinds). */
 int tg_get_pruned(tg_state* s, uint32_t* deg, uint32_t* ids, uint32_t* ep);
+/* Row width (FINAL_DEGREE) of the last pruned graph; a test accessor beside
+ * tg_get_pruned, with no counterpart in the reference (which returns nothing). */
+int tg_pruned_width(tg_state* s, uint32_t* FD);
 /* Forward-phase adjacency (after Collect+Filter, before the reverse phase). */
 int tg_get_forward(tg_state* s, uint32_t* deg, uint32_t* ids, float* dists);
 /* Index file of the last prune: "NSG", "Vamana", "NSSG", "DPG", "CAGRA"

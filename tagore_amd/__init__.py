@@ -149,6 +149,7 @@ def lib():
     L.tg_prune.argtypes = [vp, P(PruneParams)]
     L.tg_get_pruned.argtypes = [vp, vp, vp, P(c.c_uint32)]
     L.tg_get_forward.argtypes = [vp, vp, vp, vp]
+    L.tg_pruned_width.argtypes = [vp, P(c.c_uint32)]
     L.tg_store_index.argtypes = [vp, c.c_char_p, c.c_char_p, c.c_uint64]
     L.tg_recall.argtypes = [vp, c.c_uint32, c.c_uint64, c.c_uint32, P(c.c_double)]
     L.tg_buffer.argtypes = [vp, c.c_int32, P(vp), P(c.c_size_t)]

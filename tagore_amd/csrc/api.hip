@@ -1359,6 +1359,13 @@ int tg_get_pruned(tg_state* s, uint32_t* deg, uint32_t* ids, uint32_t* ep) {
   return TG_OK;
 }
 
+int tg_pruned_width(tg_state* s, uint32_t* FD) {
+  if (!s || !FD) return fail(TG_ERR_ARG, "null argument");
+  if (!s->has_prune) return fail(TG_ERR_STATE, "no pruned graph");
+  *FD = s->last_prune.FD;
+  return TG_OK;
+}
+
 int tg_get_forward(tg_state* s, uint32_t* deg, uint32_t* ids, float* dists) {
   if (!s) return fail(TG_ERR_ARG, "null handle");
   if (!s->has_prune) return fail(TG_ERR_STATE, "no pruned graph");

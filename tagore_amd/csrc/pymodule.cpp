@@ -9,6 +9,7 @@
 //   * the 8 capsules share ownership of one device handle (any of them keeps it alive),
 //     and the kNN graph is not overwritten by pruning, so one build can be pruned many times;
 //   * capsule 2 holds FP32 vectors (the reference stores scaled FP16).
+#include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
@@ -201,6 +202,20 @@ void Pruning_with_CFS(unsigned K, unsigned POINTS, unsigned DIM, unsigned FINAL_
   run_prune(h->s, p, nullptr, "");
 }
 
+// Test accessor with no counterpart in the reference (whose Pruning_with_CFS leaves the
+// pruned rows in device memory only): the last pruned graph of `ptrs` as (deg [POINTS],
+// ids [POINTS, FINAL_DEGREE], entry point).
+py::tuple pruned_graph(unsigned K, unsigned POINTS, unsigned DIM, py::tuple ptrs) {
+  auto h = holder_of(ptrs);
+  check_shape(h->s, K, POINTS, DIM);
+  uint32_t FD = 0, ep = 0;
+  check(tg_pruned_width(h->s, &FD));
+  py::array_t<uint32_t> deg(POINTS);
+  py::array_t<uint32_t> ids({size_t(POINTS), size_t(FD)});
+  check(tg_get_pruned(h->s, deg.mutable_data(), ids.mutable_data(), &ep));
+  return py::make_tuple(deg, ids, ep);
+}
+
 // largeIndex (pybind.cu:297-391) on a one-GPU devicelist
 void largeIndex(std::vector<int> devicelist, unsigned DIM, unsigned K, unsigned cluster_num,
                 unsigned max_points, std::string file_prefix, std::string local_index_path_prefix,
@@ -253,6 +268,8 @@ PYBIND11_MODULE(Tagore, m) {
         py::arg("FINAL_DEGREE"), py::arg("TOPM") = 64, py::arg("ptrs"), py::arg("index_type"),
         py::arg("thre") = 1.0f, py::arg("index_path") = "index.data");
   m.def("Pruning_with_CFS", &Pruning_with_CFS, "Pruning_with_CFS");
+  m.def("pruned_graph", &pruned_graph, "the last pruned graph of ptrs: (deg, ids, ep)",
+        py::arg("K"), py::arg("POINTS"), py::arg("DIM"), py::arg("ptrs"));
   m.def("largeIndex", &largeIndex, "largeIndex", py::arg("devicelist"), py::arg("DIM"),
         py::arg("K"), py::arg("cluster_num"), py::arg("max_points"), py::arg("file_prefix"),
         py::arg("local_index_path_prefix"), py::arg("index_store_path"), py::arg("max_degree"),
