@@ -322,6 +322,14 @@ def test_prune_invariants(small_int_data, kind, fr, rr, th):
                     assert not (np.float32(np.float32(th) * res) - fd[j] < 0)
         out = r["ids"][u, :r["deg"][u]]
         assert np.all(out < N)
+        # final rows: unique, no self, drawn from the forward row and the reverse list
+        assert len(set(out.tolist())) == out.size and u not in out.tolist()
+        rv = (r["rev_keys"][u, :r["rev_cnt"][u]] & 0xFFFFFFFF).tolist()
+        assert set(out.tolist()) <= set(fw.tolist()) | set(rv)
+        if fw.size + len(rv) <= FD:
+            # no second filter: the forward row is kept whole, in order, at the front
+            assert out[:fw.size].tolist() == fw.tolist()
+            assert set(out.tolist()) == set(fw.tolist()) | set(rv)
     if kind == 0:
         assert r["ep"] == oracle.cal_ep(X, G, 64)
 
