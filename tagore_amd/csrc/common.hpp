@@ -58,8 +58,8 @@ __device__ __forceinline__ unsigned long long* cnt_slot(unsigned long long* c) {
 // ---------------------------------------------------------------------------------------
 template <int MC>
 struct Row16 {
-  float4 v[MC];
-  int n2;  // u8 rows only: |x|^2 of the row, exact, in lane t == 0 of the group
+  float4 v[MC];  // FP32 rows: elements m*64 + 4t .. +3; u8 rows: 16 bytes (quad layout below)
+  int n2;        // u8 rows only: |x|^2 of the row, exact, in every lane
 };
 
 // Four consecutive elements as FP32: rows are stored as FP32, or as BF16 bits when every
@@ -120,32 +120,54 @@ __device__ __forceinline__ float4 load4(const uint16_t* p) {
                      __uint_as_float(q.y << 16), __uint_as_float(q.y & 0xFFFF0000u));
 }
 
-// Reduction of an integer over the 16 lanes of a DPP row (valid in lane t == 0).
-__device__ __forceinline__ int row_sum16(int q) {
-  q += __builtin_amdgcn_update_dpp(0, q, 0x108, 0xF, 0xF, false);
-  q += __builtin_amdgcn_update_dpp(0, q, 0x104, 0xF, 0xF, false);
-  q += __builtin_amdgcn_update_dpp(0, q, 0x102, 0xF, 0xF, false);
-  return q + __builtin_amdgcn_update_dpp(0, q, 0x101, 0xF, 0xF, false);
+// u8 rows use a quad layout: the four lanes j = lane & 3 of every 4-lane quad share one row,
+// lane j holding bytes m*64 + 16j .. +15 of each 64-byte chunk m (zero past D).  A row is
+// then two 16-byte loads per lane at D = 128, and a dot product ends with two DPP steps
+// inside the quad.  All of it is integer arithmetic, so the lane layout does not change a
+// result.
+__device__ __forceinline__ uint4 load_u8x16(const uint8_t* x, int m, int D, int j) {
+  const int i = m * 64 + 16 * j;
+  if ((m + 1) * 64 <= D && (D & 15) == 0)  // uniform: whole chunk, 16-byte aligned rows
+    return *reinterpret_cast<const uint4*>(x + i);
+  uint4 r = make_uint4(0u, 0u, 0u, 0u);
+  if (i < D) r.x = *reinterpret_cast<const uint32_t*>(x + i);
+  if (i + 4 < D) r.y = *reinterpret_cast<const uint32_t*>(x + i + 4);
+  if (i + 8 < D) r.z = *reinterpret_cast<const uint32_t*>(x + i + 8);
+  if (i + 12 < D) r.w = *reinterpret_cast<const uint32_t*>(x + i + 12);
+  return r;
 }
-
-// Four bytes m*64 + 4t .. +3 of a u8 row as one word, zero past D (D % 4 == 0).
-__device__ __forceinline__ uint32_t load_u8x4(const uint8_t* x, int m, int D, int t) {
-  const int i = m * 64 + 4 * t;
-  return i < D ? *reinterpret_cast<const uint32_t*>(x + i) : 0u;
+__device__ __forceinline__ uint4 as_u4(const float4& v) {
+  return make_uint4(__float_as_uint(v.x), __float_as_uint(v.y), __float_as_uint(v.z),
+                    __float_as_uint(v.w));
+}
+// acc + the dot product of 16 bytes pairs (v_dot4_u32_u8)
+__device__ __forceinline__ uint32_t dot16_u8(const uint4& a, const uint4& b, uint32_t acc) {
+  acc = __builtin_amdgcn_udot4(a.x, b.x, acc, false);
+  acc = __builtin_amdgcn_udot4(a.y, b.y, acc, false);
+  acc = __builtin_amdgcn_udot4(a.z, b.z, acc, false);
+  return __builtin_amdgcn_udot4(a.w, b.w, acc, false);
+}
+// Sum of an integer over the 4 lanes of a quad, in every lane of it.
+__device__ __forceinline__ int quad_sum(int q) {
+  q += __builtin_amdgcn_mov_dpp(q, 0xB1, 0xF, 0xF, false);  // lane ^ 1
+  return q + __builtin_amdgcn_mov_dpp(q, 0x4E, 0xF, 0xF, false);  // lane ^ 2
 }
 
 template <int MC, class T>
 __device__ __forceinline__ void load_row16(Row16<MC>& r, const T* __restrict__ x, int D, int t) {
   if constexpr (std::is_same<T, uint8_t>::value) {
-    // u8 rows: word m holds the lane's four bytes of chunk m (in v[m].x), n2 = |x|^2
+    // u8 rows, quad layout: v[m] holds the lane's 16 bytes of chunk m as bits; n2 = |x|^2
+    // in every lane (t is not used: the layout follows lane & 3)
+    const int j = threadIdx.x & 3;
     uint32_t sq = 0;
 #pragma unroll
     for (int m = 0; m < MC; ++m) {
-      const uint32_t w = load_u8x4(x, m, D, t);
-      r.v[m] = make_float4(__uint_as_float(w), 0.f, 0.f, 0.f);
-      sq = __builtin_amdgcn_udot4(w, w, sq, false);
+      const uint4 w = m * 64 < D ? load_u8x16(x, m, D, j) : make_uint4(0u, 0u, 0u, 0u);
+      r.v[m] = make_float4(__uint_as_float(w.x), __uint_as_float(w.y), __uint_as_float(w.z),
+                           __uint_as_float(w.w));
+      sq = dot16_u8(w, w, sq);
     }
-    r.n2 = row_sum16(int(sq));
+    r.n2 = quad_sum(int(sq));
   } else {
 #pragma unroll
     for (int m = 0; m < MC; ++m) {
@@ -245,41 +267,18 @@ __device__ __forceinline__ void dist16_batch(const Row16<MC>& a, const T* const 
   for (int k = 0; k < NB; ++k) out[k] = tree16(p[k][0], p[k][1], p[k][2], p[k][3]);
 }
 
-// NB distances from a u8 row a (registers, with |a|^2) to u8 rows b[0..NB): the integer
-// |a|^2 + |b|^2 - 2 a.b with v_dot4_u32_u8, exact.  Used only when every value of the data set
-// is an integer in [0, 255] and every squared distance is below 2^24: then every partial sum
-// of the canonical FP32 fold is an exact integer too, so both give the same float.  Every b[k]
-// must be a readable row; results valid in lane t == 0.
-template <int MC, int NB>
-__device__ __forceinline__ void dist16_batch_u8(const Row16<MC>& a, const uint8_t* const (&b)[NB],
-                                                int D, int t, float (&out)[NB]) {
-  uint32_t y[NB][MC];
-#pragma unroll
-  for (int k = 0; k < NB; ++k)
-#pragma unroll
-    for (int m = 0; m < MC; ++m) y[k][m] = m * 64 < D ? load_u8x4(b[k], m, D, t) : 0u;
-#pragma unroll
-  for (int k = 0; k < NB; ++k) {
-    uint32_t ab = 0, bb = 0;
-#pragma unroll
-    for (int m = 0; m < MC; ++m) {
-      if (m * 64 >= D) break;
-      ab = __builtin_amdgcn_udot4(__float_as_uint(a.v[m].x), y[k][m], ab, false);
-      bb = __builtin_amdgcn_udot4(y[k][m], y[k][m], bb, false);
-    }
-    out[k] = float(a.n2 + row_sum16(int(bb) - 2 * int(ab)));
-  }
-}
-
-// Distance between two u8 rows held in registers (exact, as dist16_batch_u8), in every lane.
+// Distance between two u8 rows held in registers (quad layout), in every lane: the integer
+// |a|^2 + |b|^2 - 2 a.b with v_dot4_u32_u8, exact.  The u8 rows are used only when every
+// value of the data set is an integer in [0, 255] and every squared distance is below 2^24:
+// then every partial sum of the canonical FP32 fold is an exact integer too, so both give
+// the same float.
 template <int MC>
 __device__ __forceinline__ float dist_rows_u8(const Row16<MC>& a, const Row16<MC>& b, int D) {
   uint32_t ab = 0;
 #pragma unroll
   for (int m = 0; m < MC; ++m)
-    if (m * 64 < D) ab = __builtin_amdgcn_udot4(__float_as_uint(a.v[m].x), __float_as_uint(b.v[m].x), ab, false);
-  const float d = float(a.n2 + b.n2 - 2 * row_sum16(int(ab)));  // valid in lane t == 0
-  return __shfl(d, 0, TG_WAVE);
+    if (m * 64 < D) ab = dot16_u8(as_u4(a.v[m]), as_u4(b.v[m]), ab);
+  return float(a.n2 + b.n2 - 2 * quad_sum(int(ab)));
 }
 
 // NB distances between the row pairs (a[k], b[k]), both read from memory; every pointer
@@ -766,78 +765,105 @@ __device__ void merge_lds(const uint64_t* A, int na, const uint64_t* B, int nb, 
   for (int j = threadIdx.x; j < nb; j += TG_WAVE) dst[j + upper_bound_lds(A, na, B[j])] = B[j];
 }
 
-// Evaluates the targets listed in ids[0..n) (selected through idx when given) against the
-// register row `tgt`, four candidates per 16-lane group at a time (item q0 + g + 4b in group
-// g).  The 16 results are then moved to lanes 0..15 (lane j holds item q0 + j) and emit(q, d)
-// runs once in those 16 lanes instead of four times in one lane per group.
+// Loads of the u8 row at r in the quad layout (MC chunks; zero past D).
+template <int MC>
+__device__ __forceinline__ void load_quad_u8(uint4 (&y)[MC], const uint8_t* r, int D) {
+  const int j = threadIdx.x & 3;
+#pragma unroll
+  for (int m = 0; m < MC; ++m) y[m] = m * 64 < D ? load_u8x16(r, m, D, j) : make_uint4(0u, 0u, 0u, 0u);
+}
+// |a - y|^2 for the register row a and the loaded row y (quad layout), in every quad lane.
+template <int MC>
+__device__ __forceinline__ float quad_dist_u8(const Row16<MC>& a, const uint4 (&y)[MC], int D) {
+  uint32_t ab = 0, bb = 0;
+#pragma unroll
+  for (int m = 0; m < MC; ++m) {
+    if (m * 64 >= D) break;
+    ab = dot16_u8(as_u4(a.v[m]), y[m], ab);
+    bb = dot16_u8(y[m], y[m], bb);
+  }
+  return float(a.n2 + quad_sum(int(bb) - 2 * int(ab)));
+}
+
 // Two u8 targets at once: each candidate row is read once and gives d(ta, c) and d(tb, c);
-// emit(q, da, db) runs in lanes 0..15 for the items q0 + lane, as in eval_batch.
+// one candidate per quad, 16 per step (item q0 + lane / 4).  emit(q, da, db) runs in lanes
+// 0..15 for the items q0 + lane.
 template <int MC, class IdOf, class Emit>
 __device__ __forceinline__ void eval_batch2_u8(const uint8_t* X, int D, const Row16<MC>& ta,
                                                const Row16<MC>& tb, int n, IdOf id_of,
                                                Emit emit) {
-  const int lane = threadIdx.x, g = lane >> 4, t = lane & 15;
-  const int src = 16 * (lane & 3), mine = (lane >> 2) & 3;
+  const int lane = threadIdx.x, c = lane >> 2, src = (lane & 15) * 4;
   for (int q0 = 0; q0 < n; q0 += 16) {
-    uint32_t y[4][MC];
+    const int q = q0 + c;
+    uint4 y[MC];
+    load_quad_u8<MC>(y, q < n ? X + size_t(id_of(q)) * D : X, D);  // unused quads read row 0
+    uint32_t aa = 0, ab = 0, bb = 0;
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int q = q0 + g + 4 * b;
-      const uint8_t* r = q < n ? X + size_t(id_of(q)) * D : X;  // unused slots read row 0
-#pragma unroll
-      for (int m = 0; m < MC; ++m) y[b][m] = m * 64 < D ? load_u8x4(r, m, D, t) : 0u;
+    for (int m = 0; m < MC; ++m) {
+      if (m * 64 >= D) break;
+      aa = dot16_u8(as_u4(ta.v[m]), y[m], aa);
+      ab = dot16_u8(as_u4(tb.v[m]), y[m], ab);
+      bb = dot16_u8(y[m], y[m], bb);
     }
-    float da[4], db[4];
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      uint32_t aa = 0, ab = 0, bb = 0;
-#pragma unroll
-      for (int m = 0; m < MC; ++m) {
-        if (m * 64 >= D) break;
-        aa = __builtin_amdgcn_udot4(__float_as_uint(ta.v[m].x), y[b][m], aa, false);
-        ab = __builtin_amdgcn_udot4(__float_as_uint(tb.v[m].x), y[b][m], ab, false);
-        bb = __builtin_amdgcn_udot4(y[b][m], y[b][m], bb, false);
-      }
-      da[b] = float(ta.n2 + row_sum16(int(bb) - 2 * int(aa)));
-      db[b] = float(tb.n2 + row_sum16(int(bb) - 2 * int(ab)));
-    }
-    float va = 0.f, vb = 0.f;
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const float xa = __shfl(da[b], src, TG_WAVE), xb = __shfl(db[b], src, TG_WAVE);
-      if (b == mine) {
-        va = xa;
-        vb = xb;
-      }
-    }
+    const float da = float(ta.n2 + quad_sum(int(bb) - 2 * int(aa)));
+    const float db = float(tb.n2 + quad_sum(int(bb) - 2 * int(ab)));
+    const float va = __shfl(da, src, TG_WAVE), vb = __shfl(db, src, TG_WAVE);
     if (lane < 16 && q0 + lane < n) emit(q0 + lane, va, vb);
   }
 }
 
+// Evaluates the targets id_of(0..n) against the register row `tgt`, emit(q, d) once per item.
+//  * u8 rows (quad layout): one candidate per quad, 32 per step (items q0 + lane/4 and
+//    q0 + 16 + lane/4) while more than 16 remain, then 16; the results are moved to lanes
+//    0..31 (0..15), lane j holding item q0 + j, and emit runs there;
+//  * FP32 / BF16 rows: four candidates per 16-lane group at a time (item q0 + g + 4b in group
+//    g); the 16 results are moved to lanes 0..15 and emit runs there, once per item.
+// Every emit writes only what its own item owns, so the lanes may run it together.
 template <int MC, class T, class IdOf, class Emit>
 __device__ __forceinline__ void eval_batch(const T* X, int D, const Row16<MC>& tgt, int n,
                                            IdOf id_of, Emit emit) {
-  const int lane = threadIdx.x, g = lane >> 4, t = lane & 15;
-  const int src = 16 * (lane & 3), mine = (lane >> 2) & 3;
-  for (int q0 = 0; q0 < n; q0 += 16) {
-    const T* rows[4];
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int q = q0 + g + 4 * b;
-      rows[b] = q < n ? X + size_t(id_of(q)) * D : X;  // unused slots read row 0
+  const int lane = threadIdx.x;
+  if constexpr (std::is_same<T, uint8_t>::value) {
+    const int c = lane >> 2, src = (lane & 15) * 4;
+    int q0 = 0;
+    for (; q0 + 16 < n; q0 += 32) {
+      const int qb = q0 + 16 + c;
+      uint4 ya[MC], yb[MC];
+      load_quad_u8<MC>(ya, X + size_t(id_of(q0 + c)) * D, D);  // q0 + c < q0 + 16 < n
+      load_quad_u8<MC>(yb, qb < n ? X + size_t(id_of(qb)) * D : X, D);
+      const float da = quad_dist_u8<MC>(tgt, ya, D), db = quad_dist_u8<MC>(tgt, yb, D);
+      const float xa = __shfl(da, src, TG_WAVE), xb = __shfl(db, src, TG_WAVE);
+      const float v = lane < 16 ? xa : xb;
+      if (lane < 32 && q0 + lane < n) emit(q0 + lane, v);
     }
-    float dd[4];
-    if constexpr (std::is_same<T, uint8_t>::value)
-      dist16_batch_u8<MC, 4>(tgt, rows, D, t, dd);
-    else
+    if (q0 < n) {
+      const int q = q0 + c;
+      uint4 y[MC];
+      load_quad_u8<MC>(y, q < n ? X + size_t(id_of(q)) * D : X, D);
+      const float d = quad_dist_u8<MC>(tgt, y, D);
+      const float v = __shfl(d, src, TG_WAVE);
+      if (lane < 16 && q0 + lane < n) emit(q0 + lane, v);
+    }
+  } else {
+    const int g = lane >> 4, t = lane & 15;
+    const int src = 16 * (lane & 3), mine = (lane >> 2) & 3;
+    for (int q0 = 0; q0 < n; q0 += 16) {
+      const T* rows[4];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int q = q0 + g + 4 * b;
+        rows[b] = q < n ? X + size_t(id_of(q)) * D : X;  // unused slots read row 0
+      }
+      float dd[4];
       dist16_batch<MC, 4>(tgt, rows, D, t, dd);
-    float v = 0.f;
+      float v = 0.f;
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const float x = __shfl(dd[b], src, TG_WAVE);
-      if (b == mine) v = x;
+      for (int b = 0; b < 4; ++b) {
+        const float x = __shfl(dd[b], src, TG_WAVE);
+        if (b == mine) v = x;
+      }
+      if (lane < 16 && q0 + lane < n) emit(q0 + lane, v);
     }
-    if (lane < 16 && q0 + lane < n) emit(q0 + lane, v);
   }
 }
 
