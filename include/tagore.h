@@ -277,6 +277,22 @@ typedef struct {
 } tg_search_stats;
 int tg_search(tg_state* s, uint32_t nq, const float* queries, const tg_search_params* p,
               uint32_t* ids, float* dists, tg_search_stats* stats);
+/* Allocation-free variant of tg_search.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same kernel and the same results as tg_search, but
+ * queries_dev [nq*D] f32, ids_dev [nq*k] u32 and dists_dev [nq*k] f32 are DEVICE buffers
+ * the caller owns, and the search's scratch (visited bitmaps, visited-id lists, counters)
+ * lives in a 64 MiB static device array that each GPU the library runs on holds until the
+ * process ends; the call allocates and frees nothing.  At most
+ * (64 MiB - 256) / ((8192 + ceil(N/32)) * 4) workgroups run at once (TG_ERR_CAPACITY when
+ * not even one fits, i.e. N above about 5.3e8).  stream: a hipStream_t, or NULL for the
+ * handle's stream.  stats == NULL: the call only enqueues work and returns; otherwise it
+ * waits for the search and fills stats (evals, expansions; ms is not measured, 0).  Every
+ * call on one GPU shares that GPU's array: calls that overlap in time on one GPU (other
+ * streams, other host threads, other handles) race and must be serialised by the caller. */
+int tg_search_noalloc(tg_state* s, uint32_t nq, const float* queries_dev,
+                      const tg_search_params* p, uint32_t* ids_dev, float* dists_dev,
+                      void* stream, tg_search_stats* stats);
 /* recall@k (k <= 16) of result rows ids [nq*k] for host queries [nq*D] against an FP64
  * brute force over all N vectors of s (a tie at the k-th distance counts as a hit). */
 int tg_search_recall(tg_state* s, uint32_t nq, const float* queries, uint32_t k,

@@ -165,6 +165,8 @@ def lib():
     L.tg_mlp_adam.argtypes = [c.c_int, c.c_uint32, c.c_uint32, c.c_float, vp, vp, vp, vp]
     L.tg_selftest_sort.argtypes = [c.c_int, c.c_uint32, c.c_uint32, c.c_int32, vp, vp]
     L.tg_search.argtypes = [vp, c.c_uint32, vp, P(SearchParams), vp, vp, P(SearchStats)]
+    L.tg_search_noalloc.argtypes = [vp, c.c_uint32, vp, P(SearchParams), vp, vp, vp,
+                                    P(SearchStats)]
     L.tg_search_recall.argtypes = [vp, c.c_uint32, vp, c.c_uint32, vp, P(c.c_double)]
     for name in exported_symbols():
         f = getattr(L, name)

@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <initializer_list>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1643,6 +1644,109 @@ int tg_search(tg_state* s, uint32_t nq, const float* queries, const tg_search_pa
   dfree(oi);
   dfree(od);
   return rc;
+}
+
+// ------------------------------------------------------------ allocation-free search
+// tg_search_noalloc runs the same kernel as tg_search on caller-owned device buffers, with
+// its scratch (the two evaluation counters, the per-workgroup visited-id lists and visited
+// bitmaps) in one file-scope device array instead of handle buffers allocated on first use.
+// Each GPU the library runs on holds its own copy of the array until the process ends.
+// Host code never passes the array's name to a kernel (on the host it is only a stand-in
+// for the symbol): the device address comes from hipGetSymbolAddress, once per device with
+// that device current, and is kept per device.
+namespace {
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;  // 64 MiB per GPU
+constexpr size_t NOALLOC_HEAD = 256;  // counters (2 x u64), padded to the alignment
+constexpr int NOALLOC_MAX_DEVICES = 64;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// Device address of `scratch` on `device` (current on the calling thread), looked up once.
+int scratch_on(int device, unsigned char** out) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES)
+    return fail(TG_ERR_ARG, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(TG_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                  hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<unsigned char*>(ptr);
+  }
+  *out = g_scratch_ptr[device];
+  return TG_OK;
+}
+}  // namespace
+
+int tg_search_noalloc(tg_state* s, uint32_t nq, const float* queries_dev,
+                      const tg_search_params* p, uint32_t* ids_dev, float* dists_dev,
+                      void* stream, tg_search_stats* out) {
+  if (!s || !queries_dev || !p || !ids_dev || !dists_dev) return fail(TG_ERR_ARG, "null argument");
+  if (!s->has_prune) return fail(TG_ERR_STATE, "search needs a pruned graph");
+  if (s->nseg) return fail(TG_ERR_STATE, "search runs on a handle without segments");
+  if (nq < 1) return fail(TG_ERR_ARG, "nq must be >= 1");
+  if (p->L < 1 || p->L > 256) return fail(TG_ERR_ARG, "L must be in [1, 256]");
+  if (p->k < 1 || p->k > p->L) return fail(TG_ERR_ARG, "k must be in [1, L]");
+  const uint32_t ep = p->ep == TG_INV_ID ? s->ep : p->ep;
+  if (ep >= s->N) return fail(TG_ERR_ARG, "entry point out of range");
+  const uint32_t N = s->N, words = (N + 31) / 32;
+  // per workgroup: a visited-id list of SEARCH_VCAP ids and a bitmap of N bits
+  const size_t per_wg = (size_t(SEARCH_VCAP) + words) * 4;
+  const size_t fit = (NOALLOC_SCRATCH_BYTES - NOALLOC_HEAD) / per_wg;
+  if (fit < 1)
+    return fail(TG_ERR_CAPACITY, "allocation-free search: N = " + std::to_string(N) +
+                                     " needs " + std::to_string(per_wg + NOALLOC_HEAD) +
+                                     " bytes of scratch per workgroup, above the " +
+                                     std::to_string(NOALLOC_SCRATCH_BYTES) + "-byte maximum");
+  const uint32_t grid = uint32_t(std::min<size_t>({size_t(nq), size_t(SEARCH_GRID), fit}));
+  HIPX(hipSetDevice(s->device));
+  unsigned char* base = nullptr;
+  if (int rc = scratch_on(s->device, &base)) return rc;
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : s->stream;
+  auto* counters = reinterpret_cast<unsigned long long*>(base);
+  auto* vlist = reinterpret_cast<uint32_t*>(base + NOALLOC_HEAD);
+  uint32_t* bitmap = vlist + size_t(grid) * SEARCH_VCAP;
+  // the kernel leaves its bitmaps zero, but an interrupted earlier call may not have:
+  // clear the counters and the bitmaps this launch uses (fill kernels, no memset nodes)
+  fill_u32(reinterpret_cast<uint32_t*>(counters), 0, 4, st);
+  fill_u32(bitmap, 0, size_t(grid) * words, st);
+  SearchArgs a{};
+  a.X = s->X;
+  a.N = N;
+  a.D = s->D;
+  a.deg = s->out_deg;
+  a.ids = s->out_ids;
+  a.FD = s->last_prune.FD;
+  a.ep = ep;
+  a.Q = queries_dev;
+  a.nq = nq;
+  a.L = p->L;
+  a.k = p->k;
+  a.seed = p->seed;
+  a.bitmap = bitmap;
+  a.words = words;
+  a.vlist = vlist;
+  a.vcap = SEARCH_VCAP;
+  a.out_ids = ids_dev;
+  a.out_d = dists_dev;
+  a.counters = counters;
+  launch_search(a, grid, st);
+  HIPX(hipGetLastError());
+  if (out) {
+    unsigned long long cnt[2] = {0, 0};
+    HIPX(hipStreamSynchronize(st));
+    HIPX(hipMemcpy(cnt, counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    out->ms = 0.0;
+    out->evals = cnt[0];
+    out->expansions = cnt[1];
+  }
+  return TG_OK;
 }
 
 int tg_search_recall(tg_state* s, uint32_t nq, const float* queries, uint32_t k,
