@@ -185,17 +185,63 @@ __device__ __forceinline__ void beam_search(const PruneArgs& a, const Row16<MC>&
       hset_insert(tab, key_id(v[q]), hbits);
     }
     const long long s2 = pclock(a);
-    reg_bitonic<W>(v);
-    const long long s3 = pclock(a);
-    uint64_t c[W];
+    // Only candidates below the beam's TOPM-th key can enter it (a fresh id is in neither
+    // the beam nor the previous candidates, so no key ties one in the beam).  With two
+    // slots per lane and at most 64 such candidates (and TOPM <= 64), they are compacted
+    // and merged with one-slot sorts; none leaves the beam as it is.  Either way the beam
+    // becomes the TOPM smallest keys of beam and candidates.
+    int m = 64 * W;
+    bool small = false;
+    if (W > 1 && TOPM <= 64) {
+      const uint64_t wk = bk[0];
+      const uint64_t worst = (uint64_t(uint32_t(__builtin_amdgcn_readlane(int(wk >> 32), TOPM - 1))) << 32) |
+                             uint32_t(__builtin_amdgcn_readlane(int(uint32_t(wk)), TOPM - 1));
+      uint64_t qm[W];
+      m = 0;
 #pragma unroll
-    for (int b = 0; b < W; ++b) {
-      const uint64_t rv = lane_xor64(v[W - 1 - b], 63);  // candidate 64W-1-p
-      c[b] = bk[b] < rv ? bk[b] : rv;
+      for (int q = 0; q < W; ++q) {
+        qm[q] = ballot(key_finite(v[q]) && v[q] < worst);
+        m += __popcll(qm[q]);
+      }
+      small = m <= 64;
+      if (small && m > 0) {
+        int base = 0;
+#pragma unroll
+        for (int q = 0; q < W; ++q) {
+          if ((qm[q] >> lane) & 1ull) S[base + mask_rank(qm[q])] = v[q];
+          base += __popcll(qm[q]);
+        }
+        if (lane >= m) S[lane] = TG_KEY_INF;
+        __syncthreads();
+        v[0] = S[lane];
+      }
     }
-    reg_bitonic_merge<W>(c);
+    long long s3 = 0;
+    if (small) {
+      if (m > 0) {
+        uint64_t w1[1] = {v[0]};
+        reg_bitonic<1>(w1);
+        s3 = pclock(a);
+        const uint64_t rv = lane_xor64(w1[0], 63);
+        uint64_t c[1] = {bk[0] < rv ? bk[0] : rv};
+        reg_bitonic_merge<1>(c);
+        bk[0] = lane < TOPM ? c[0] : TG_KEY_INF;
+      } else {
+        s3 = pclock(a);
+      }
+    } else {
+      reg_bitonic<W>(v);
+      s3 = pclock(a);
+      uint64_t c[W];
 #pragma unroll
-    for (int b = 0; b < W; ++b) bk[b] = b * 64 + lane < TOPM ? c[b] : TG_KEY_INF;
+      for (int b = 0; b < W; ++b) {
+        const uint64_t rv = lane_xor64(v[W - 1 - b], 63);  // candidate 64W-1-p
+        c[b] = bk[b] < rv ? bk[b] : rv;
+      }
+      reg_bitonic_merge<W>(c);
+#pragma unroll
+      for (int b = 0; b < W; ++b) bk[b] = b * 64 + lane < TOPM ? c[b] : TG_KEY_INF;
+    }
     __syncthreads();
     const long long s4 = pclock(a);
     ph.sub[1] += s3 - s2;
