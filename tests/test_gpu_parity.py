@@ -161,6 +161,20 @@ def test_prune_parity_small_k_and_odd_sizes():
     _compare(X, G, GD, 1, 1, 1, 30, 64, 0.3)
 
 
+@pytest.mark.parametrize("case", ["int", "unit"])
+def test_prune_parity_noncanonical_list_distances(int_case, unit_case, case):
+    """kNN lists whose stored distances are not the canonical ones (every row shifted by a
+    constant, so the rows stay sorted): the pool then holds one id under two keys, and the
+    filter's rule that invalidates a repeat of the selected id without counting it decides
+    the rows and the evaluation counts (u8 rows with the two-round passes for the integer
+    data, FP32 rows for the unit data)."""
+    X, G, GD = int_case if case == "int" else unit_case
+    GDs = (GD + (np.float32(3.0) if case == "int" else np.float32(0.25))).astype(np.float32)
+    _compare(X, G, GDs, 0, 0, 0, 32, 64, 1.0)
+    _compare(X, G, GDs, 0, 0, 0, 32, 64, 1.2)
+    _compare(X, G, GDs, 1, 1, 1, 32, 64, 0.5)
+
+
 def test_index_files_are_byte_identical(int_case, tmp_path):
     X, G, GD = int_case
     with State(*X.shape) as s:
