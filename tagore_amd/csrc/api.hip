@@ -61,6 +61,18 @@ void dfree(T*& p) {
   p = nullptr;
 }
 
+// The two window events of a build or a prune, destroyed on every return path.
+struct EventPair {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  EventPair() = default;
+  EventPair(const EventPair&) = delete;
+  EventPair& operator=(const EventPair&) = delete;
+  ~EventPair() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  }
+};
+
 struct PB {
   PushBuf d{};
   uint32_t N = 0;
@@ -1040,9 +1052,10 @@ int tg_knn_build(tg_state* s, const tg_knn_params* p) {
   hipStream_t st = s->stream;
   if (ensure_events(s, 2)) return TG_ERR_HIP;
   // window events (separate from the per-section pool)
-  hipEvent_t w0, w1;
-  HIPX(hipEventCreate(&w0));
-  HIPX(hipEventCreate(&w1));
+  EventPair win;
+  HIPX(hipEventCreate(&win.e0));
+  HIPX(hipEventCreate(&win.e1));
+  hipEvent_t w0 = win.e0, w1 = win.e1;
   Sections sec{s};
   sec.next = 0;
   fill_u32(reinterpret_cast<uint32_t*>(s->counters), 0, 2 * CNT_SLOTS * CNT_N, st);
@@ -1148,8 +1161,6 @@ int tg_knn_build(tg_state* s, const tg_knn_params* p) {
   if (int rc = check_launch("GNN-Descent")) return rc;
   float ms = 0.f;
   HIPX(hipEventElapsedTime(&ms, w0, w1));
-  (void)hipEventDestroy(w0);
-  (void)hipEventDestroy(w1);
   s->stats.knn_ms = ms;
   for (double& v : s->stats.knn_kernel_ms) v = 0.0;
   if (sec.collect(s->stats.knn_kernel_ms)) return TG_ERR_HIP;
@@ -1268,9 +1279,10 @@ int tg_prune(tg_state* s, const tg_prune_params* p) {
                            (p->precision == TG_PREC_FP8 && s->D % 16 == 0));
   if (dpg_screen && p->precision == TG_PREC_FP8)
     if (int rc = ensure_prep8(s)) return rc;
-  hipEvent_t w0, w1;
-  HIPX(hipEventCreate(&w0));
-  HIPX(hipEventCreate(&w1));
+  EventPair win;
+  HIPX(hipEventCreate(&win.e0));
+  HIPX(hipEventCreate(&win.e1));
+  hipEvent_t w0 = win.e0, w1 = win.e1;
   Sections sec{s};
   fill_u32(reinterpret_cast<uint32_t*>(s->counters), 0, 2 * CNT_SLOTS * CNT_N, st);
   fill_u32(s->ep_dev, 0, 1, st);
@@ -1325,8 +1337,6 @@ int tg_prune(tg_state* s, const tg_prune_params* p) {
   if (int rc = check_launch("prune")) return rc;
   float ms = 0.f;
   HIPX(hipEventElapsedTime(&ms, w0, w1));
-  (void)hipEventDestroy(w0);
-  (void)hipEventDestroy(w1);
   s->stats.prune_ms = ms;
   for (double& v : s->stats.prune_kernel_ms) v = 0.0;
   if (sec.collect(s->stats.prune_kernel_ms)) return TG_ERR_HIP;
