@@ -434,17 +434,59 @@ __global__ __launch_bounds__(256) void k_localjoin_screen(KnnArgs a) {
 // row job (row 16w + 4g + (t & 3), tile t >> 2) of both row kinds, and lane l owns column
 // job (column l, tile w).
 // (d, id) minimum over the lanes l ^ x for the masks x in [x0, x1) (powers of two): the
-// smallest d, then the smallest id among the lanes holding it (distances are >= 0 or +inf,
-// so this is the minimum of the keys mk_key(d, id)).
+// smallest d, then the smallest id among the lanes holding it.  The exact screen's distances
+// are integers >= 0 or +inf, whose FP32 bits order as unsigned integers, so both minima are
+// integer minima (no NaN handling), and the result is the minimum of the keys mk_key(d, id).
 template <int X0, int X1>
 __device__ __forceinline__ uint64_t key_min_lanes(float d, uint32_t id) {
-  float m = d;
+  const uint32_t db = __float_as_uint(d);
+  uint32_t m = db;
 #pragma unroll
-  for (int x = X0; x < X1; x <<= 1) m = fminf(m, __uint_as_float(lane_xor32(__float_as_uint(m), x)));
-  uint32_t k = d == m ? id : 0xFFFFFFFFu;
+  for (int x = X0; x < X1; x <<= 1) m = min(m, lane_xor32(m, x));
+  uint32_t k = db == m ? id : 0xFFFFFFFFu;
 #pragma unroll
   for (int x = X0; x < X1; x <<= 1) k = min(k, lane_xor32(k, x));
-  return mk_key(m, k);
+  return (uint64_t(m) << 32) | k;
+}
+
+// Membership of Q keys in kNN rows (rows sorted by (distance, id), canonical distances):
+// found[q] = key[q] is an entry of row v[q], for the searches with on[q].  The binary
+// searches run in lockstep, so the dependent loads of one step are in flight together.
+template <int Q>
+__device__ __forceinline__ void rows_have_keys(const uint32_t* G, const float* GD, int K,
+                                               const uint32_t (&v)[Q], const uint64_t (&key)[Q],
+                                               const bool (&on)[Q], bool (&found)[Q]) {
+  int lo[Q], hi[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    lo[q] = 0;
+    hi[q] = on[q] ? K : 0;
+  }
+  for (;;) {
+    bool any = false;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) any = any || lo[q] < hi[q];
+    if (!any) break;
+    uint64_t kk[Q];
+    int mid[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      mid[q] = (lo[q] + hi[q]) >> 1;
+      kk[q] = lo[q] < hi[q] ? mk_key(GD[size_t(v[q]) * K + mid[q]], G[size_t(v[q]) * K + mid[q]])
+                            : 0ull;
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      if (lo[q] < hi[q]) {
+        if (kk[q] < key[q]) lo[q] = mid[q] + 1;
+        else hi[q] = mid[q];
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < Q; ++q)
+    found[q] = on[q] && lo[q] < K &&
+               mk_key(GD[size_t(v[q]) * K + lo[q]], G[size_t(v[q]) * K + lo[q]]) == key[q];
 }
 
 template <int PREC, int DC>
@@ -505,19 +547,21 @@ void k_localjoin_exact_screen(KnnArgs a) {
     if (g == c) col_b = kc;
   }
   // pushes: a job's minimum enters when it beats the destination's worst entry and is not
-  // already listed (the merge would drop it)
+  // already listed (the merge would drop it); the three row searches run together
   {
     const int i = 16 * w + 4 * g + (t & 3);
-    if (i < nA) {
-      const uint32_t dest = A[i];
-      const float worst = wA[i];
-      if (key_d(row_b) < worst && !row_has_key(a.G, a.GD, K, dest, row_b)) pb_push(a.pc, dest, row_b);
-      if (key_d(row_a) < worst && !row_has_key(a.G, a.GD, K, dest, row_a)) pb_push(a.pc, dest, row_a);
-    }
-    if (l < nB) {
-      const uint32_t dest = B[l];
-      if (key_d(col_b) < wB[l] && !row_has_key(a.G, a.GD, K, dest, col_b)) pb_push(a.pc, dest, col_b);
-    }
+    const bool inA = i < nA, inB = l < nB;
+    const uint32_t destA = inA ? A[i] : 0u, destB = inB ? B[l] : 0u;
+    const float worstA = inA ? wA[i] : 0.f, worstB = inB ? wB[l] : 0.f;
+    const uint32_t dv[3] = {destA, destA, destB};
+    const uint64_t kv[3] = {row_b, row_a, col_b};
+    const bool on[3] = {inA && key_d(row_b) < worstA, inA && key_d(row_a) < worstA,
+                        inB && key_d(col_b) < worstB};
+    bool found[3];
+    rows_have_keys<3>(a.G, a.GD, K, dv, kv, on, found);
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+      if (on[q] && !found[q]) pb_push(a.pc, dv[q], kv[q]);
   }
   if (a.counters && tid == 0)
     atomicAdd(&cnt_slot(a.counters)[0], (unsigned long long)(nA * nB + nA * (nA - 1) / 2));
