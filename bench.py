@@ -40,11 +40,11 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def measure(st, args, precision, iters, init, label):
+def measure(st, args, precision, iters, init, label, graph_iters=False):
     import tagore_amd
 
     def step():
-        st.knn_build(args.k, iters, args.seed, precision, init=init)
+        st.knn_build(args.k, iters, args.seed, precision, init=init, graph_iters=graph_iters)
         st.prune("path", "dist", "dist", args.fd, args.topm, 1.0,
                  args.prune_rows if precision == "bf16" else "fp32")
 
@@ -416,6 +416,9 @@ def main():
             ablation = measure(st, args, args.precision, args.ablation_iters, "random",
                                "ablation")
     head = measure(st, args, args.precision, args.iters, args.init, "headline")
+    # the same step with the GNN-Descent iterations replayed from a captured HIP graph
+    head_graph = measure(st, args, args.precision, args.iters, args.init,
+                         "headline, graph-captured iterations", graph_iters=True)
     fdeg, _, _ = st.get_forward()
     fwd_edges = int(fdeg.astype(np.int64).sum())
     if args.phase_clocks:
@@ -535,6 +538,9 @@ def main():
         "index_search_recall_at_10": next((r["recall_at_10"] for r in search["sweep"]
                                            if r["L"] == 64), None) if search else None,
         "headline": head,
+        "headline_graph_iterations": {k: head_graph[k] for k in
+                                      ("ms_per_step", "vectors_per_s", "knn_ms", "prune_ms",
+                                       "recall_at_10")},
         "plain_fp32": plain,
         "plain_fp32_at_target": plain_target,
         "ablation_random_bf16": ablation,

@@ -70,6 +70,13 @@ typedef struct {
  * measured slower on MI355X with ROCm 7.2 (26.7 ms vs 21.7 ms per 1M-vector build), so
  * off by default. */
 #define TG_FLAG_GRAPH 1u
+/* Capture the iteration sequence after the initial lists (local-join iterations, reverse
+ * lists, expansion iterations, final sort: every kernel of them, all clusters of a segmented
+ * handle included) as one HIP graph, and replay it in later builds whose launch arguments
+ * are the same (handle buffers, K, iterations, precision, tile, segments).  Same graph as
+ * individual launches; the per-kernel section times of tg_stats stay 0 for those builds
+ * (the window time is measured). */
+#define TG_FLAG_GRAPH_ITERS 2u
 
 typedef struct {
   int32_t kind;      /* TG_KIND_* */

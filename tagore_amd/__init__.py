@@ -230,9 +230,12 @@ class State:
     # kNN
     def knn_build(self, K: int, iters: int, seed: int = 0x7A600000, precision: str = "exact",
                   init: str = "random", sample: int = 0, epochs: int = 0, tile: int = 0,
-                  graph: bool = False):
+                  graph: bool = False, graph_iters: bool = False):
+        """GNN-Descent.  graph: capture the encoder's training loop as a HIP graph;
+        graph_iters: capture the iteration sequence as a HIP graph (replayed by later builds
+        with the same launch arguments)."""
         p = KnnParams(K, iters, seed, PREC[precision], INIT[init], sample, epochs, tile,
-                      1 if graph else 0)
+                      (1 if graph else 0) | (2 if graph_iters else 0))
         _check(lib().tg_knn_build(self._h, ctypes.byref(p)))
         self.K = K
 

@@ -172,6 +172,9 @@ struct tg_state {
   int profiling = 0;  // 1: per-kernel events, 2: also prune phase clocks
   tg_stats stats{};
   std::vector<hipEvent_t> ev_pool;
+  // captured iteration sequences of tg_knn_build (TG_FLAG_GRAPH_ITERS), keyed by their
+  // launch arguments; two entries cover the G / G1 alternation of odd expansion counts
+  std::vector<std::pair<std::vector<unsigned char>, hipGraphExec_t>> iter_graphs;
 };
 
 namespace {
@@ -437,6 +440,8 @@ int tg_free(tg_state* s) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   tg_free(s->sub);
   s->sub = nullptr;
+  for (auto& g : s->iter_graphs) (void)hipGraphExecDestroy(g.second);
+  s->iter_graphs.clear();
   learner_release(s->learn);
   dfree(s->samp);
   dfree(s->seg_of);
@@ -1009,6 +1014,14 @@ int learned_init(tg_state* s, const tg_knn_params* p, uint64_t* evals) {
   return TG_OK;
 }
 
+// the list buffers of one expansion iteration trade places (new lists become current)
+void swap_lists(tg_state* s) {
+  std::swap(s->G, s->G1);
+  std::swap(s->GD, s->GD1);
+  std::swap(s->R0, s->R1);
+  std::swap(s->Rn0, s->Rn1);
+}
+
 }  // namespace
 
 int tg_knn_build(tg_state* s, const tg_knn_params* p) {
@@ -1108,53 +1121,109 @@ int tg_knn_build(tg_state* s, const tg_knn_params* p) {
   }
   if (sec.stop()) return TG_ERR_HIP;
   const uint32_t it1 = p->iters / 2, it2 = p->iters / 2;
-  for (uint32_t it = 0; it < it1; ++it) {
-    if (sec.start(1)) return TG_ERR_HIP;
-    pb_reset(a.pn, N, s->fill, st);
-    pb_reset(a.po, N, s->fill, st);
-    launch_knn_sample(a, st);
-    if (sec.stop()) return TG_ERR_HIP;
-    if (sec.start(2)) return TG_ERR_HIP;
-    pb_group_overflow(a.pn, N, s->fill, s->scan_tmp, st);
-    fill_u32(s->fill, 0, N, st);
-    pb_group_overflow(a.po, N, s->fill, s->scan_tmp, st);
-    launch_knn_rsample(a, st);
-    if (sec.stop()) return TG_ERR_HIP;
-    if (sec.start(3)) return TG_ERR_HIP;
-    pb_reset(a.pc, N, s->fill, st);
-    launch_knn_localjoin(a, p->precision, int(p->tile), st);
-    if (sec.stop()) return TG_ERR_HIP;
-    if (sec.start(4)) return TG_ERR_HIP;
-    pb_group_overflow(a.pc, N, s->fill, s->scan_tmp, st);
-    launch_knn_merge(a, st);
-    if (sec.stop()) return TG_ERR_HIP;
-    if (int rc = check_launch("GNN-Descent phase 1")) return rc;
-  }
-  if (p->iters > 0) {  // iters == 0 returns the initial lists
-    if (sec.start(5)) return TG_ERR_HIP;
+  // The iteration sequence.  `timed`: per-kernel sections (not while capturing a graph).
+  // It ends with the handle's list buffers swapped once per expansion iteration.
+  auto run_iters = [&](bool timed) -> int {
+    Sections none{s};
+    Sections& sc = timed ? sec : none;
+    const int prof = s->profiling;
+    if (!timed) s->profiling = 0;
+    struct Restore {
+      tg_state* s;
+      int prof;
+      ~Restore() { s->profiling = prof; }
+    } restore{s, prof};
+    for (uint32_t it = 0; it < it1; ++it) {
+      if (sc.start(1)) return TG_ERR_HIP;
+      pb_reset(a.pn, N, s->fill, st);
+      pb_reset(a.po, N, s->fill, st);
+      launch_knn_sample(a, st);
+      if (sc.stop()) return TG_ERR_HIP;
+      if (sc.start(2)) return TG_ERR_HIP;
+      pb_group_overflow(a.pn, N, s->fill, s->scan_tmp, st);
+      fill_u32(s->fill, 0, N, st);
+      pb_group_overflow(a.po, N, s->fill, s->scan_tmp, st);
+      launch_knn_rsample(a, st);
+      if (sc.stop()) return TG_ERR_HIP;
+      if (sc.start(3)) return TG_ERR_HIP;
+      pb_reset(a.pc, N, s->fill, st);
+      launch_knn_localjoin(a, p->precision, int(p->tile), st);
+      if (sc.stop()) return TG_ERR_HIP;
+      if (sc.start(4)) return TG_ERR_HIP;
+      pb_group_overflow(a.pc, N, s->fill, s->scan_tmp, st);
+      launch_knn_merge(a, st);
+      if (sc.stop()) return TG_ERR_HIP;
+      if (int rc = check_launch("GNN-Descent phase 1")) return rc;
+    }
+    if (p->iters == 0) return TG_OK;  // iters == 0 returns the initial lists
+    if (sc.start(5)) return TG_ERR_HIP;
     if (launch_knn_rev_build(a, s->rv_hist, s->rv_scan, st))
       return fail(TG_ERR_ARG, "reverse lists: N and K beyond 2^26 ids or 2^32 edges");
     KnnArgs ar = a;
     ar.pr.cap = 0;  // the runs are the grouped overflow area
     launch_knn_rev_reduce(ar, st);
-    if (sec.stop()) return TG_ERR_HIP;
+    if (sc.stop()) return TG_ERR_HIP;
     for (uint32_t it = 0; it < it2; ++it) {
-      if (sec.start(6)) return TG_ERR_HIP;
+      if (sc.start(6)) return TG_ERR_HIP;
       pb_reset(a.pr, N, s->fill, st);
       launch_knn_phase2(a, p->precision, st);
       pb_group_overflow(a.pr, N, s->fill, s->scan_tmp, st);
       launch_knn_rupdate(a, st);
-      if (sec.stop()) return TG_ERR_HIP;
-      std::swap(s->G, s->G1);
-      std::swap(s->GD, s->GD1);
-      std::swap(s->R0, s->R1);
-      std::swap(s->Rn0, s->Rn1);
+      if (sc.stop()) return TG_ERR_HIP;
+      swap_lists(s);
       a = args();
       if (int rc = check_launch("GNN-Descent phase 2")) return rc;
     }
-    if (sec.start(7)) return TG_ERR_HIP;
+    if (sc.start(7)) return TG_ERR_HIP;
     launch_knn_final(a, st);
-    if (sec.stop()) return TG_ERR_HIP;
+    if (sc.stop()) return TG_ERR_HIP;
+    return TG_OK;
+  };
+  if ((p->flags & TG_FLAG_GRAPH_ITERS) && p->iters > 0) {
+    // key: every launch argument of the sequence
+    std::vector<unsigned char> key;
+    auto put = [&](const void* v, size_t n) {
+      const unsigned char* b = static_cast<const unsigned char*>(v);
+      key.insert(key.end(), b, b + n);
+    };
+    put(&a, sizeof(a));
+    const uint64_t extra[] = {it1, it2, uint64_t(p->precision), p->tile,
+                              uint64_t(reinterpret_cast<uintptr_t>(s->fill)),
+                              uint64_t(reinterpret_cast<uintptr_t>(s->scan_tmp)),
+                              uint64_t(reinterpret_cast<uintptr_t>(s->rv_hist)),
+                              uint64_t(reinterpret_cast<uintptr_t>(s->rv_scan)), s->nseg};
+    put(extra, sizeof(extra));
+    put(s->seg_lo_host.data(), s->seg_lo_host.size() * 4);
+    hipGraphExec_t exec = nullptr;
+    for (auto& g : s->iter_graphs)
+      if (g.first == key) exec = g.second;
+    if (exec) {
+      for (uint32_t it = 0; it < it2; ++it) swap_lists(s);  // what the sequence leaves
+      a = args();
+    } else {
+      hipGraph_t g = nullptr;
+      HIPX(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      const int rc = run_iters(false);
+      const hipError_t ec = hipStreamEndCapture(st, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (ec != hipSuccess)
+        return fail(TG_ERR_HIP, std::string("capture of the iterations: ") + hipGetErrorString(ec));
+      const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ei != hipSuccess)
+        return fail(TG_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+      if (s->iter_graphs.size() >= 2) {
+        (void)hipGraphExecDestroy(s->iter_graphs.front().second);
+        s->iter_graphs.erase(s->iter_graphs.begin());
+      }
+      s->iter_graphs.push_back({std::move(key), exec});
+    }
+    HIPX(hipGraphLaunch(exec, st));
+  } else {
+    if (int rc = run_iters(true)) return rc;
   }
   HIPX(hipEventRecord(w1, st));
   HIPX(hipStreamSynchronize(st));
