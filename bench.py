@@ -160,68 +160,113 @@ def weak_components(deg, ids):
     return int(n), float(np.bincount(lab).max() / N)
 
 
-def bench_clusters(args):
-    """One GPU's share of the clustered pipeline: uneven clusters (log-normal sizes) of unit
-    Gaussian-mixture vectors, each built (GNN-Descent 8+8 iterations, K=64) and pruned
-    (Vamana alpha 1.2, FD=32, m=64).  Batched: every cluster in the same launches
-    (segmented handle, autotuned kernel variant).  Per cluster: one handle and one launch
-    sequence per cluster, the reference's loop (large_index.cu:173-277).  Times are device
-    windows of the builds and prunes; the two produce identical graphs."""
+def write_cluster_files(d, nclusters, median, D, seed):
+    """The clustered large-index input at one GPU's share: cluster sizes (rows) log-normal
+    around `median` (sigma 0.945, so the mean is 1.5625x the median), clipped to
+    [median/8, 5 x median]; every point lies in two clusters (a primary and a secondary one,
+    as the two-centre file of pybind.cu:311-355 lists them), so the points number half the
+    rows.  Vectors: unit-norm, around their primary cluster's centre.  Writes the
+    reference's files: cluster_<c> (u32 count, then the members' rows in ascending point
+    order) and centres.bin (u32 first, u32 second centre per point)."""
     import numpy as np
+
+    rng = np.random.default_rng(seed)
+    size = np.clip(np.exp(rng.normal(np.log(median), 0.945, nclusters)), median / 8, median * 5)
+    prim = np.maximum((size / 2).astype(np.int64), 1)
+    npts = int(prim.sum())
+    cen = np.repeat(np.arange(nclusters, dtype=np.uint32), prim)
+    cen = cen[rng.permutation(npts)]
+    # secondary centre: drawn in proportion to the clusters' primaries, never the primary
+    cen2 = rng.choice(nclusters, npts, p=prim / npts).astype(np.uint32)
+    while True:
+        same = np.nonzero(cen2 == cen)[0]
+        if same.size == 0:
+            break
+        cen2[same] = rng.choice(nclusters, same.size, p=prim / npts).astype(np.uint32)
+    centres = rng.standard_normal((nclusters, D)).astype(np.float32)
+    centres /= np.linalg.norm(centres, axis=1, keepdims=True)
+    X = np.empty((npts, D), np.float32)
+    for a in range(0, npts, 1 << 20):
+        b = min(npts, a + (1 << 20))
+        x = 3.0 * centres[cen[a:b]] + rng.standard_normal((b - a, D), dtype=np.float32)
+        X[a:b] = x / np.linalg.norm(x, axis=1, keepdims=True)
+    np.stack([cen, cen2], 1).tofile(os.path.join(d, "centres.bin"))
+    rows = 0
+    for c in range(nclusters):
+        mem = np.nonzero((cen == c) | (cen2 == c))[0]
+        with open(os.path.join(d, f"cluster_{c}"), "wb") as f:
+            np.asarray([mem.size], np.uint32).tofile(f)
+            X[mem].tofile(f)
+        rows += mem.size
+    return npts, rows, int(size.min()), int(size.max())
+
+
+def bench_clusters(args):
+    """One GPU's share of the clustered large-index pipeline through the drop-in
+    largeIndex([0], ...) (pybind.cu:297-391): 64 clusters with log-normal sizes (median
+    400k rows, ~40M rows in total, 96-dim unit vectors), per cluster GNN-Descent K=64 with
+    8+8 iterations and Vamana (alpha 1.2, FD=32, m=64), then the streaming union merge into
+    one index of max degree 22.  Batched: every cluster of a batch (batches sized from free
+    device memory) in the same launches, with the kernel variant the autotuner picks on
+    the first batch.  Per cluster: one launch sequence per cluster (the reference's loop,
+    large_index.cu:173-277), same variant.  Both must write byte-identical index files."""
+    import filecmp
+    import shutil
 
     import tagore_amd
 
-    rng = np.random.default_rng(0x7A600004)
-    sizes = np.exp(rng.normal(np.log(args.cluster_median), 0.95, args.clusters))
-    sizes = np.clip(sizes, args.cluster_median / 8, args.cluster_median * 5).astype(np.int64)
-    starts = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint32)
-    N, D, K = int(starts[-1]), 96, 64
-    log(f"clusters: {args.clusters}, {N} vectors, sizes {sizes.min()}..{sizes.max()}")
-    st = tagore_amd.State(N, D, 0)
-    st.gen_data(1, 512, 0x7A600004, 1.0)
-    st.set_segments(starts)
+    d = args.cluster_dir
+    os.makedirs(d, exist_ok=True)
+    need = int(args.cluster_median * 1.6 * args.clusters * 96 * 4 * 1.6)
+    free = shutil.disk_usage(d).free
+    if free < need:
+        print(f"bench.py: {d} has {free / 1e9:.1f} GB free, the cluster files need about "
+              f"{need / 1e9:.1f} GB", file=sys.stderr)
+        sys.exit(2)
     t = time.perf_counter()
-    (prec, tile), cands = st.autotune(K, 0x51)
-    tune_s = time.perf_counter() - t
-    log(f"autotune ({tune_s:.2f} s): {cands} -> {prec} tile {tile}")
-
-    def batched():
-        st.knn_build(K, 16, 0x51, prec, tile=tile)
-        a = st.stats()["knn_ms"]
-        st.prune("path", "dist", "dist", 32, 64, 1.2)
-        return a + st.stats()["prune_ms"]
-
-    batched()
-    t_b = [batched() for _ in range(args.steps)]
-    X = None
-    t_c = []
-    for c in range(min(args.clusters, args.per_cluster_limit)):
-        lo, hi = int(starts[c]), int(starts[c + 1])
-        rows = st.get_data(lo, hi - lo)
-        with tagore_amd.State(hi - lo, D, 0) as sc:
-            sc.set_data(rows)
-            sc.set_segments(np.array([0, hi - lo], np.uint32), keys=[c])
-            sc.knn_build(K, 16, 0x51, prec, tile=tile)
-            ms = sc.stats()["knn_ms"]
-            sc.prune("path", "dist", "dist", 32, 64, 1.2)
-            t_c.append(ms + sc.stats()["prune_ms"])
-    st.close()
-    done = min(args.clusters, args.per_cluster_limit)
-    per_cluster_ms = sum(t_c) * (N / float(starts[done]))  # scaled to all clusters
-    batched_ms = sum(t_b) / len(t_b)
+    npts, rows, smin, smax = write_cluster_files(d, args.clusters, args.cluster_median, 96,
+                                                 0x7A600004)
+    gen_s = time.perf_counter() - t
+    log(f"clusters: {args.clusters} files, {rows} rows ({npts} points), cluster sizes "
+        f"{smin}..{smax}, written in {gen_s:.1f}s")
+    K, FD, TOPM, md, B, thre = 64, 32, 64, 22, 4, 1.2
+    common = dict(DIM=96, K=K, cluster_num=args.clusters, max_points=10 ** 8,
+                  file_prefix=os.path.join(d, "cluster_"),
+                  local_index_path_prefix=os.path.join(d, "local_"),
+                  max_degree=md, buffer_size=B, all_node_num=npts,
+                  centers_path=os.path.join(d, "centres.bin"), thre=thre, FINAL_DEGREE=FD,
+                  TOPM=TOPM, iters=16, verbose=True)
+    idx_b = os.path.join(d, "index_batched.bin")
+    idx_c = os.path.join(d, "index_per_cluster.bin")
+    log("largeIndex: batched launches (autotuned variant)")
+    sb = tagore_amd.large_index(index_store_path=idx_b, precision="bf16", batch_rows=0,
+                                tune=True, **common)
+    log(f"batched: {sb}")
+    log("largeIndex: one launch sequence per cluster")
+    sc = tagore_amd.large_index(index_store_path=idx_c, precision=sb["precision"],
+                                tile=sb["tile"], batch_rows=1, tune=False, **common)
+    log(f"per cluster: {sc}")
+    same = filecmp.cmp(idx_b, idx_c, shallow=False)
+    for f in os.listdir(d):
+        os.remove(os.path.join(d, f))
     out = {
-        "metric": "clustered kNN+prune vectors/s, batched launches vs one launch per cluster",
-        "value": N / (batched_ms / 1e3), "unit": "vectors/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": 1, "ms_per_step": batched_ms, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": prec, "data": "synthetic",
-        "config": {"workload": f"{args.clusters} log-normal clusters (median "
-                               f"{args.cluster_median}), {N}x{D} unit GMM(512); GNN-Descent K=64 "
-                               "8+8 iterations + Vamana alpha 1.2 FD=32 m=64",
-                   "clusters": args.clusters, "N": N, "dim": D, "K": K},
-        "per_cluster_ms": per_cluster_ms,
-        "per_cluster_measured": done,
-        "speedup_vs_per_cluster": per_cluster_ms / batched_ms,
-        "autotune": {"choice": [prec, tile], "candidates": cands, "seconds": tune_s},
+        "metric": "clustered large-index rows/s (GNN-Descent + Vamana per cluster + merge), "
+                  "batched launches vs one launch sequence per cluster, 1 GPU",
+        "value": rows / (sb["device_ms"] / 1e3), "unit": "rows/s", "n_gpus": 1, "steps": 1,
+        "warmup": 0, "ms_per_step": sb["device_ms"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": sb["precision"], "data": "synthetic",
+        "config": {"workload": f"one GPU's cluster share: {args.clusters} log-normal clusters "
+                               f"(median {args.cluster_median} rows, {smin}..{smax}), {rows} rows "
+                               f"of {npts} 96-dim unit vectors; GNN-Descent K=64 8+8 "
+                               f"iterations + Vamana alpha 1.2 FD=32 m=64, merge max_degree 22",
+                   "clusters": args.clusters, "rows": rows, "points": npts, "dim": 96, "K": K},
+        "value_definition": "rows over the summed device windows of the batched builds and "
+                            "prunes; the host I/O and the merge are reported beside it",
+        "batched": sb, "per_cluster": sc,
+        "device_speedup_vs_per_cluster": sc["device_ms"] / sb["device_ms"],
+        "build_speedup_vs_per_cluster": sc["build_ms"] / sb["build_ms"],
+        "index_files_identical": same,
+        "input_generation_s": gen_s,
     }
     print(json.dumps(out), flush=True)
 
@@ -360,12 +405,14 @@ def main():
                     help="vectors in the CPU-oracle baseline sample (0 skips it)")
     ap.add_argument("--workload", default="headline",
                     choices=["headline", "clusters", "unit", "wide"],
-                    help="clusters: batched multi-cluster build (64 log-normal clusters); "
+                    help="clusters: the large-index pipeline on one GPU's cluster share; "
                          "unit: 1M x 96 unit vectors + NSSG; wide: 5M x 960 FP8 + DPG/CAGRA")
     ap.add_argument("--clusters", type=int, default=64)
-    ap.add_argument("--cluster-median", type=int, default=40000)
-    ap.add_argument("--per-cluster-limit", type=int, default=64,
-                    help="clusters timed one by one (the rest is scaled by row count)")
+    ap.add_argument("--cluster-median", type=int, default=400000,
+                    help="median cluster size in rows (clusters workload)")
+    ap.add_argument("--cluster-dir", default=os.path.join(os.environ.get("TMPDIR", "/tmp"),
+                                                          "tagore_clusters"),
+                    help="scratch directory of the clusters workload's files")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))

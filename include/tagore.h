@@ -203,12 +203,14 @@ typedef struct {
   int32_t verbose;
   int32_t tune;                    /* 1: tg_autotune_knn on the first batch picks precision
                                       and tile for every batch (same graphs, less time) */
+  uint32_t tile;                   /* local-join tile of every build when tune == 0 */
 } tg_large_params;
 typedef struct {
   uint32_t clusters, batches;      /* after splitting; batched builds */
   double build_ms, merge_ms, total_ms;
   int32_t precision;               /* precision and tile the builds used */
   uint32_t tile;
+  double device_ms;                /* summed device windows of the builds and prunes */
 } tg_large_stats;
 int tg_large_index(const tg_large_params* p, tg_large_stats* stats);
 

@@ -79,14 +79,14 @@ class LargeParams(ctypes.Structure):
                 ("thre", ctypes.c_float), ("FINAL_DEGREE", ctypes.c_uint32),
                 ("TOPM", ctypes.c_uint32), ("iters", ctypes.c_uint32), ("seed", ctypes.c_uint64),
                 ("precision", ctypes.c_int32), ("batch_rows", ctypes.c_uint64),
-                ("verbose", ctypes.c_int32), ("tune", ctypes.c_int32)]
+                ("verbose", ctypes.c_int32), ("tune", ctypes.c_int32), ("tile", ctypes.c_uint32)]
 
 
 class LargeStats(ctypes.Structure):
     _fields_ = [("clusters", ctypes.c_uint32), ("batches", ctypes.c_uint32),
                 ("build_ms", ctypes.c_double), ("merge_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double), ("precision", ctypes.c_int32),
-                ("tile", ctypes.c_uint32), ("flags", ctypes.c_uint32)]
+                ("tile", ctypes.c_uint32), ("device_ms", ctypes.c_double)]
 
 
 class SearchParams(ctypes.Structure):
@@ -380,20 +380,20 @@ class State:
 def large_index(DIM, K, cluster_num, max_points, file_prefix, local_index_path_prefix,
                 index_store_path, max_degree, buffer_size, all_node_num, centers_path, thre,
                 FINAL_DEGREE, TOPM, device=0, seed=0x7A600000, precision="bf16", batch_rows=0,
-                iters=16, verbose=False, tune=False):
+                iters=16, verbose=False, tune=False, tile=0):
     """tg_large_index through ctypes; returns the stats as a dict."""
     dev = (ctypes.c_int32 * 1)(device)
     p = LargeParams(dev, 1, DIM, K, cluster_num, max_points, file_prefix.encode(),
                     local_index_path_prefix.encode(), index_store_path.encode(), max_degree,
                     buffer_size, all_node_num, centers_path.encode(), thre, FINAL_DEGREE, TOPM,
                     iters, seed, PREC[precision], batch_rows, 1 if verbose else 0,
-                    1 if tune else 0)
+                    1 if tune else 0, tile)
     st = LargeStats()
     _check(lib().tg_large_index(ctypes.byref(p), ctypes.byref(st)))
     inv = {v: k for k, v in PREC.items()}
     return dict(clusters=st.clusters, batches=st.batches, build_ms=st.build_ms,
                 merge_ms=st.merge_ms, total_ms=st.total_ms, precision=inv[st.precision],
-                tile=st.tile)
+                tile=st.tile, device_ms=st.device_ms)
 
 
 def segment_seed(seed: int, c: int) -> int:

@@ -128,6 +128,7 @@ extern "C" int tg_large_index(const tg_large_params* p, tg_large_stats* out_stat
   // with one slot the reference never releases the second cluster's slot (its builder waits
   // on has_merged forever, large_index.cu:255 / 398-412)
   if (p->buffer_size < 2) return lfail(TG_ERR_ARG, "buffer_size must be at least 2");
+  if (p->tile > 1) return lfail(TG_ERR_ARG, "tile must be 0 or 1");
   if (!p->file_prefix || !p->local_index_path_prefix || !p->index_store_path || !p->centers_path)
     return lfail(TG_ERR_ARG, "null path");
   const uint32_t Npts = p->all_node_num;
@@ -204,8 +205,9 @@ extern "C" int tg_large_index(const tg_large_params* p, tg_large_stats* out_stat
   }
   const auto t_build = clk::now();
   uint32_t nbatches = 0;
+  double device_ms = 0.0;
   int32_t precision = p->precision;
-  uint32_t tile = 0;
+  uint32_t tile = p->tile;
   bool tuned = !p->tune;
   for (uint32_t b0 = 0; b0 < C;) {
     uint32_t b1 = b0;
@@ -267,6 +269,11 @@ extern "C" int tg_large_index(const tg_large_params* p, tg_large_stats* out_stat
     pp.thre = p->thre;
     pp.precision = TG_PREC_FP32;
     if ((rc = tg_prune(s, &pp)) != TG_OK) return bail(rc);
+    {
+      tg_stats ts{};
+      if ((rc = tg_get_stats(s, &ts)) != TG_OK) return bail(rc);
+      device_ms += ts.knn_ms + ts.prune_ms;
+    }
     std::vector<uint32_t> deg(rows), ids(rows * FD), eps(b1 - b0);
     uint32_t ep0 = 0;
     if ((rc = tg_get_pruned(s, deg.data(), ids.data(), &ep0)) != TG_OK) return bail(rc);
@@ -444,6 +451,7 @@ extern "C" int tg_large_index(const tg_large_params* p, tg_large_stats* out_stat
     out_stats->total_ms = std::chrono::duration<double, std::milli>(t_end - t_start).count();
     out_stats->precision = precision;
     out_stats->tile = tile;
+    out_stats->device_ms = device_ms;
   }
   return TG_OK;
 }

@@ -253,7 +253,8 @@ void largeIndex(std::vector<int> devicelist, unsigned DIM, unsigned K, unsigned 
     check(tg_large_index(&p, &st));
   }
   std::cout << "Clusters: " << st.clusters << " in " << st.batches << " batched builds; build "
-            << st.build_ms / 1000.0 << "s, merge " << st.merge_ms / 1000.0 << "s" << std::endl;
+            << st.build_ms / 1000.0 << "s (device " << st.device_ms / 1000.0 << "s), merge "
+            << st.merge_ms / 1000.0 << "s" << std::endl;
 }
 
 }  // namespace
