@@ -785,28 +785,49 @@ __device__ __forceinline__ float quad_dist_u8(const Row16<MC>& a, const uint4 (&
   return float(a.n2 + quad_sum(int(bb) - 2 * int(ab)));
 }
 
-// Two u8 targets at once: each candidate row is read once and gives d(ta, c) and d(tb, c);
-// one candidate per quad, 16 per step (item q0 + lane / 4).  emit(q, da, db) runs in lanes
-// 0..15 for the items q0 + lane.
+// d(ta, y) and d(tb, y) for the loaded row y (quad layout), in every quad lane.
+template <int MC>
+__device__ __forceinline__ void quad_dist2_u8(const Row16<MC>& ta, const Row16<MC>& tb,
+                                              const uint4 (&y)[MC], int D, float& da, float& db) {
+  uint32_t aa = 0, ab = 0, bb = 0;
+#pragma unroll
+  for (int m = 0; m < MC; ++m) {
+    if (m * 64 >= D) break;
+    aa = dot16_u8(as_u4(ta.v[m]), y[m], aa);
+    ab = dot16_u8(as_u4(tb.v[m]), y[m], ab);
+    bb = dot16_u8(y[m], y[m], bb);
+  }
+  da = float(ta.n2 + quad_sum(int(bb) - 2 * int(aa)));
+  db = float(tb.n2 + quad_sum(int(bb) - 2 * int(ab)));
+}
+
+// Two u8 targets at once: each candidate row is read once and gives d(ta, c) and d(tb, c).
+// One candidate per quad, 32 per step while more than 16 remain, then 16 (as eval_batch);
+// emit(q, da, db) runs in lanes 0..31 (0..15) for the items q0 + lane.
 template <int MC, class IdOf, class Emit>
 __device__ __forceinline__ void eval_batch2_u8(const uint8_t* X, int D, const Row16<MC>& ta,
                                                const Row16<MC>& tb, int n, IdOf id_of,
                                                Emit emit) {
   const int lane = threadIdx.x, c = lane >> 2, src = (lane & 15) * 4;
-  for (int q0 = 0; q0 < n; q0 += 16) {
+  int q0 = 0;
+  for (; q0 + 16 < n; q0 += 32) {
+    const int qb = q0 + 16 + c;
+    uint4 ya[MC], yb[MC];
+    load_quad_u8<MC>(ya, X + size_t(id_of(q0 + c)) * D, D);  // q0 + c < q0 + 16 < n
+    load_quad_u8<MC>(yb, qb < n ? X + size_t(id_of(qb)) * D : X, D);
+    float da0, db0, da1, db1;
+    quad_dist2_u8<MC>(ta, tb, ya, D, da0, db0);
+    quad_dist2_u8<MC>(ta, tb, yb, D, da1, db1);
+    const float xa0 = __shfl(da0, src, TG_WAVE), xb0 = __shfl(db0, src, TG_WAVE);
+    const float xa1 = __shfl(da1, src, TG_WAVE), xb1 = __shfl(db1, src, TG_WAVE);
+    if (lane < 32 && q0 + lane < n) emit(q0 + lane, lane < 16 ? xa0 : xa1, lane < 16 ? xb0 : xb1);
+  }
+  if (q0 < n) {
     const int q = q0 + c;
     uint4 y[MC];
     load_quad_u8<MC>(y, q < n ? X + size_t(id_of(q)) * D : X, D);  // unused quads read row 0
-    uint32_t aa = 0, ab = 0, bb = 0;
-#pragma unroll
-    for (int m = 0; m < MC; ++m) {
-      if (m * 64 >= D) break;
-      aa = dot16_u8(as_u4(ta.v[m]), y[m], aa);
-      ab = dot16_u8(as_u4(tb.v[m]), y[m], ab);
-      bb = dot16_u8(y[m], y[m], bb);
-    }
-    const float da = float(ta.n2 + quad_sum(int(bb) - 2 * int(aa)));
-    const float db = float(tb.n2 + quad_sum(int(bb) - 2 * int(ab)));
+    float da, db;
+    quad_dist2_u8<MC>(ta, tb, y, D, da, db);
     const float va = __shfl(da, src, TG_WAVE), vb = __shfl(db, src, TG_WAVE);
     if (lane < 16 && q0 + lane < n) emit(q0 + lane, va, vb);
   }
