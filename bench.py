@@ -543,12 +543,26 @@ def main():
     for name, (kind_, fr, rr, th) in {"Vamana": ("path", "dist", "dist", 1.2),
                                       "CAGRA": ("cagra", "dist", "rank", 0.0)}.items():
         ms = []
+        rows = args.prune_rows if args.precision == "bf16" else "fp32"
         for _ in range(max(1, args.steps)):
-            st.prune(kind_, fr, rr, args.fd, args.topm, th,
-                     args.prune_rows if args.precision == "bf16" else "fp32")
+            st.prune(kind_, fr, rr, args.fd, args.topm, th, rows)
             ms.append(st.stats()["prune_ms"])
-        other_prunes[name] = {"prune_ms": min(ms), "step_ms": head["knn_ms"] + min(ms)}
-        log(f"prune {name}: {min(ms):.1f} ms")
+        s_ = st.stats()
+        other_prunes[name] = {"prune_ms": min(ms), "step_ms": head["knn_ms"] + min(ms),
+                              "sections": s_["prune_kernel_ms"],
+                              "counters": {k: s_["prune_counters"][k] for k in
+                                           ("collect", "filter", "reverse", "adj_rows")}}
+        if args.phase_clocks and kind_ == "path":
+            st.set_profiling(2)
+            st.prune(kind_, fr, rr, args.fd, args.topm, th, rows)
+            pcs = st.stats()["prune_counters"]
+            other_prunes[name]["phase_cycles_per_node"] = {
+                k: pcs[k] / N for k in ("collect_cycles", "filter_cycles", "pool_cycles",
+                                        "eval_cycles", "beam_pre_cycles", "beam_sort_cycles",
+                                        "beam_merge_cycles", "beam_hash_cycles",
+                                        "pool_sort_cycles")}
+            st.set_profiling(True)
+        log(f"prune {name}: {min(ms):.1f} ms, {other_prunes[name]}")
 
     cpu = None
     if args.cpu_sample > 0:

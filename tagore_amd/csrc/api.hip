@@ -1251,6 +1251,19 @@ int tg_set_knn(tg_state* s, uint32_t K, const uint32_t* ids, const float* dists)
   if (K < 2 || K > 128 || K >= s->N) return fail(TG_ERR_ARG, "K must be in [2, min(128, N-1)]");
   for (size_t i = 0; i < size_t(s->N) * K; ++i)
     if (ids[i] >= s->N) return fail(TG_ERR_ARG, "kNN id out of range");
+  // rows sorted by (distance, id) with distances >= 0: the pruning kernels merge them as
+  // sorted lists and the local join binary-searches them
+  for (uint32_t u = 0; u < s->N; ++u) {
+    const float* d = dists + size_t(u) * K;
+    const uint32_t* g = ids + size_t(u) * K;
+    for (uint32_t j = 0; j < K; ++j) {
+      if (!(d[j] >= 0.0f)) return fail(TG_ERR_ARG, "kNN distances must be >= 0 (row " +
+                                                       std::to_string(u) + ")");
+      if (j && (d[j] < d[j - 1] || (d[j] == d[j - 1] && g[j] < g[j - 1])))
+        return fail(TG_ERR_ARG, "kNN row " + std::to_string(u) +
+                                    " is not sorted by (distance, id)");
+    }
+  }
   HIPX(hipSetDevice(s->device));
   if (ensure_graph(s, K)) return TG_ERR_HIP;
   HIPX(hipMemcpy(s->G, ids, size_t(s->N) * K * 4, hipMemcpyHostToDevice));

@@ -526,7 +526,8 @@ __global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
   }
   __syncthreads();
   const long long c1 = pclock(a);
-  // pool = merge(sorted own list, sorted ring), in registers
+  // pool = merge(own list, sorted ring), in registers; the own list is a kNN row, sorted by
+  // (distance, id) already (builds keep them so, tg_set_knn rejects others)
   uint64_t pv[16];
   {
     uint64_t o[2];
@@ -538,7 +539,6 @@ __global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
     uint64_t rr[RING_P / 64];
 #pragma unroll
     for (int q = 0; q < RING_P / 64; ++q) rr[q] = R[q * 64 + lane];
-    reg_bitonic<2>(o);
     reg_bitonic<RING_P / 64, uint64_t, true>(rr);
     pv[0] = o[0];
     pv[1] = o[1];

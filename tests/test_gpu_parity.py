@@ -11,7 +11,7 @@ import pytest
 
 import oracle
 import tagore_amd
-from tagore_amd import State
+from tagore_amd import State, TagoreError
 from tests.conftest import gmm_int, gmm_unit
 
 pytestmark = pytest.mark.gpu
@@ -356,3 +356,23 @@ def test_knn_bit_identical_d960(precision):
     _say(f"  re-scored {st['knn_rescored']} of {st['knn_evals']} evaluations")
     assert np.array_equal(G, Gr) and np.array_equal(GD, GDr)
     assert st["knn_evals"] == ev
+
+
+def test_set_knn_rejects_unsorted_or_negative_rows(int_case):
+    """tg_set_knn takes rows sorted by (distance, id) with distances >= 0 (the prune kernels
+    merge them as sorted lists); others are rejected before anything is copied."""
+    X, G, GD = int_case
+    with State(*X.shape) as s:
+        s.set_data(X)
+        G2, D2 = G.copy(), GD.copy()
+        G2[7, [3, 4]] = G2[7, [4, 3]]
+        D2[7, [3, 4]] = D2[7, [4, 3]]
+        if D2[7, 3] == D2[7, 4]:  # a tie: swapped ids are out of id order
+            assert G2[7, 3] > G2[7, 4]
+        with pytest.raises(TagoreError, match="not sorted"):
+            s.set_knn(G2, D2)
+        D3 = GD.copy()
+        D3[5, 0] = -1.0
+        with pytest.raises(TagoreError, match=">= 0"):
+            s.set_knn(G, D3)
+        s.set_knn(G, GD)  # the sorted rows are accepted
