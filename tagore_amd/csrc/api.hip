@@ -133,6 +133,7 @@ struct tg_state {
   bool cert_int = false;           // integer data whose screen distances are exact
   bool cert_u8 = false;            // and every value in [0, 255]: X8 holds the u8 copy
   uint8_t* X8 = nullptr;
+  float* n2s = nullptr;            // [N] |x - 128|^2 of the u8 rows
   uint32_t* fill = nullptr;
   uint32_t* scan_tmp = nullptr;
   unsigned long long* counters = nullptr;  // [CNT_SLOTS * CNT_N], summed over the slots
@@ -277,7 +278,9 @@ int ensure_prep(tg_state* s) {
   s->cert_u8 = s->cert_int && f[2] == 0 && f[4] == 0 && s->D % 4 == 0;
   if (s->cert_u8) {
     if (!s->X8 && dalloc(&s->X8, N * s->D)) return TG_ERR_HIP;
+    if (!s->n2s && dalloc(&s->n2s, N)) return TG_ERR_HIP;
     launch_make_u8(s->X, N * s->D, s->X8, s->stream);
+    if (s->D <= 256) launch_u8_shift_norms(s->X8, s->N, s->D, s->n2s, s->stream);
   }
   s->prep_valid = true;
   return TG_OK;
@@ -483,6 +486,7 @@ int tg_free(tg_state* s) {
   dfree(s->ne);
   dfree(s->cert_flags);
   dfree(s->X8);
+  dfree(s->n2s);
   dfree(s->fill);
   dfree(s->scan_tmp);
   dfree(s->counters);
@@ -991,6 +995,7 @@ int learned_init(tg_state* s, const tg_knn_params* p, uint64_t* evals) {
   sp.seed = p->seed ^ 0x5A3D1E5Bull;
   sp.precision = TG_PREC_EXACT;
   sp.init = TG_INIT_RANDOM;
+  sp.flags = TG_FLAG_GRAPH_ITERS;  // short kernels: later builds replay the captured sequence
   if (int rc = tg_knn_build(s->sub, &sp)) return rc;
   *evals += s->sub->stats.knn_evals;
   s->stats.learn_ms[0] = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
@@ -1094,6 +1099,7 @@ int tg_knn_build(tg_state* s, const tg_knn_params* p) {
     k.exact_screen = exact_screen ? 1 : 0;
     k.Xr16 = rows16;
     k.X8 = (p->precision != TG_PREC_EXACT && s->cert_u8 && s->D <= 256) ? s->X8 : nullptr;
+    k.n2s = k.X8 ? s->n2s : nullptr;
     if (p->precision == TG_PREC_FP8) {
       k.Xq8 = s->Xq8;
       k.sc8 = s->sc8;

@@ -107,6 +107,7 @@ struct KnnArgs {
   // rows used for canonical re-scores: BF16 copy when the data set is BF16-exact, else X
   const uint16_t* Xr16;
   const uint8_t* X8;     // u8 copy of X (every value an integer in [0, 255], D <= 256) or null
+  const float* n2s;      // [N] |x - 128|^2 of the u8 rows (exact), with X8
   // FP8 e4m3 copy with a per-vector scale (FP8 screen: nq/ne then hold its norms)
   const uint8_t* Xq8;
   const float* sc8;
@@ -123,6 +124,8 @@ void launch_screen_prep8(const float* X, uint32_t N, uint32_t D, uint8_t* Xq, fl
                          float* nq, float* ne, hipStream_t s);
 // u8 copy of the n values of X (n % 4 == 0; every value an integer in [0, 255])
 void launch_make_u8(const float* X, size_t n, uint8_t* X8, hipStream_t s);
+// |x - 128|^2 of every u8 row (the norms of the rows shifted into int8)
+void launch_u8_shift_norms(const uint8_t* X8, uint32_t N, uint32_t D, float* n2s, hipStream_t s);
 // screen-mode preparation: BF16 copy (if Xb), norms, quantisation-error norms and the
 // data-set certificates in flags[5] (zeroed by the caller)
 void launch_screen_prep(const float* X, uint32_t N, uint32_t D, uint16_t* Xb, float* nrm2,

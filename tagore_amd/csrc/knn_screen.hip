@@ -82,6 +82,26 @@ void launch_make_u8(const float* X, size_t n, uint8_t* X8, hipStream_t s) {
                                  reinterpret_cast<uint32_t*>(X8));
 }
 
+// |x - 128|^2 of every u8 row, one wave per row (an integer below 2^24 for D <= 256, so
+// exact as FP32)
+__global__ void k_u8_shift_norms(const uint8_t* X8, uint32_t N, uint32_t D, float* n2s) {
+  const uint32_t row = blockIdx.x;
+  if (row >= N) return;
+  const uint8_t* x = X8 + size_t(row) * D;
+  int acc = 0;
+  for (uint32_t j = threadIdx.x; j < D; j += TG_WAVE) {
+    const int v = int(x[j]) - 128;
+    acc += v * v;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, TG_WAVE);
+  if (threadIdx.x == 0) n2s[row] = float(acc);
+}
+
+void launch_u8_shift_norms(const uint8_t* X8, uint32_t N, uint32_t D, float* n2s, hipStream_t s) {
+  k_u8_shift_norms<<<N, 64, 0, s>>>(X8, N, D, n2s);
+}
+
 // FP8 e4m3 copy with a per-vector scale: q = e4m3(x / s), s = max|x| / 440 (below the
 // format's 448 maximum), and the norms the screen bound needs: |s q| and |x - s q| in FP64,
 // rounded up.  One wave per row, four elements per lane per step.
@@ -489,6 +509,82 @@ __device__ __forceinline__ void rows_have_keys(const uint32_t* G, const float* G
                mk_key(GD[size_t(v[q]) * K + lo[q]], G[size_t(v[q]) * K + lo[q]]) == key[q];
 }
 
+// Column block c of an exact screen (wave w, lane l = 16g + t): accB holds A rows
+// 16w + 4g + r against B row 16c + t, accA the same A rows against A row 16c + t, as exact
+// products with s = (na[i] + na[j]) - 2 acc.  Folds the block into the row jobs' minima
+// (row_b / row_a: the lane t == 4c + r owns job (16w + 4g + r, c)) and into this lane's
+// column job of the block (col_b: the minimum over the wave's 16 A rows, held by g == c).
+__device__ __forceinline__ void lj_block_minima(int c, const f32x4& accB, const f32x4& accA,
+                                                const uint32_t* A, const uint32_t* B,
+                                                const float* naA, const float* naB, int nA,
+                                                int nB, int w, int g, int t, uint64_t& row_b,
+                                                uint64_t& row_a, uint64_t& col_b) {
+  const float INF = __uint_as_float(0x7F800000u);
+  const int j = 16 * c + t;
+  float cd = INF;  // column job: this lane's 4 rows, then across the groups
+  uint32_t cid = 0xFFFFFFFFu;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = 16 * w + 4 * g + r;
+    const float sb = (i < nA && j < nB && A[i] != B[j]) ? (naA[i] + naB[j]) - 2.0f * accB[r] : INF;
+    const float sa = (i < nA && j < nA && i != j) ? (naA[i] + naA[j]) - 2.0f * accA[r] : INF;
+    const uint64_t kb = key_min_lanes<1, 16>(sb, B[j]);
+    const uint64_t ka = key_min_lanes<1, 16>(sa, A[j]);
+    if (t == 4 * c + r) {
+      row_b = kb;
+      row_a = ka;
+    }
+    if (sb < cd || (sb == cd && A[i] < cid)) {
+      cd = sb;
+      cid = A[i];
+    }
+  }
+  const uint64_t kc = key_min_lanes<16, 64>(cd, cid);
+  if (g == c) col_b = kc;
+}
+
+// The jobs' pushes: a job's minimum enters when it beats the destination's worst entry and
+// is not already listed (the merge would drop it); the three row searches run together.
+__device__ __forceinline__ void lj_push(const KnnArgs& a, const uint32_t* A, const uint32_t* B,
+                                        const float* wA, const float* wB, int nA, int nB,
+                                        int w, int g, int t, int l, uint64_t row_b,
+                                        uint64_t row_a, uint64_t col_b) {
+  const int i = 16 * w + 4 * g + (t & 3);
+  const bool inA = i < nA, inB = l < nB;
+  const uint32_t destA = inA ? A[i] : 0u, destB = inB ? B[l] : 0u;
+  const float worstA = inA ? wA[i] : 0.f, worstB = inB ? wB[l] : 0.f;
+  const uint32_t dv[3] = {destA, destA, destB};
+  const uint64_t kv[3] = {row_b, row_a, col_b};
+  const bool on[3] = {inA && key_d(row_b) < worstA, inA && key_d(row_a) < worstA,
+                      inB && key_d(col_b) < worstB};
+  bool found[3];
+  rows_have_keys<3>(a.G, a.GD, a.K, dv, kv, on, found);
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+    if (on[q] && !found[q]) pb_push(a.pc, dv[q], kv[q]);
+}
+
+// The node's sample lists, their norms (na) and their destinations' worst distances (wl).
+__device__ __forceinline__ void lj_lists(const KnnArgs& a, uint32_t u, const float* norms,
+                                         int nA, int nB, uint32_t* A, uint32_t* B, float* naA,
+                                         float* wA, float* naB, float* wB) {
+  const int tid = threadIdx.x, K = a.K;
+  if (tid < 64) {
+    const bool in = tid < nA;
+    const uint32_t v = in ? a.H[size_t(u) * HROW_S + tid] : 0u;
+    A[tid] = v;
+    naA[tid] = in ? norms[v] : 0.f;
+    wA[tid] = in ? a.GD[size_t(v) * K + K - 1] : 0.f;
+  } else if (tid < 128) {
+    const int j = tid - 64;
+    const bool in = j < nB;
+    const uint32_t v = in ? a.H[size_t(u) * HROW_S + 64 + j] : 0u;
+    B[j] = v;
+    naB[j] = in ? norms[v] : 0.f;
+    wB[j] = in ? a.GD[size_t(v) * K + K - 1] : 0.f;
+  }
+}
+
 template <int PREC, int DC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 void k_localjoin_exact_screen(KnnArgs a) {
@@ -501,68 +597,87 @@ void k_localjoin_exact_screen(KnnArgs a) {
   const uint32_t u = row_of_block(a.perm, a.N);
   if (u >= a.N) return;
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, g = l >> 4, t = l & 15;
-  const int K = a.K;
   const int nA = a.na[u], nB = a.nb[u];
-  const float INF = __uint_as_float(0x7F800000u);
-  if (tid < 64) {
-    const bool in = tid < nA;
-    const uint32_t v = in ? a.H[size_t(u) * HROW_S + tid] : 0u;
-    A[tid] = v;
-    naA[tid] = in ? a.nrm2[v] : 0.f;
-    wA[tid] = in ? a.GD[size_t(v) * K + K - 1] : 0.f;
-  } else if (tid < 128) {
-    const int j = tid - 64;
-    const bool in = j < nB;
-    const uint32_t v = in ? a.H[size_t(u) * HROW_S + 64 + j] : 0u;
-    B[j] = v;
-    naB[j] = in ? a.nrm2[v] : 0.f;
-    wB[j] = in ? a.GD[size_t(v) * K + K - 1] : 0.f;
-  }
+  lj_lists(a, u, a.nrm2, nA, nB, A, B, naA, wA, naB, wB);
   __syncthreads();
   f32x4 accB[4], accA[4];
   screen_products<PREC, DC>(a, A, B, nA, nB, smem, smem + STAGE, accB, accA);
   uint64_t row_b = TG_KEY_INF, row_a = TG_KEY_INF, col_b = TG_KEY_INF;
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const int j = 16 * c + t;
-    float cd = INF;  // column job: this lane's 4 rows, then across the groups
-    uint32_t cid = 0xFFFFFFFFu;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int i = 16 * w + 4 * g + r;
-      const float sb = (i < nA && j < nB && A[i] != B[j]) ? (naA[i] + naB[j]) - 2.0f * accB[c][r] : INF;
-      const float sa = (i < nA && j < nA && i != j) ? (naA[i] + naA[j]) - 2.0f * accA[c][r] : INF;
-      const uint64_t kb = key_min_lanes<1, 16>(sb, B[j]);
-      const uint64_t ka = key_min_lanes<1, 16>(sa, A[j]);
-      if (t == 4 * c + r) {
-        row_b = kb;
-        row_a = ka;
+  for (int c = 0; c < 4; ++c)
+    lj_block_minima(c, accB[c], accA[c], A, B, naA, naB, nA, nB, w, g, t, row_b, row_a, col_b);
+  lj_push(a, A, B, wA, wB, nA, nB, w, g, t, l, row_b, row_a, col_b);
+  if (a.counters && tid == 0)
+    atomicAdd(&cnt_slot(a.counters)[0], (unsigned long long)(nA * nB + nA * (nA - 1) / 2));
+}
+
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+
+// Exact screen on int8 MFMA for u8 data (every value an integer in [0, 255], D <= DC <= 256):
+// rows are staged whole as x - 128 (the byte XOR 0x80, a signed int8), and
+// v_mfma_i32_16x16x64_i8 gives a'.b' exactly, so s = |a'|^2 + |b'|^2 - 2 a'.b' = |a - b|^2
+// exactly (|a'|^2 from n2s): the canonical distance of this data, as in the BF16 exact
+// screen.  Half the staged bytes and MFMA instructions of the BF16 rows; the column blocks
+// are formed and folded one at a time, so only two accumulators are live.
+template <int DC>
+__global__ __launch_bounds__(256) void k_localjoin_u8_screen(KnnArgs a) {
+  constexpr int RS = DC + 16;  // padded LDS row stride in bytes
+  constexpr int CPR = DC / 16;  // 16-byte pieces per staged row
+  constexpr int KQ = DC / 64;   // MFMA K-steps
+  __shared__ __attribute__((aligned(16))) unsigned char sA[64 * RS], sB[64 * RS];
+  __shared__ uint32_t A[64], B[64];
+  __shared__ float naA[64], wA[64], naB[64], wB[64];
+  const uint32_t u = row_of_block(a.perm, a.N);
+  if (u >= a.N) return;
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, g = l >> 4, t = l & 15;
+  const int D = a.D;
+  const int nA = a.na[u], nB = a.nb[u];
+  lj_lists(a, u, a.n2s, nA, nB, A, B, naA, wA, naB, wB);
+  __syncthreads();
+  // stage both lists' rows: bytes past D and rows past the list ends are 0 (no product)
+  for (int e = tid; e < 2 * 64 * CPR; e += 256) {
+    const int op = e / (64 * CPR), r = (e / CPR) % 64, col = (e % CPR) * 16;
+    const int n = op == 0 ? nA : nB;
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    if (r < n && col < D) {
+      const uint8_t* src = a.X8 + size_t(op == 0 ? A[r] : B[r]) * D + col;
+      if ((D & 15) == 0) {
+        v = *reinterpret_cast<const uint4*>(src);
+      } else {
+        v.x = *reinterpret_cast<const uint32_t*>(src);
+        if (col + 4 < D) v.y = *reinterpret_cast<const uint32_t*>(src + 4);
+        if (col + 8 < D) v.z = *reinterpret_cast<const uint32_t*>(src + 8);
+        if (col + 12 < D) v.w = *reinterpret_cast<const uint32_t*>(src + 12);
       }
-      if (sb < cd || (sb == cd && A[i] < cid)) {
-        cd = sb;
-        cid = A[i];
-      }
+      v.x ^= col < D ? 0x80808080u : 0u;
+      v.y ^= col + 4 < D ? 0x80808080u : 0u;
+      v.z ^= col + 8 < D ? 0x80808080u : 0u;
+      v.w ^= col + 12 < D ? 0x80808080u : 0u;
     }
-    const uint64_t kc = key_min_lanes<16, 64>(cd, cid);
-    if (g == c) col_b = kc;
+    *reinterpret_cast<uint4*>((op == 0 ? sA : sB) + r * RS + col) = v;
   }
-  // pushes: a job's minimum enters when it beats the destination's worst entry and is not
-  // already listed (the merge would drop it); the three row searches run together
-  {
-    const int i = 16 * w + 4 * g + (t & 3);
-    const bool inA = i < nA, inB = l < nB;
-    const uint32_t destA = inA ? A[i] : 0u, destB = inB ? B[l] : 0u;
-    const float worstA = inA ? wA[i] : 0.f, worstB = inB ? wB[l] : 0.f;
-    const uint32_t dv[3] = {destA, destA, destB};
-    const uint64_t kv[3] = {row_b, row_a, col_b};
-    const bool on[3] = {inA && key_d(row_b) < worstA, inA && key_d(row_a) < worstA,
-                        inB && key_d(col_b) < worstB};
-    bool found[3];
-    rows_have_keys<3>(a.G, a.GD, K, dv, kv, on, found);
+  __syncthreads();
+  // A-operand pieces of this wave's 16 rows, every K-step (lane group g: bytes 16g..16g+15)
+  i32x4 fa[KQ];
 #pragma unroll
-    for (int q = 0; q < 3; ++q)
-      if (on[q] && !found[q]) pb_push(a.pc, dv[q], kv[q]);
+  for (int k = 0; k < KQ; ++k)
+    fa[k] = *reinterpret_cast<const i32x4*>(sA + (16 * w + t) * RS + 64 * k + 16 * g);
+  uint64_t row_b = TG_KEY_INF, row_a = TG_KEY_INF, col_b = TG_KEY_INF;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    i32x4 pB = {0, 0, 0, 0}, pA = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < KQ; ++k) {
+      const i32x4 fb = *reinterpret_cast<const i32x4*>(sB + (16 * c + t) * RS + 64 * k + 16 * g);
+      const i32x4 fc = *reinterpret_cast<const i32x4*>(sA + (16 * c + t) * RS + 64 * k + 16 * g);
+      pB = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[k], fb, pB, 0, 0, 0);
+      pA = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[k], fc, pA, 0, 0, 0);
+    }
+    const f32x4 accB = {float(pB[0]), float(pB[1]), float(pB[2]), float(pB[3])};
+    const f32x4 accA = {float(pA[0]), float(pA[1]), float(pA[2]), float(pA[3])};
+    lj_block_minima(c, accB, accA, A, B, naA, naB, nA, nB, w, g, t, row_b, row_a, col_b);
   }
+  lj_push(a, A, B, wA, wB, nA, nB, w, g, t, l, row_b, row_a, col_b);
   if (a.counters && tid == 0)
     atomicAdd(&cnt_slot(a.counters)[0], (unsigned long long)(nA * nB + nA * (nA - 1) / 2));
 }
@@ -571,6 +686,13 @@ void k_localjoin_exact_screen(KnnArgs a) {
 // 64-byte rows (64 / 32 / 16 elements per round): less zero padding at D not a multiple of
 // the depth, more LDS rounds
 void launch_knn_localjoin_screen(const KnnArgs& a, int precision, int tile, hipStream_t s) {
+  if (a.exact_screen && precision == KP_BF16 && a.X8 && a.n2s && a.D <= 256) {
+    // u8 data: the BF16 screen's products on int8 MFMA (exact either way)
+    const dim3 grid(grid_rows(a.perm, a.N));
+    if (a.D <= 128) k_localjoin_u8_screen<128><<<grid, 256, 0, s>>>(a);
+    else k_localjoin_u8_screen<256><<<grid, 256, 0, s>>>(a);
+    return;
+  }
   if (a.exact_screen && (precision == KP_BF16 || precision == KP_FP32)) {
     // 256-byte staged rows (128 BF16 / 64 FP32 elements), 64-byte rows with tile 1
     const dim3 grid(grid_rows(a.perm, a.N));

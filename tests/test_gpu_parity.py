@@ -376,3 +376,24 @@ def test_set_knn_rejects_unsorted_or_negative_rows(int_case):
         with pytest.raises(TagoreError, match=">= 0"):
             s.set_knn(G, D3)
         s.set_knn(G, GD)  # the sorted rows are accepted
+
+
+@pytest.mark.parametrize("D", [104, 128, 200, 248])
+def test_knn_bit_identical_u8_rows(D):
+    """Integer data in [0, 255] under the BF16 screen: the local join's products on int8
+    MFMA from the u8 copy (rows staged whole, 16-byte pieces, partial pieces when D is not a
+    multiple of 16, one or four K-steps), exact, so the graph is the oracle's bit for bit."""
+    rng = np.random.default_rng(D)
+    C = rng.integers(0, 256, (20, D))
+    X = np.clip(C[rng.integers(0, 20, 2500)] + rng.integers(-20, 21, (2500, D)), 0, 255)
+    X = X.astype(np.float32)
+    K, it, seed = 32, 6, 0x0B
+    Gr, GDr, ev = oracle.gnn_descent(X, K, it, seed)
+    with State(*X.shape) as s:
+        s.set_data(X)
+        s.knn_build(K, it, seed, "bf16")
+        G, GD = s.get_knn()
+        assert s.stats()["knn_evals"] == ev
+    bad = np.nonzero((G != Gr).any(1))[0]
+    assert bad.size == 0, f"D={D}: {bad.size} rows differ, first {bad[:5]}"
+    assert np.array_equal(GD, GDr)
