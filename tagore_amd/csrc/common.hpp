@@ -601,8 +601,9 @@ struct PushBuf {
   uint32_t ov_cap;
 };
 
-__device__ __forceinline__ void pb_push(const PushBuf& pb, uint32_t v, uint64_t key) {
-  const uint32_t s = atomicAdd(&pb.cnt[v], 1u);
+// Stores key at position s of v's push list (s from the list's counter): a slot, or the
+// overflow area when the slots are full.
+__device__ __forceinline__ void pb_put(const PushBuf& pb, uint32_t v, uint32_t s, uint64_t key) {
   if (s < pb.cap) {
     pb.slots[size_t(v) * pb.cap + s] = key;
   } else {
@@ -619,6 +620,10 @@ __device__ __forceinline__ void pb_push(const PushBuf& pb, uint32_t v, uint64_t 
       atomicOr(pb.err, 1u);
     }
   }
+}
+
+__device__ __forceinline__ void pb_push(const PushBuf& pb, uint32_t v, uint64_t key) {
+  pb_put(pb, v, atomicAdd(&pb.cnt[v], 1u), key);
 }
 
 // Number of keys pushed to v, and accessor of the i-th (slot area first, then overflow).

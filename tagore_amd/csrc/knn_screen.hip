@@ -545,10 +545,15 @@ __device__ __forceinline__ void lj_block_minima(int c, const f32x4& accB, const 
 
 // The jobs' pushes: a job's minimum enters when it beats the destination's worst entry and
 // is not already listed (the merge would drop it); the three row searches run together.
+// Keys for one destination take one counter update and consecutive slots, so they share
+// cache lines: the row jobs of A row i sit in the four lanes t = (i % 4) + 4c of a 16-lane
+// row (up to 8 keys), the column jobs of B row l in lane l of the four waves (up to 4 keys,
+// gathered through colk, 4 x 64 keys of LDS that no wave reads any more).  The slot order
+// of a list is immaterial (its reduction sorts).
 __device__ __forceinline__ void lj_push(const KnnArgs& a, const uint32_t* A, const uint32_t* B,
                                         const float* wA, const float* wB, int nA, int nB,
                                         int w, int g, int t, int l, uint64_t row_b,
-                                        uint64_t row_a, uint64_t col_b) {
+                                        uint64_t row_a, uint64_t col_b, uint64_t* colk) {
   const int i = 16 * w + 4 * g + (t & 3);
   const bool inA = i < nA, inB = l < nB;
   const uint32_t destA = inA ? A[i] : 0u, destB = inB ? B[l] : 0u;
@@ -559,9 +564,38 @@ __device__ __forceinline__ void lj_push(const KnnArgs& a, const uint32_t* A, con
                       inB && key_d(col_b) < worstB};
   bool found[3];
   rows_have_keys<3>(a.G, a.GD, a.K, dv, kv, on, found);
+  // row jobs: counts of the four lanes c = t / 4, their prefix in c order, one counter update
+  const bool pb = on[0] && !found[0], pa = on[1] && !found[1];
+  const int cnt = int(pb) + int(pa), c = t >> 2;
+  const int x4 = int(lane_xor32(uint32_t(cnt), 4));
+  const int p1 = cnt + x4;
+  const int x8 = int(lane_xor32(uint32_t(p1), 8));
+  const int total = p1 + x8;
+  const int prefix = ((c & 1) ? x4 : 0) + ((c & 2) ? x8 : 0);
+  uint32_t base = 0;
+  if (c == 0 && total > 0) base = atomicAdd(&a.pc.cnt[destA], uint32_t(total));
+  base = uint32_t(__shfl(int(base), (l & ~15) | (t & 3), TG_WAVE));
+  if (pb) pb_put(a.pc, destA, base + uint32_t(prefix), row_b);
+  if (pa) pb_put(a.pc, destA, base + uint32_t(prefix) + (pb ? 1u : 0u), row_a);
+  // column jobs: wave 0 pushes the four waves' keys of B row l together
+  __syncthreads();
+  colk[w * 64 + l] = (on[2] && !found[2]) ? col_b : TG_KEY_INF;
+  __syncthreads();
+  if (w == 0 && inB) {
+    uint64_t k4[4];
+    uint32_t n = 0;
 #pragma unroll
-  for (int q = 0; q < 3; ++q)
-    if (on[q] && !found[q]) pb_push(a.pc, dv[q], kv[q]);
+    for (int q = 0; q < 4; ++q) {
+      k4[q] = colk[q * 64 + l];
+      n += k4[q] != TG_KEY_INF ? 1u : 0u;
+    }
+    if (n) {
+      uint32_t s = atomicAdd(&a.pc.cnt[destB], n);
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (k4[q] != TG_KEY_INF) pb_put(a.pc, destB, s++, k4[q]);
+    }
+  }
 }
 
 // The node's sample lists, their norms (na) and their destinations' worst distances (wl).
@@ -606,7 +640,8 @@ void k_localjoin_exact_screen(KnnArgs a) {
 #pragma unroll
   for (int c = 0; c < 4; ++c)
     lj_block_minima(c, accB[c], accA[c], A, B, naA, naB, nA, nB, w, g, t, row_b, row_a, col_b);
-  lj_push(a, A, B, wA, wB, nA, nB, w, g, t, l, row_b, row_a, col_b);
+  lj_push(a, A, B, wA, wB, nA, nB, w, g, t, l, row_b, row_a, col_b,
+          reinterpret_cast<uint64_t*>(smem));
   if (a.counters && tid == 0)
     atomicAdd(&cnt_slot(a.counters)[0], (unsigned long long)(nA * nB + nA * (nA - 1) / 2));
 }
@@ -677,7 +712,8 @@ __global__ __launch_bounds__(256) void k_localjoin_u8_screen(KnnArgs a) {
     const f32x4 accA = {float(pA[0]), float(pA[1]), float(pA[2]), float(pA[3])};
     lj_block_minima(c, accB, accA, A, B, naA, naB, nA, nB, w, g, t, row_b, row_a, col_b);
   }
-  lj_push(a, A, B, wA, wB, nA, nB, w, g, t, l, row_b, row_a, col_b);
+  lj_push(a, A, B, wA, wB, nA, nB, w, g, t, l, row_b, row_a, col_b,
+          reinterpret_cast<uint64_t*>(sA));
   if (a.counters && tid == 0)
     atomicAdd(&cnt_slot(a.counters)[0], (unsigned long long)(nA * nB + nA * (nA - 1) / 2));
 }
