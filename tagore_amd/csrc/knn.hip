@@ -745,7 +745,23 @@ __global__ __launch_bounds__(64) void k_phase2(KnnArgs a) {
   // worst distance; everything kept is re-scored exactly.  The keep flags use kv's space
   // (the keys are cleared again after the compaction).
   int nre = cnt;
-  if (SCREEN) {
+  bool done = false;
+  if constexpr (PREC == KP_BF16 && MC <= 4) {
+    if (a.exact_screen && a.X8) {
+      // integer data in [0, 255] (D <= 256): exact distances from the u8 copy (quad layout,
+      // 32 candidates per step), the canonical distances themselves
+      Row16<MC> xu8;
+      load_row16<MC>(xu8, a.X8 + size_t(u) * D, D, t);
+      __syncthreads();
+      eval_batch<MC>(a.X8, D, xu8, cnt, [&](int q) { return ids[q]; },
+                     [&](int q, float d) {
+                       if (d < worst) kv[q] = mk_key(d, ids[q]);
+                     });
+      nre = 0;
+      done = true;
+    }
+  }
+  if (SCREEN && !done) {
     uint16_t* keep = reinterpret_cast<uint16_t*>(kv);
     // item q0 + g + 4b in group g, four candidates per group in flight
     for (int q0 = 0; q0 < cnt; q0 += 16) {
