@@ -10,8 +10,10 @@ Three paths are timed in the same run:
   * plain FP32: FP32 MFMA screen + canonical FP32 re-score, random init, 10 iterations
     (the configuration the reference runs: pybind.cu:101 with iter=10);
   * headline:   learned initial lists + BF16 MFMA screen + canonical FP32 re-score,
-    2 iterations (--init, --precision, --iters); the encoder's sample graph, training,
-    encoding and bucketing are inside the timed step;
+    2 iterations (--init, --precision, --iters); the encoder is trained for one epoch on a
+    25k-vector sample (--epochs, --sample); its sample graph, training, encoding and
+    bucketing are inside the timed step (on integer data the screen's products run on int8
+    MFMA, exact);
   * ablation:   random init + BF16 screen at the fewest iterations that reach the recall
     target (--ablation-iters), to separate the two ingredients.
 `value` is the headline's vectors/s over the timed steps; the other paths' figures and the
@@ -44,7 +46,8 @@ def measure(st, args, precision, iters, init, label, graph_iters=False):
     import tagore_amd
 
     def step():
-        st.knn_build(args.k, iters, args.seed, precision, init=init, graph_iters=graph_iters)
+        st.knn_build(args.k, iters, args.seed, precision, init=init, graph_iters=graph_iters,
+                     sample=args.sample, epochs=args.epochs)
         st.prune("path", "dist", "dist", args.fd, args.topm, 1.0,
                  args.prune_rows if precision == "bf16" else "fp32")
 
@@ -271,10 +274,11 @@ def bench_clusters(args):
     print(json.dumps(out), flush=True)
 
 
-def bench_iterations(args, N, D, K, kind, ncomp, seed, precision, prunes, sweep):
+def bench_iterations(args, N, D, K, kind, ncomp, seed, precision, prunes, sweep, epochs=0):
     """Iterations to recall@10 >= 0.95, learned vs random initial lists, then the prunes.
     Each build's device window is timed; recall is the FP64 judge on a seeded sample.
-    precision "auto": the autotuner's fastest kNN variant (all give the same graph)."""
+    precision "auto": the autotuner's fastest kNN variant (all give the same graph).
+    epochs: training epochs of the learned lists' encoder (0: the library's default, 2)."""
     import tagore_amd
 
     log(f"workload N={N} D={D} K={K} data kind {kind} precision {precision}")
@@ -286,10 +290,10 @@ def bench_iterations(args, N, D, K, kind, ncomp, seed, precision, prunes, sweep)
         log(f"autotune: {cands} -> {precision} tile {tile}")
     st.set_profiling(True)
     rows = {}
-    st.knn_build(K, 2, seed, precision, init="learned", tile=tile)  # warm-up
+    st.knn_build(K, 2, seed, precision, init="learned", tile=tile, epochs=epochs)  # warm-up
     nq = min(args.recall_queries, N, 2000 if D > 256 else 10000)
     for init, iters in sweep:
-        st.knn_build(K, iters, seed, precision, init=init, tile=tile)
+        st.knn_build(K, iters, seed, precision, init=init, tile=tile, epochs=epochs)
         s = st.stats()
         best = s["knn_ms"]
         rec = st.recall(nq, 10, 0x5A11)
@@ -303,7 +307,7 @@ def bench_iterations(args, N, D, K, kind, ncomp, seed, precision, prunes, sweep)
         first[init] = min(ok, key=lambda x: x[0]) if ok else None
     # prunes on the learned graph at its first iteration count reaching the target
     it = first["learned"][0] if first["learned"] else max(i for n, i in sweep if n == "learned")
-    st.knn_build(K, it, seed, precision, init="learned", tile=tile)
+    st.knn_build(K, it, seed, precision, init="learned", tile=tile, epochs=epochs)
     knn_ms = st.stats()["knn_ms"]
     pr = {}
     for name, (kind_, fr, rr, th, pprecs) in prunes.items():
@@ -324,21 +328,23 @@ def bench_iterations(args, N, D, K, kind, ncomp, seed, precision, prunes, sweep)
 
 
 def bench_unit(args):
-    """1M x 96 unit-norm Gaussian mixture (512 components), K=64: learned vs random
-    initial lists (iterations to recall@10 >= 0.95), then NSSG (2-hop / angle 60 degrees)."""
+    """1M x 96 unit-norm Gaussian mixture (512 components), K=64: learned (encoder trained
+    5 epochs on the 50k sample's pairs) vs random initial lists (iterations to recall@10 >=
+    0.95), then NSSG (2-hop / angle 60 degrees)."""
     N = args.n
     rows, first, knn_ms, pr, step_ms, var, it = bench_iterations(
         args, N, 96, 64, 1, 512, 0x7A600003, args.precision,
         {"NSSG": ("2-hop", "angle", "angle", 0.5, ["fp32"])},
         [("learned", 2), ("learned", 4), ("random", 2), ("random", 4), ("random", 6),
-         ("random", 8)])
+         ("random", 8)], epochs=5)
     out = {
         "metric": "kNN+NSSG vectors/s at recall@10>=0.95, learned/BF16, 1 GPU",
         "value": N / (step_ms / 1e3), "unit": "vectors/s", "n_gpus": 1, "steps": 1,
         "warmup": 1, "ms_per_step": step_ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": args.precision, "data": "synthetic",
         "config": {"workload": f"GNN-Descent K=64 + NSSG FD=32 angle 60 over {N}x96 unit "
-                               "GMM(512)", "N": N, "dim": 96, "K": 64},
+                               "GMM(512), learned lists from a 5-epoch encoder",
+                   "N": N, "dim": 96, "K": 64, "epochs": 5},
         "iterations_to_target": {k: (v[0] if v else None) for k, v in first.items()},
         "timed_iterations": it, "knn_variant": list(var),
         "builds": rows, "prunes": pr,
@@ -382,6 +388,10 @@ def main():
     ap.add_argument("--iters", type=int, default=2, help="headline iterations")
     ap.add_argument("--init", default="learned", choices=["learned", "random"],
                     help="headline initial lists")
+    ap.add_argument("--sample", type=int, default=25000,
+                    help="learned init: vectors in the encoder's training sample")
+    ap.add_argument("--epochs", type=int, default=1,
+                    help="learned init: training epochs of the encoder")
     ap.add_argument("--plain-iters", type=int, default=10)
     ap.add_argument("--ablation-iters", type=int, default=6,
                     help="random-init BF16 iterations of the ablation line (0 skips it)")
@@ -589,7 +599,8 @@ def main():
         "config": {"workload": f"GNN-Descent K={K} + NSG prune FD={args.fd} m={args.topm} over "
                                f"{N}x{D} integer GMM({args.ncomp}), sigma 20",
                    "N": N, "dim": D, "K": K, "iters": args.iters, "final_degree": args.fd,
-                   "m": args.topm, "precision": args.precision, "init": args.init},
+                   "m": args.topm, "precision": args.precision, "init": args.init,
+                   "learned_sample": args.sample, "learned_epochs": args.epochs},
         "recall_at_10": head["recall_at_10"],
         "recall_gate": "recall@10 of the kNN graph against FP64 brute force on a seeded "
                        "sample of the vectors; the NSG index's own beam-search recall is "
