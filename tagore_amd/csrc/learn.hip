@@ -345,78 +345,136 @@ __global__ void k_km_init(const float* __restrict__ Z, const uint32_t* __restric
   cent[e] = Z[size_t(samp[c]) * MLP_CODE + k];
 }
 
-__global__ void k_km_norms(const float* __restrict__ cent, uint32_t C, float* cn) {
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_km_norms(const float* __restrict__ cent, uint32_t C, float* cn, uint16_t* chi,
+                           uint16_t* clo) {
+  const uint32_t c = blockIdx.x * 256 + threadIdx.x;
   if (c >= C) return;
   float s = 0.f;
-  for (uint32_t k = 0; k < MLP_CODE; ++k) s = fmaf(cent[c * MLP_CODE + k], cent[c * MLP_CODE + k], s);
+  for (uint32_t k = 0; k < MLP_CODE; ++k) {
+    const float x = cent[c * MLP_CODE + k];
+    s = fmaf(x, x, s);
+    // the split the matrix-core assignment multiplies: x = hi + lo to about 16 bits
+    const uint16_t h = f2bf(x);
+    chi[c * MLP_CODE + k] = h;
+    clo[c * MLP_CODE + k] = f2bf(x - bf2f(h));
+  }
   cn[c] = s;
 }
 
-// nearest (and second nearest) centroid of codes Z[idx[i]] (idx null: Z[i]); score
-// |c|^2 - 2 z.c, ties to the smaller centroid index
-__global__ __launch_bounds__(256) void k_km_assign(const float* __restrict__ Z,
-                                                   const uint32_t* __restrict__ idx, uint32_t n,
-                                                   const float* __restrict__ cent,
-                                                   const float* __restrict__ cn, uint32_t C,
-                                                   uint32_t* lab, uint32_t* lab2) {
-  // every lane reads the same centroid at a time (broadcast), so rows need no padding and
-  // are read as float4
-  __shared__ float4 sc[64][MLP_CODE / 4];
+// (s1, c1) <= (s2, c2): the two smallest (s, c) pairs seen, kept with selects (no branches).
+// `ordered`: the pairs arrive in increasing c, so an equal s never displaces a kept one.
+template <bool ordered>
+__device__ __forceinline__ void km_top2(float s, uint32_t c, float& s1, uint32_t& c1, float& s2,
+                                        uint32_t& c2) {
+  const bool b1 = ordered ? s < s1 : (s < s1 || (s == s1 && c < c1));
+  const bool b2 = ordered ? s < s2 : (s < s2 || (s == s2 && c < c2));
+  const float ns2 = b1 ? s1 : (b2 ? s : s2);
+  const uint32_t nc2 = b1 ? c1 : (b2 ? c : c2);
+  s1 = b1 ? s : s1;
+  c1 = b1 ? c : c1;
+  s2 = ns2;
+  c2 = nc2;
+}
+
+// Nearest and second-nearest centroid of each code (idx: codes idx[0..n), else 0..n), by
+// s = |c|^2 - 2 c.z and then the centroid index.  C[c][j] = cent_c . z_j
+// as a 16x16x32 BF16 product per tile (centroids as the A rows, codes as the B columns), each
+// FP32 operand split into a BF16 high part and a BF16 remainder, and the three products
+// hi.hi + hi.lo + lo.hi summed in FP32 (about 16 mantissa bits of each operand).  A lane
+// holds one code column and four centroid rows of a tile, so it keeps its code's two
+// nearest centroids by (s, c) with no exchange until the end.  16 QN codes per wave (QN column
+// tiles), 64 QN per workgroup; the centroids' hi / lo parts (k_km_norms) are staged in LDS 64
+// rows at a time.
+template <int QN>
+__global__ __launch_bounds__(256) void k_km_assign_mfma(const float* __restrict__ Z,
+                                                        const uint32_t* __restrict__ idx,
+                                                        uint32_t n, const uint16_t* __restrict__ chi,
+                                                        const uint16_t* __restrict__ clo,
+                                                        const float* __restrict__ cn, uint32_t C,
+                                                        uint32_t* lab, uint32_t* lab2) {
+  static_assert(MLP_CODE == 32, "one 16x16x32 K-step per tile");
+  constexpr int CP = MLP_CODE + 8;  // padded LDS row (80 bytes)
+  __shared__ __attribute__((aligned(16))) uint16_t shi[64 * CP];
+  __shared__ __attribute__((aligned(16))) uint16_t slo[64 * CP];
   __shared__ float sn[64];
-  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  float z[MLP_CODE];
-  if (i < n) {
-    const float* zp = Z + size_t(idx ? idx[i] : i) * MLP_CODE;
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, g = l >> 4, t = l & 15;
+  // B fragments: code j = base + 16q + t, elements 8g .. 8g+7
+  const uint32_t base = blockIdx.x * (64 * QN) + 16 * QN * w;
+  bf16x8 bh[QN], bl[QN];
 #pragma unroll
-    for (int k = 0; k < int(MLP_CODE); ++k) z[k] = zp[k];
-  } else {
+  for (int q = 0; q < QN; ++q) {
+    const uint32_t j = base + 16 * q + t;
+    const uint32_t row = j < n ? (idx ? idx[j] : j) : 0u;
+    const float4* zp = reinterpret_cast<const float4*>(Z + size_t(row) * MLP_CODE + 8 * g);
+    const float4 z0 = zp[0], z1 = zp[1];
+    const float x[8] = {z0.x, z0.y, z0.z, z0.w, z1.x, z1.y, z1.z, z1.w};
 #pragma unroll
-    for (int k = 0; k < int(MLP_CODE); ++k) z[k] = 0.f;
+    for (int e = 0; e < 8; ++e) {
+      const uint16_t h = f2bf(x[e]);
+      bh[q][e] = __builtin_bit_cast(__bf16, h);
+      bl[q][e] = __builtin_bit_cast(__bf16, f2bf(x[e] - bf2f(h)));
+    }
   }
-  float s1 = INFINITY, s2 = INFINITY;
-  uint32_t c1 = 0, c2 = 0;
+  float s1[QN], s2[QN];
+  uint32_t c1[QN], c2[QN];
+#pragma unroll
+  for (int q = 0; q < QN; ++q) {
+    s1[q] = s2[q] = INFINITY;
+    c1[q] = c2[q] = 0xFFFFFFFFu;
+  }
   for (uint32_t c0 = 0; c0 < C; c0 += 64) {
     __syncthreads();
-    for (uint32_t e = threadIdx.x; e < 64 * MLP_CODE / 4; e += 256) {
-      const uint32_t c = c0 + e / (MLP_CODE / 4), k4 = e % (MLP_CODE / 4);
-      sc[e / (MLP_CODE / 4)][k4] =
-          c < C ? *reinterpret_cast<const float4*>(cent + size_t(c) * MLP_CODE + 4 * k4)
-                : make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-    if (threadIdx.x < 64) sn[threadIdx.x] = c0 + threadIdx.x < C ? cn[c0 + threadIdx.x] : INFINITY;
-    __syncthreads();
-    const uint32_t cend = min(64u, C - c0);
-    for (uint32_t q = 0; q < cend; ++q) {
-      float dot = 0.f;
-#pragma unroll
-      for (int k4 = 0; k4 < int(MLP_CODE) / 4; ++k4) {
-        const float4 w = sc[q][k4];
-        dot = fmaf(z[4 * k4], w.x, dot);
-        dot = fmaf(z[4 * k4 + 1], w.y, dot);
-        dot = fmaf(z[4 * k4 + 2], w.z, dot);
-        dot = fmaf(z[4 * k4 + 3], w.w, dot);
+    {
+      const int r = tid >> 2, pc = tid & 3;  // 64 rows x 4 pieces of 16 bytes
+      uint4 vh = make_uint4(0u, 0u, 0u, 0u), vl = make_uint4(0u, 0u, 0u, 0u);
+      if (c0 + r < C) {
+        vh = *reinterpret_cast<const uint4*>(chi + size_t(c0 + r) * MLP_CODE + 8 * pc);
+        vl = *reinterpret_cast<const uint4*>(clo + size_t(c0 + r) * MLP_CODE + 8 * pc);
       }
-      const float s = sn[q] - 2.f * dot;
-      const uint32_t c = c0 + q;
-      if (s < s1) {
-        s2 = s1;
-        c2 = c1;
-        s1 = s;
-        c1 = c;
-      } else if (s < s2) {
-        s2 = s;
-        c2 = c;
+      *reinterpret_cast<uint4*>(shi + r * CP + 8 * pc) = vh;
+      *reinterpret_cast<uint4*>(slo + r * CP + 8 * pc) = vl;
+      if (tid < 64) sn[tid] = c0 + tid < C ? cn[c0 + tid] : INFINITY;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) {
+      const bf16x8 ah = *reinterpret_cast<const bf16x8*>(shi + (16 * ct + t) * CP + 8 * g);
+      const bf16x8 al = *reinterpret_cast<const bf16x8*>(slo + (16 * ct + t) * CP + 8 * g);
+      float snr[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) snr[r] = sn[16 * ct + 4 * g + r];
+#pragma unroll
+      for (int q = 0; q < QN; ++q) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[q], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[q], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[q], acc, 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          km_top2<true>(snr[r] - 2.f * acc[r], c0 + uint32_t(16 * ct + 4 * g + r), s1[q], c1[q],
+                        s2[q], c2[q]);
       }
     }
   }
-  if (i < n) {
-    lab[i] = c1;
-    if (lab2) lab2[i] = C > 1 ? c2 : c1;
+  // the four lane groups hold disjoint centroid rows of each code: merge their top-2
+#pragma unroll
+  for (int q = 0; q < QN; ++q) {
+#pragma unroll
+    for (int o = 16; o <= 32; o <<= 1) {
+      const float ps1 = __shfl_xor(s1[q], o, TG_WAVE), ps2 = __shfl_xor(s2[q], o, TG_WAVE);
+      const uint32_t pc1 = uint32_t(__shfl_xor(int(c1[q]), o, TG_WAVE));
+      const uint32_t pc2 = uint32_t(__shfl_xor(int(c2[q]), o, TG_WAVE));
+      km_top2<false>(ps1, pc1, s1[q], c1[q], s2[q], c2[q]);
+      km_top2<false>(ps2, pc2, s1[q], c1[q], s2[q], c2[q]);
+    }
+    const uint32_t j = base + 16 * q + t;
+    if (g == 0 && j < n) {
+      lab[j] = c1[q];
+      if (lab2) lab2[j] = C > 1 ? c2[q] : c1[q];
+    }
   }
 }
 
-// centroid c = mean of the sample codes labelled c (fixed-order sums); empty keeps its value
 __global__ __launch_bounds__(64) void k_km_update(const float* __restrict__ Z,
                                                   const uint32_t* __restrict__ samp,
                                                   const uint32_t* __restrict__ slab, uint32_t S,
@@ -733,6 +791,8 @@ hipError_t learner_reserve(Learner& L, uint32_t D, uint32_t M, uint32_t N, uint3
   if (L.Ccap < C) {
     LX(rmalloc(&L.cent, size_t(C) * MLP_CODE));
     LX(rmalloc(&L.cn, C));
+    LX(rmalloc(&L.chi, size_t(C) * MLP_CODE));
+    LX(rmalloc(&L.clo, size_t(C) * MLP_CODE));
     LX(rmalloc(&L.gcnt, size_t(nch) * C));
     LX(rmalloc(&L.gtot, C));
     LX(rmalloc(&L.gstart, size_t(C) + 1));
@@ -767,6 +827,8 @@ void learner_release(Learner& L) {
   rfree(L.codes);
   rfree(L.cent);
   rfree(L.cn);
+  rfree(L.chi);
+  rfree(L.clo);
   rfree(L.slab);
   rfree(L.lab);
   rfree(L.lab2);
@@ -975,12 +1037,14 @@ hipError_t learned_seed(Learner& L, const float* X, uint32_t N, uint32_t D, uint
   // k-means on the sample's codes, then the two nearest buckets of every vector
   k_km_init<<<(C * MLP_CODE + 255) / 256, 256, 0, st>>>(L.codes, samp, C, L.cent);
   for (uint32_t it = 0; it < c.kmeans_iters; ++it) {
-    k_km_norms<<<(C + 255) / 256, 256, 0, st>>>(L.cent, C, L.cn);
-    k_km_assign<<<(S + 255) / 256, 256, 0, st>>>(L.codes, samp, S, L.cent, L.cn, C, L.slab, nullptr);
+    k_km_norms<<<(C + 255) / 256, 256, 0, st>>>(L.cent, C, L.cn, L.chi, L.clo);
+    k_km_assign_mfma<1><<<(S + 63) / 64, 256, 0, st>>>(L.codes, samp, S, L.chi, L.clo, L.cn, C,
+                                                        L.slab, nullptr);
     k_km_update<<<C, 64, 0, st>>>(L.codes, samp, L.slab, S, L.cent);
   }
-  k_km_norms<<<(C + 255) / 256, 256, 0, st>>>(L.cent, C, L.cn);
-  k_km_assign<<<(N + 255) / 256, 256, 0, st>>>(L.codes, nullptr, N, L.cent, L.cn, C, L.lab, L.lab2);
+  k_km_norms<<<(C + 255) / 256, 256, 0, st>>>(L.cent, C, L.cn, L.chi, L.clo);
+  k_km_assign_mfma<4><<<(N + 255) / 256, 256, 0, st>>>(L.codes, nullptr, N, L.chi, L.clo, L.cn,
+                                                       C, L.lab, L.lab2);
   const uint32_t nch = (N + CH - 1) / CH;
   k_grp_hist<<<nch, 256, 0, st>>>(L.lab, N, C, L.gcnt);
   k_grp_scan<<<(C + 255) / 256, 256, 0, st>>>(L.gcnt, nch, C, L.gtot);

@@ -44,6 +44,7 @@ struct Learner {
   float* codes = nullptr;  // [N * CODE]
   uint32_t Ncap = 0, Ccap = 0;
   float *cent = nullptr, *cn = nullptr;
+  uint16_t *chi = nullptr, *clo = nullptr;  // centroids as BF16 high parts and remainders
   uint32_t *slab = nullptr, *lab = nullptr, *lab2 = nullptr, *gcnt = nullptr, *gtot = nullptr,
            *gstart = nullptr, *order = nullptr;
   uint32_t Scap = 0;
