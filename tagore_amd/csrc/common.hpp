@@ -445,6 +445,24 @@ __device__ __forceinline__ void reg_bitonic(T (&v)[NPL]) {
   }
 }
 
+// One key per lane: sorts each block of KMAX lanes (KMAX a power of two up to 64) ascending.
+// When only the first KMAX lanes hold keys below the rest, the whole wave ends up sorted.
+template <int KMAX, class T>
+__device__ __forceinline__ void reg_bitonic_upto(T& v) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int k = 2; k <= KMAX; k <<= 1) {
+    const bool asc = k == KMAX ? true : !((lane >> __builtin_ctz(unsigned(k))) & 1);
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const bool lower = !((lane >> __builtin_ctz(unsigned(j))) & 1);
+      const T p = lane_xor(v, j);
+      const bool take = (p < v) == (lower == asc);
+      v = take ? p : v;
+    }
+  }
+}
+
 // Sorts the bitonic sequence of the 64*NPL keys v (layout above) ascending.
 template <int NPL, class T>
 __device__ __forceinline__ void reg_bitonic_merge(T (&v)[NPL]) {

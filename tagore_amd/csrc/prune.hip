@@ -219,10 +219,13 @@ __device__ __forceinline__ void beam_search(const PruneArgs& a, const Row16<MC>&
     long long s3 = 0;
     if (small) {
       if (m > 0) {
-        uint64_t w1[1] = {v[0]};
-        reg_bitonic<1>(w1);
+        // keys in lanes 0..m-1, +inf above: sorting blocks of the next power of two is enough
+        uint64_t w1 = v[0];
+        if (m <= 16) reg_bitonic_upto<16>(w1);
+        else if (m <= 32) reg_bitonic_upto<32>(w1);
+        else reg_bitonic_upto<64>(w1);
         s3 = pclock(a);
-        const uint64_t rv = lane_xor64(w1[0], 63);
+        const uint64_t rv = lane_xor64(w1, 63);
         uint64_t c[1] = {bk[0] < rv ? bk[0] : rv};
         reg_bitonic_merge<1>(c);
         bk[0] = lane < TOPM ? c[0] : TG_KEY_INF;
