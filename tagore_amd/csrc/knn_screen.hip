@@ -469,6 +469,27 @@ __device__ __forceinline__ uint64_t key_min_lanes(float d, uint32_t id) {
   return (uint64_t(m) << 32) | k;
 }
 
+// Minimum over the 16 lanes of the lane's DPP row, in every lane of it: quad exchanges, then
+// row rotations by 4 and 8 (a minimum needs no exact partners).  Each step is one DPP-fused
+// v_min_u32.
+template <int CTRL>
+__device__ __forceinline__ uint32_t min_dpp(uint32_t m) {
+  return min(m, uint32_t(__builtin_amdgcn_update_dpp(int(0xFFFFFFFFu), int(m), CTRL, 0xF, 0xF,
+                                                     false)));
+}
+__device__ __forceinline__ uint32_t row_min16(uint32_t m) {
+  m = min_dpp<0xB1>(m);   // quad: lane ^ 1
+  m = min_dpp<0x4E>(m);   // quad: lane ^ 2
+  m = min_dpp<0x124>(m);  // row_ror:4
+  return min_dpp<0x128>(m);  // row_ror:8
+}
+// key_min_lanes<1, 16> with row_min16: the (d, id) minimum over the lane's 16-lane row.
+__device__ __forceinline__ uint64_t key_min_row16(float d, uint32_t id) {
+  const uint32_t db = __float_as_uint(d);
+  const uint32_t m = row_min16(db);
+  return (uint64_t(m) << 32) | row_min16(db == m ? id : 0xFFFFFFFFu);
+}
+
 // Membership of Q keys in kNN rows (rows sorted by (distance, id), canonical distances):
 // found[q] = key[q] is an entry of row v[q], for the searches with on[q].  The binary
 // searches run in lockstep, so the dependent loads of one step are in flight together.
@@ -528,8 +549,8 @@ __device__ __forceinline__ void lj_block_minima(int c, const f32x4& accB, const 
     const int i = 16 * w + 4 * g + r;
     const float sb = (i < nA && j < nB && A[i] != B[j]) ? (naA[i] + naB[j]) - 2.0f * accB[r] : INF;
     const float sa = (i < nA && j < nA && i != j) ? (naA[i] + naA[j]) - 2.0f * accA[r] : INF;
-    const uint64_t kb = key_min_lanes<1, 16>(sb, B[j]);
-    const uint64_t ka = key_min_lanes<1, 16>(sa, A[j]);
+    const uint64_t kb = key_min_row16(sb, B[j]);
+    const uint64_t ka = key_min_row16(sa, A[j]);
     if (t == 4 * c + r) {
       row_b = kb;
       row_a = ka;
