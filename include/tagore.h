@@ -207,7 +207,10 @@ typedef struct {
 } tg_large_params;
 typedef struct {
   uint32_t clusters, batches;      /* after splitting; batched builds */
-  double build_ms, merge_ms, total_ms;
+  double build_ms;                 /* reads and builds of every batch (the streaming merge
+                                      runs beside them in a thread of its own) */
+  double merge_ms;                 /* streaming merge left after the last build, final merge */
+  double total_ms;
   int32_t precision;               /* precision and tile the builds used */
   uint32_t tile;
   double device_ms;                /* summed device windows of the builds and prunes */

@@ -11,6 +11,9 @@
 // large_index.cu:121-280): clusters are packed back to back into segmented handles, so each
 // GNN-Descent and pruning kernel covers a whole batch of uneven clusters in one launch.
 // Each cluster's graph is independent of the batching (its seed is keyed by its id).
+// As in the reference's builder / merger threads, the host work overlaps the GPU: the next
+// batch's cluster files are read while a batch builds, and a merger thread runs the
+// streaming merge over every cluster whose batch is done.
 //
 // Decisions where the reference is undefined (all documented in tagore.h):
 //   * reorder() reads an uninitialised max_id / max_evict when no candidate overlaps the
@@ -22,11 +25,15 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <future>
 #include <map>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/tagore.h"
@@ -204,107 +211,84 @@ extern "C" int tg_large_index(const tg_large_params* p, tg_large_stats* out_stat
     batch_rows = std::max<size_t>(1, size_t(double(free_b) * 0.6 / (24.0 * 1024.0 + 8.0 * D)));
   }
   const auto t_build = clk::now();
-  uint32_t nbatches = 0;
   double device_ms = 0.0;
   int32_t precision = p->precision;
   uint32_t tile = p->tile;
   bool tuned = !p->tune;
+  // batches: consecutive positions [b0, b1) of the dispatch order
+  std::vector<std::pair<uint32_t, uint32_t>> batches;
   for (uint32_t b0 = 0; b0 < C;) {
     uint32_t b1 = b0;
     size_t rows = 0;
     while (b1 < C && (b1 == b0 || rows + clusters[order[b1]].size() <= batch_rows))
       rows += clusters[order[b1++]].size();
-    ++nbatches;
-    std::vector<float> X(rows * D);
-    std::vector<uint32_t> starts(1, 0), keys;
+    batches.push_back({b0, b1});
+    b0 = b1;
+  }
+  const uint32_t nbatches = uint32_t(batches.size());
+  // one batch's rows, read from its cluster files (status, message)
+  struct Rows {
+    std::vector<float> X;
+    std::vector<uint32_t> starts, keys;
+    int rc = TG_OK;
+    std::string msg;
+  };
+  auto read_batch = [&](uint32_t bi) {
+    Rows r;
+    const uint32_t b0 = batches[bi].first, b1 = batches[bi].second;
+    size_t rows = 0;
+    for (uint32_t k = b0; k < b1; ++k) rows += clusters[order[k]].size();
+    r.X.resize(rows * D);
+    r.starts.assign(1, 0);
     size_t r0 = 0;
     for (uint32_t k = b0; k < b1; ++k) {
       const uint32_t c = order[k];
       const size_t n = clusters[c].size();
       const std::string path = std::string(p->file_prefix) + std::to_string(c);
       std::ifstream in(path, std::ios::binary);
-      if (!in) return lfail(TG_ERR_IO, "cannot open cluster file " + path);
+      if (!in) {
+        r.rc = TG_ERR_IO;
+        r.msg = "cannot open cluster file " + path;
+        return r;
+      }
       uint32_t header = 0;
       in.read(reinterpret_cast<char*>(&header), 4);
-      in.read(reinterpret_cast<char*>(X.data() + r0 * D), std::streamsize(n * D * 4));
-      if (!in) return lfail(TG_ERR_IO, "cluster file " + path + " holds fewer than its points");
-      r0 += n;
-      starts.push_back(uint32_t(r0));
-      keys.push_back(c);
-    }
-    tg_state* s = nullptr;
-    int rc = tg_create(uint32_t(rows), D, p->devicelist[0], &s);
-    if (rc != TG_OK) return lfail(rc, tg_last_error());
-    auto bail = [&](int code) {
-      const std::string m = tg_last_error();
-      tg_free(s);
-      return lfail(code, m);
-    };
-    if ((rc = tg_set_data(s, X.data())) != TG_OK) return bail(rc);
-    if ((rc = tg_set_segments(s, b1 - b0, starts.data(), keys.data())) != TG_OK) return bail(rc);
-    if (!tuned) {
-      tg_tune_result tr{};
-      if ((rc = tg_autotune_knn(s, K, p->seed, &tr)) != TG_OK) return bail(rc);
-      precision = tr.precision;
-      tile = tr.tile;
-      tuned = true;
-      if (p->verbose)
-        for (uint32_t c = 0; c < tr.ncand; ++c)
-          printf("autotune: precision %d tile %u: %.2f ms\n", tr.cand_precision[c],
-                 tr.cand_tile[c], tr.cand_ms[c]);
-    }
-    tg_knn_params kp{};
-    kp.K = K;
-    kp.iters = p->iters ? p->iters : 16;
-    kp.seed = p->seed;
-    kp.precision = precision;
-    kp.tile = tile;
-    kp.init = TG_INIT_RANDOM;
-    if ((rc = tg_knn_build(s, &kp)) != TG_OK) return bail(rc);
-    tg_prune_params pp{};
-    pp.kind = TG_KIND_PATH;
-    pp.fwd_rule = pp.rev_rule = TG_RULE_DIST;
-    pp.FD = FD;
-    pp.TOPM = p->TOPM;
-    pp.thre = p->thre;
-    pp.precision = TG_PREC_FP32;
-    if ((rc = tg_prune(s, &pp)) != TG_OK) return bail(rc);
-    {
-      tg_stats ts{};
-      if ((rc = tg_get_stats(s, &ts)) != TG_OK) return bail(rc);
-      device_ms += ts.knn_ms + ts.prune_ms;
-    }
-    std::vector<uint32_t> deg(rows), ids(rows * FD), eps(b1 - b0);
-    uint32_t ep0 = 0;
-    if ((rc = tg_get_pruned(s, deg.data(), ids.data(), &ep0)) != TG_OK) return bail(rc);
-    if ((rc = tg_get_entry_points(s, eps.data())) != TG_OK) return bail(rc);
-    tg_free(s);
-    // local -> global ids (large_index.cu:265-271)
-    for (uint32_t k = b0; k < b1; ++k) {
-      const uint32_t c = order[k];
-      const uint32_t lo = starts[k - b0];
-      const std::vector<uint32_t>& mem = clusters[c];
-      std::vector<uint32_t>& g = graph[c];
-      g.assign(mem.size() * stride, 0u);
-      for (size_t j = 0; j < mem.size(); ++j) {
-        const uint32_t d = deg[lo + j];
-        g[j * stride] = d;
-        for (uint32_t t = 0; t < d; ++t) g[j * stride + 1 + t] = mem[ids[(lo + j) * FD + t] - lo];
+      in.read(reinterpret_cast<char*>(r.X.data() + r0 * D), std::streamsize(n * D * 4));
+      if (!in) {
+        r.rc = TG_ERR_IO;
+        r.msg = "cluster file " + path + " holds fewer than its points";
+        return r;
       }
-      metroid[k] = mem[eps[k - b0] - lo];
+      r0 += n;
+      r.starts.push_back(uint32_t(r0));
+      r.keys.push_back(c);
     }
-    if (p->verbose) printf("built clusters %u..%u of %u (%zu rows)\n", b0, b1 - 1, C, rows);
-    b0 = b1;
-  }
-  const auto t_merge = clk::now();
-  // ---- streaming merge (order_merge_new, large_index.cu:287-421)
+    return r;
+  };
+  // ---- streaming merge (order_merge_new, large_index.cu:287-421), in its own thread: the
+  // cluster at dispatch position i is merged once its batch is built (built > i)
   std::vector<uint32_t> pointers(Npts, 0), buffer(B, INV);
   std::vector<uint8_t> visited(Npts, 0), node_set(Npts, 0), cached(Npts, 0);
   std::vector<Spill> spill(C);
-  auto write_spill = [&](uint32_t clu) -> int {
+  std::mutex mu;
+  std::condition_variable cv;
+  uint32_t built = 0;
+  bool stop = false;
+  int merge_rc = TG_OK;
+  std::string merge_msg;
+  auto wait_built = [&](uint32_t pos) {
+    std::unique_lock<std::mutex> lk(mu);
+    cv.wait(lk, [&] { return built > pos || stop; });
+    return built > pos;
+  };
+  auto write_spill = [&](uint32_t clu) -> bool {
     const std::string path = std::string(p->local_index_path_prefix) + std::to_string(clu);
     FILE* f = fopen(path.c_str(), "wb");
-    if (!f) return lfail(TG_ERR_IO, "cannot write " + path);
+    if (!f) {
+      merge_rc = TG_ERR_IO;
+      merge_msg = "cannot write " + path;
+      return false;
+    }
     Spill& sp = spill[clu];
     const std::vector<uint32_t>& g = graph[clu];
     std::vector<uint32_t> rec;
@@ -324,58 +308,154 @@ extern "C" int tg_large_index(const tg_large_params* p, tg_large_stats* out_stat
     }
     const bool ok = fwrite(rec.data(), 4, rec.size(), f) == rec.size();
     fclose(f);
-    if (!ok) return lfail(TG_ERR_IO, "short write to " + path);
-    return TG_OK;
+    if (!ok) {
+      merge_rc = TG_ERR_IO;
+      merge_msg = "short write to " + path;
+    }
+    return ok;
   };
-  for (size_t i = 0; i < clusters[order[0]].size(); ++i) {
-    visited[clusters[order[0]][i]] = 1;
-    pointers[clusters[order[0]][i]] = uint32_t(i);
-  }
-  buffer[0] = order[0];
-  std::vector<uint32_t> nhood;
-  for (uint32_t i = 1; i < C; ++i) {
-    const uint32_t c = order[i];
-    std::vector<uint32_t>& gc = graph[c];
-    for (size_t j = 0; j < clusters[c].size(); ++j) {
-      const uint32_t node = clusters[c][j];
-      if (!visited[node]) {
-        visited[node] = 1;
-        pointers[node] = uint32_t(j);
-        continue;
-      }
-      if (pointers[node] == INV) continue;
-      const uint32_t ano = cen[node] == c ? cen2[node] : cen[node];
-      uint32_t* ra = graph[ano].data() + size_t(pointers[node]) * stride;
-      uint32_t* rc = gc.data() + j * stride;
-      const uint32_t* first = cen[node] != c ? ra : rc;
-      const uint32_t* second = cen[node] != c ? rc : ra;
-      for (uint32_t t = 1; t <= first[0]; ++t) {
-        node_set[first[t]] = 1;
-        nhood.push_back(first[t]);
-      }
-      for (uint32_t t = 1; t <= second[0]; ++t)
-        if (!node_set[second[t]]) {
-          node_set[second[t]] = 1;
-          nhood.push_back(second[t]);
+  auto stream_merge = [&]() {
+    if (!wait_built(0)) return;
+    for (size_t i = 0; i < clusters[order[0]].size(); ++i) {
+      visited[clusters[order[0]][i]] = 1;
+      pointers[clusters[order[0]][i]] = uint32_t(i);
+    }
+    buffer[0] = order[0];
+    std::vector<uint32_t> nhood;
+    for (uint32_t i = 1; i < C; ++i) {
+      if (!wait_built(i)) return;
+      const uint32_t c = order[i];
+      std::vector<uint32_t>& gc = graph[c];
+      for (size_t j = 0; j < clusters[c].size(); ++j) {
+        const uint32_t node = clusters[c][j];
+        if (!visited[node]) {
+          visited[node] = 1;
+          pointers[node] = uint32_t(j);
+          continue;
         }
-      const uint32_t nd = std::min<uint32_t>(uint32_t(nhood.size()), p->max_degree);
-      for (uint32_t t = 0; t < nd; ++t) ra[1 + t] = nhood[t];
-      ra[0] = nd;
-      rc[0] = 0;
-      for (uint32_t v : nhood) node_set[v] = 0;
-      nhood.clear();
-      cached[node] = 1;
+        if (pointers[node] == INV) continue;
+        const uint32_t ano = cen[node] == c ? cen2[node] : cen[node];
+        uint32_t* ra = graph[ano].data() + size_t(pointers[node]) * stride;
+        uint32_t* rc = gc.data() + j * stride;
+        const uint32_t* first = cen[node] != c ? ra : rc;
+        const uint32_t* second = cen[node] != c ? rc : ra;
+        for (uint32_t t = 1; t <= first[0]; ++t) {
+          node_set[first[t]] = 1;
+          nhood.push_back(first[t]);
+        }
+        for (uint32_t t = 1; t <= second[0]; ++t)
+          if (!node_set[second[t]]) {
+            node_set[second[t]] = 1;
+            nhood.push_back(second[t]);
+          }
+        const uint32_t nd = std::min<uint32_t>(uint32_t(nhood.size()), p->max_degree);
+        for (uint32_t t = 0; t < nd; ++t) ra[1 + t] = nhood[t];
+        ra[0] = nd;
+        rc[0] = 0;
+        for (uint32_t v : nhood) node_set[v] = 0;
+        nhood.clear();
+        cached[node] = 1;
+      }
+      buffer[position[c]] = c;
+      if (i + 1 >= B && i + 1 < C) {
+        const uint32_t clu = buffer[position[order[i + 1]]];
+        if (clu != INV && !write_spill(clu)) return;
+      }
     }
-    buffer[position[c]] = c;
-    if (i + 1 >= B && i + 1 < C) {
-      const uint32_t clu = buffer[position[order[i + 1]]];
-      if (clu != INV)
-        if (int rc = write_spill(clu)) return rc;
+    for (uint32_t k = 0; k < B; ++k)
+      if (buffer[k] != INV && !write_spill(buffer[k])) return;
+  };
+  std::thread merger(stream_merge);
+  auto finish_merger = [&]() {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      stop = true;
     }
+    cv.notify_all();
+    merger.join();
+  };
+  // ---- batched construction on the GPU; the next batch's files are read meanwhile
+  std::future<Rows> pending = std::async(std::launch::async, read_batch, 0u);
+  for (uint32_t bi = 0; bi < nbatches; ++bi) {
+    Rows cur = pending.get();
+    if (bi + 1 < nbatches) pending = std::async(std::launch::async, read_batch, bi + 1);
+    auto bail = [&](int code, const std::string& m, tg_state* s) {
+      if (s) tg_free(s);
+      if (pending.valid()) pending.wait();
+      finish_merger();
+      return lfail(code, m);
+    };
+    if (cur.rc != TG_OK) return bail(cur.rc, cur.msg, nullptr);
+    const uint32_t b0 = batches[bi].first, b1 = batches[bi].second;
+    const size_t rows = cur.X.size() / D;
+    tg_state* s = nullptr;
+    int rc = tg_create(uint32_t(rows), D, p->devicelist[0], &s);
+    if (rc != TG_OK) return bail(rc, tg_last_error(), nullptr);
+    if ((rc = tg_set_data(s, cur.X.data())) != TG_OK) return bail(rc, tg_last_error(), s);
+    if ((rc = tg_set_segments(s, b1 - b0, cur.starts.data(), cur.keys.data())) != TG_OK)
+      return bail(rc, tg_last_error(), s);
+    if (!tuned) {
+      tg_tune_result tr{};
+      if ((rc = tg_autotune_knn(s, K, p->seed, &tr)) != TG_OK) return bail(rc, tg_last_error(), s);
+      precision = tr.precision;
+      tile = tr.tile;
+      tuned = true;
+      if (p->verbose)
+        for (uint32_t c = 0; c < tr.ncand; ++c)
+          printf("autotune: precision %d tile %u: %.2f ms\n", tr.cand_precision[c],
+                 tr.cand_tile[c], tr.cand_ms[c]);
+    }
+    tg_knn_params kp{};
+    kp.K = K;
+    kp.iters = p->iters ? p->iters : 16;
+    kp.seed = p->seed;
+    kp.precision = precision;
+    kp.tile = tile;
+    kp.init = TG_INIT_RANDOM;
+    if ((rc = tg_knn_build(s, &kp)) != TG_OK) return bail(rc, tg_last_error(), s);
+    tg_prune_params pp{};
+    pp.kind = TG_KIND_PATH;
+    pp.fwd_rule = pp.rev_rule = TG_RULE_DIST;
+    pp.FD = FD;
+    pp.TOPM = p->TOPM;
+    pp.thre = p->thre;
+    pp.precision = TG_PREC_FP32;
+    if ((rc = tg_prune(s, &pp)) != TG_OK) return bail(rc, tg_last_error(), s);
+    {
+      tg_stats ts{};
+      if ((rc = tg_get_stats(s, &ts)) != TG_OK) return bail(rc, tg_last_error(), s);
+      device_ms += ts.knn_ms + ts.prune_ms;
+    }
+    std::vector<uint32_t> deg(rows), ids(rows * FD), eps(b1 - b0);
+    uint32_t ep0 = 0;
+    if ((rc = tg_get_pruned(s, deg.data(), ids.data(), &ep0)) != TG_OK)
+      return bail(rc, tg_last_error(), s);
+    if ((rc = tg_get_entry_points(s, eps.data())) != TG_OK) return bail(rc, tg_last_error(), s);
+    tg_free(s);
+    // local -> global ids (large_index.cu:265-271)
+    for (uint32_t k = b0; k < b1; ++k) {
+      const uint32_t c = order[k];
+      const uint32_t lo = cur.starts[k - b0];
+      const std::vector<uint32_t>& mem = clusters[c];
+      std::vector<uint32_t>& g = graph[c];
+      g.assign(mem.size() * stride, 0u);
+      for (size_t j = 0; j < mem.size(); ++j) {
+        const uint32_t d = deg[lo + j];
+        g[j * stride] = d;
+        for (uint32_t t = 0; t < d; ++t) g[j * stride + 1 + t] = mem[ids[(lo + j) * FD + t] - lo];
+      }
+      metroid[k] = mem[eps[k - b0] - lo];
+    }
+    if (p->verbose) printf("built clusters %u..%u of %u (%zu rows)\n", b0, b1 - 1, C, rows);
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      built = b1;
+    }
+    cv.notify_all();
   }
-  for (uint32_t k = 0; k < B; ++k)
-    if (buffer[k] != INV)
-      if (int rc = write_spill(buffer[k])) return rc;
+  const auto t_merge = clk::now();
+  merger.join();
+  if (merge_rc != TG_OK) return lfail(merge_rc, merge_msg);
   // ---- final merge into the DiskANN layout (large_index.cu:423-521)
   FILE* out = fopen(p->index_store_path, "wb");
   if (!out) return lfail(TG_ERR_IO, std::string("cannot write ") + p->index_store_path);
@@ -384,6 +464,7 @@ extern "C" int tg_large_index(const tg_large_params* p, tg_large_stats* out_stat
   uint64_t index_size = 0;
   const uint64_t zero64 = 0;
   const uint32_t md = p->max_degree;
+  std::vector<uint32_t> nhood;
   bool ok = fwrite(&zero64, 8, 1, out) == 1 && fwrite(&md, 4, 1, out) == 1 &&
             fwrite(&md, 4, 1, out) == 1 && fwrite(&zero64, 8, 1, out) == 1;
   auto next = [&](uint32_t clu, const uint32_t** ids) -> uint32_t {
