@@ -97,6 +97,46 @@ def test_large_index_rejects(tmp_path):
     assert os.path.getsize(os.path.join(tmp, "i.bin")) > 24
 
 
+def test_large_index_reports_io_errors_of_overlapped_stages(tmp_path):
+    """The file reader (a batch ahead of the builds) and the merger thread (beside them)
+    report their I/O errors as status codes, and the library stays usable afterwards."""
+    tmp = str(tmp_path)
+    X, cen, cen2, cpath = _make_case(tmp, Npts=900, ncl=4, seed=5)
+    clusters, *_ = L.clusters_from_centres(cen, cen2, 4, 10 ** 6)
+    prefix = _write_clusters(tmp, X, clusters)
+    def args(pre, local, index):
+        return (32, 24, 4, 10 ** 6, pre, local, index, 20, 2, 900, cpath, 1.2, 16, 32)
+    # a later cluster's file is missing: found while earlier batches build
+    gone = os.path.join(tmp, "gone_")
+    for c in range(len(clusters)):
+        if c != 2:
+            os.link(prefix + str(c), gone + str(c))
+    _say("largeIndex: missing cluster file")
+    with pytest.raises(tagore_amd.TagoreError, match="cannot open cluster file"):
+        tagore_amd.large_index(*args(gone, os.path.join(tmp, "l1_"), os.path.join(tmp, "i1")),
+                               batch_rows=1)
+    # a truncated cluster file
+    short = os.path.join(tmp, "short_")
+    for c in range(len(clusters)):
+        data = open(prefix + str(c), "rb").read()
+        open(short + str(c), "wb").write(data[:-4] if c == 1 else data)
+    _say("largeIndex: truncated cluster file")
+    with pytest.raises(tagore_amd.TagoreError, match="holds fewer than its points"):
+        tagore_amd.large_index(*args(short, os.path.join(tmp, "l2_"), os.path.join(tmp, "i2")),
+                               batch_rows=1)
+    # the spill files cannot be written: the merger thread's error
+    _say("largeIndex: unwritable spill prefix")
+    with pytest.raises(tagore_amd.TagoreError, match="cannot write"):
+        tagore_amd.large_index(*args(prefix, os.path.join(tmp, "no_dir", "l3_"),
+                                     os.path.join(tmp, "i3")), batch_rows=1)
+    # and a valid call afterwards, batched and per cluster, gives one index
+    _say("largeIndex: valid calls after the errors")
+    tagore_amd.large_index(*args(prefix, os.path.join(tmp, "l4_"), os.path.join(tmp, "i4")))
+    tagore_amd.large_index(*args(prefix, os.path.join(tmp, "l5_"), os.path.join(tmp, "i5")),
+                           batch_rows=1)
+    assert open(os.path.join(tmp, "i4"), "rb").read() == open(os.path.join(tmp, "i5"), "rb").read()
+
+
 def test_dropin_large_index(tmp_path):
     """The drop-in module's largeIndex (reference argument order) writes the same index."""
     tmp = str(tmp_path)
