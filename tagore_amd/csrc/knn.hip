@@ -353,7 +353,8 @@ __global__ __launch_bounds__(64) void k_merge(KnnArgs a) {
 // The runs land in the reverse PushBuf as its grouped overflow with cap 0 (cnt, ov_off,
 // ov_sorted), the form pb_top reads; partitioned keys and destinations use its slot area and
 // ov_dest as scratch.  The order inside a run is unspecified: pb_top selects by (d, id).
-constexpr int REV_THREADS = 512;
+constexpr int REV_THREADS = 1024;  // 16 waves per workgroup: one workgroup per CU and tile
+                                   // (hist, part) or block (runs)
 constexpr uint32_t REV_MAXP = 8192;   // destination blocks (LDS counters of hist and part)
 constexpr uint32_t REV_MAXR = 8192;   // ids per block (LDS counters of runs)
 constexpr uint32_t REV_TILES = 256;
@@ -366,7 +367,7 @@ struct RevPlan {
 
 RevPlan rev_plan(uint32_t N, uint32_t K, uint32_t* H) {
   RevPlan r;
-  r.lgR = 9;  // at least 512 ids per block, so every thread of k_rev_runs owns one
+  r.lgR = 10;  // at least 1024 ids per block, so every thread of k_rev_runs owns one
   while (r.lgR < 13 && (size_t(N) >> r.lgR) > 256) ++r.lgR;
   r.P = uint32_t((size_t(N) + (size_t(1) << r.lgR) - 1) >> r.lgR);
   r.n = uint64_t(N) * K;
