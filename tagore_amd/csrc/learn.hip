@@ -24,7 +24,13 @@ namespace {
 constexpr int GT = 64;  // GEMM tile edge
 constexpr int GK = 32;  // GEMM k step
 constexpr int GPAD = 8;
-constexpr uint32_t CH = 4096;          // grouping chunk
+// Grouping chunk: at least 1024 vectors and at most about 1024 chunks, so the stable scatter
+// (one wave per chunk, serial within it) has about four waves per CU and the per-chunk
+// counts stay near 1024 x C words.
+inline uint32_t grp_chunk(uint32_t N) {
+  const uint32_t c = ((N + 1023) / 1024 + 63) / 64 * 64;
+  return c < 1024 ? 1024 : c;
+}
 constexpr uint32_t MAX_BUCKETS = 4096;
 constexpr uint32_t ENC_CHUNK = 131072;
 
@@ -515,7 +521,7 @@ __global__ __launch_bounds__(64) void k_km_update(const float* __restrict__ Z,
 
 // ------------------------------------------ stable grouping of the vectors by bucket label
 __global__ __launch_bounds__(256) void k_grp_hist(const uint32_t* __restrict__ lab, uint32_t N,
-                                                  uint32_t C, uint32_t* cnt) {
+                                                  uint32_t C, uint32_t CH, uint32_t* cnt) {
   __shared__ uint32_t h[MAX_BUCKETS];
   for (uint32_t b = threadIdx.x; b < C; b += 256) h[b] = 0;
   __syncthreads();
@@ -565,7 +571,8 @@ __global__ __launch_bounds__(1024) void k_grp_start(const uint32_t* __restrict__
 }
 // order[start[b] + rank] = u with rank = number of smaller ids labelled b (one wave per chunk)
 __global__ __launch_bounds__(64) void k_grp_scatter(const uint32_t* __restrict__ lab, uint32_t N,
-                                                    uint32_t C, const uint32_t* __restrict__ off,
+                                                    uint32_t C, uint32_t CH,
+                                                    const uint32_t* __restrict__ off,
                                                     const uint32_t* __restrict__ start,
                                                     uint32_t* order) {
   __shared__ uint32_t base[MAX_BUCKETS];
@@ -787,7 +794,7 @@ hipError_t learner_reserve(Learner& L, uint32_t D, uint32_t M, uint32_t N, uint3
     LX(rmalloc(&L.slab, S));
     L.Scap = S;
   }
-  const uint32_t nch = (N + CH - 1) / CH;
+  const uint32_t nch = (N + grp_chunk(N) - 1) / grp_chunk(N);
   if (L.Ccap < C) {
     LX(rmalloc(&L.cent, size_t(C) * MLP_CODE));
     LX(rmalloc(&L.cn, C));
@@ -1045,11 +1052,11 @@ hipError_t learned_seed(Learner& L, const float* X, uint32_t N, uint32_t D, uint
   k_km_norms<<<(C + 255) / 256, 256, 0, st>>>(L.cent, C, L.cn, L.chi, L.clo);
   k_km_assign_mfma<4><<<(N + 255) / 256, 256, 0, st>>>(L.codes, nullptr, N, L.chi, L.clo, L.cn,
                                                        C, L.lab, L.lab2);
-  const uint32_t nch = (N + CH - 1) / CH;
-  k_grp_hist<<<nch, 256, 0, st>>>(L.lab, N, C, L.gcnt);
+  const uint32_t ch = grp_chunk(N), nch = (N + ch - 1) / ch;
+  k_grp_hist<<<nch, 256, 0, st>>>(L.lab, N, C, ch, L.gcnt);
   k_grp_scan<<<(C + 255) / 256, 256, 0, st>>>(L.gcnt, nch, C, L.gtot);
   k_grp_start<<<1, 1024, 0, st>>>(L.gtot, C, L.gstart);
-  k_grp_scatter<<<nch, 64, 0, st>>>(L.lab, N, C, L.gcnt, L.gstart, L.order);
+  k_grp_scatter<<<nch, 64, 0, st>>>(L.lab, N, C, ch, L.gcnt, L.gstart, L.order);
   if (stats) {
     stats->ms_cluster = now_ms(tc);
     stats->buckets = C;

@@ -134,7 +134,8 @@ void pb_group_overflow(const PushBuf& pb, uint32_t N, uint32_t* fill, uint32_t* 
   k_scan1<<<nb, 256, 0, s>>>(pb.cnt, pb.cap, N, pb.ov_off, scan_tmp);
   k_scan2<<<1, 256, 0, s>>>(scan_tmp, nb, pb.ov_off, N);
   k_scan3<<<nb, 256, 0, s>>>(pb.ov_off, scan_tmp, N);
-  k_ov_scatter<<<256, 256, 0, s>>>(pb, fill);
+  // a grid-stride loop of returning atomics: 8 workgroups per CU keep enough waves in flight
+  k_ov_scatter<<<2048, 256, 0, s>>>(pb, fill);
 }
 
 // ---------------------------------------------------------------------- synthetic data
