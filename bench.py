@@ -42,12 +42,12 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def measure(st, args, precision, iters, init, label, graph_iters=False):
+def measure(st, args, precision, iters, init, label, graph_iters=False, graph=False):
     import tagore_amd
 
     def step():
         st.knn_build(args.k, iters, args.seed, precision, init=init, graph_iters=graph_iters,
-                     sample=args.sample, epochs=args.epochs)
+                     graph=graph, sample=args.sample, epochs=args.epochs)
         st.prune("path", "dist", "dist", args.fd, args.topm, 1.0,
                  args.prune_rows if precision == "bf16" else "fp32")
 
@@ -473,9 +473,11 @@ def main():
             ablation = measure(st, args, args.precision, args.ablation_iters, "random",
                                "ablation")
     head = measure(st, args, args.precision, args.iters, args.init, "headline")
-    # the same step with the GNN-Descent iterations replayed from a captured HIP graph
+    # the same step with the encoder's training loop and the GNN-Descent iterations replayed
+    # from captured HIP graphs
     head_graph = measure(st, args, args.precision, args.iters, args.init,
-                         "headline, graph-captured iterations", graph_iters=True)
+                         "headline, graph-captured training and iterations", graph_iters=True,
+                         graph=True)
     fdeg, _, _ = st.get_forward()
     fwd_edges = int(fdeg.astype(np.int64).sum())
     if args.phase_clocks:

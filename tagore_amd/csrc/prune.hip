@@ -805,12 +805,23 @@ __global__ __launch_bounds__(64) void k_cal_ep(PruneArgs a, const float* centroi
 }
 
 // Deterministic centroid: FP64 sums over chunks of 1024 rows, then over chunks in order.
+// The centroid's FP64 sums in index order (chunks of 1024 rows, then the chunk partials):
+// each step loads 16 values together and adds them in order, so 16 loads are in flight per
+// lane and the sums are the serial ones.
 __global__ void k_centroid_part(const float* X, uint32_t N, uint32_t D, double* part) {
   const uint32_t j = blockIdx.y * blockDim.x + threadIdx.x;
   if (j >= D) return;
   const size_t lo = size_t(blockIdx.x) * 1024, hi = min(size_t(N), lo + 1024);
   double s = 0.0;
-  for (size_t i = lo; i < hi; ++i) s += double(X[i * D + j]);
+  size_t i = lo;
+  for (; i + 16 <= hi; i += 16) {
+    float v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = X[(i + q) * D + j];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) s += double(v[q]);
+  }
+  for (; i < hi; ++i) s += double(X[i * D + j]);
   part[size_t(blockIdx.x) * D + j] = s;
 }
 __global__ void k_centroid_final(const double* part, uint32_t nch, uint32_t N, uint32_t D,
@@ -818,7 +829,15 @@ __global__ void k_centroid_final(const double* part, uint32_t nch, uint32_t N, u
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= D) return;
   double s = 0.0;
-  for (uint32_t ch = 0; ch < nch; ++ch) s += part[size_t(ch) * D + j];
+  uint32_t ch = 0;
+  for (; ch + 16 <= nch; ch += 16) {
+    double v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = part[size_t(ch + q) * D + j];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) s += v[q];
+  }
+  for (; ch < nch; ++ch) s += part[size_t(ch) * D + j];
   c[j] = float(s / double(N));
 }
 
