@@ -120,7 +120,7 @@ struct Spill {
 
 }  // namespace
 
-extern "C" int tg_large_index(const tg_large_params* p, tg_large_stats* out_stats) {
+static int large_index(const tg_large_params* p, tg_large_stats* out_stats) {
   using clk = std::chrono::steady_clock;
   const auto t_start = clk::now();
   if (!p) return lfail(TG_ERR_ARG, "null params");
@@ -365,32 +365,56 @@ extern "C" int tg_large_index(const tg_large_params* p, tg_large_stats* out_stat
     for (uint32_t k = 0; k < B; ++k)
       if (buffer[k] != INV && !write_spill(buffer[k])) return;
   };
-  std::thread merger(stream_merge);
-  auto finish_merger = [&]() {
-    {
-      std::lock_guard<std::mutex> lk(mu);
-      stop = true;
+  std::thread merger([&] {
+    try {
+      stream_merge();
+    } catch (const std::bad_alloc&) {
+      merge_rc = TG_ERR_CAPACITY;
+      merge_msg = "largeIndex merge: host memory exhausted";
+    } catch (const std::exception& e) {
+      merge_rc = TG_ERR_STATE;
+      merge_msg = std::string("largeIndex merge: ") + e.what();
     }
-    cv.notify_all();
-    merger.join();
+  });
+  // every way out of this function (errors and exceptions included) stops and joins it
+  struct MergerStop {
+    std::thread& t;
+    std::mutex& mu;
+    std::condition_variable& cv;
+    bool& stop;
+    ~MergerStop() {
+      if (!t.joinable()) return;
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        stop = true;
+      }
+      cv.notify_all();
+      t.join();
+    }
+  } merger_stop{merger, mu, cv, stop};
+  // a batch's handle, freed on every way out of its iteration
+  struct Handle {
+    tg_state* s = nullptr;
+    ~Handle() {
+      if (s) tg_free(s);
+    }
   };
   // ---- batched construction on the GPU; the next batch's files are read meanwhile
   std::future<Rows> pending = std::async(std::launch::async, read_batch, 0u);
   for (uint32_t bi = 0; bi < nbatches; ++bi) {
     Rows cur = pending.get();
     if (bi + 1 < nbatches) pending = std::async(std::launch::async, read_batch, bi + 1);
-    auto bail = [&](int code, const std::string& m, tg_state* s) {
-      if (s) tg_free(s);
+    Handle h;
+    auto bail = [&](int code, const std::string& m, tg_state*) {
       if (pending.valid()) pending.wait();
-      finish_merger();
       return lfail(code, m);
     };
     if (cur.rc != TG_OK) return bail(cur.rc, cur.msg, nullptr);
     const uint32_t b0 = batches[bi].first, b1 = batches[bi].second;
     const size_t rows = cur.X.size() / D;
-    tg_state* s = nullptr;
-    int rc = tg_create(uint32_t(rows), D, p->devicelist[0], &s);
+    int rc = tg_create(uint32_t(rows), D, p->devicelist[0], &h.s);
     if (rc != TG_OK) return bail(rc, tg_last_error(), nullptr);
+    tg_state* s = h.s;
     if ((rc = tg_set_data(s, cur.X.data())) != TG_OK) return bail(rc, tg_last_error(), s);
     if ((rc = tg_set_segments(s, b1 - b0, cur.starts.data(), cur.keys.data())) != TG_OK)
       return bail(rc, tg_last_error(), s);
@@ -431,7 +455,8 @@ extern "C" int tg_large_index(const tg_large_params* p, tg_large_stats* out_stat
     if ((rc = tg_get_pruned(s, deg.data(), ids.data(), &ep0)) != TG_OK)
       return bail(rc, tg_last_error(), s);
     if ((rc = tg_get_entry_points(s, eps.data())) != TG_OK) return bail(rc, tg_last_error(), s);
-    tg_free(s);
+    tg_free(h.s);  // device memory back before the next batch's handle
+    h.s = nullptr;
     // local -> global ids (large_index.cu:265-271)
     for (uint32_t k = b0; k < b1; ++k) {
       const uint32_t c = order[k];
@@ -535,4 +560,16 @@ extern "C" int tg_large_index(const tg_large_params* p, tg_large_stats* out_stat
     out_stats->device_ms = device_ms;
   }
   return TG_OK;
+}
+
+extern "C" int tg_large_index(const tg_large_params* p, tg_large_stats* out_stats) {
+  // host buffers of GB size (a batch's rows, the cluster graphs) can fail to allocate: that
+  // and any other exception of the host side becomes a status, never an unwinding C call
+  try {
+    return large_index(p, out_stats);
+  } catch (const std::bad_alloc&) {
+    return lfail(TG_ERR_CAPACITY, "largeIndex: host memory exhausted");
+  } catch (const std::exception& e) {
+    return lfail(TG_ERR_STATE, std::string("largeIndex: ") + e.what());
+  }
 }
