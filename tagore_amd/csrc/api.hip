@@ -404,6 +404,19 @@ uint64_t splitmix64(uint64_t& x) {
 }  // namespace
 
 // ===================================================================== C ABI
+// Host-side exceptions (allocation failures of data-sized host buffers, mostly) become
+// status codes: nothing unwinds through the C boundary.
+template <class F>
+static int guarded(F&& f) {
+  try {
+    return f();
+  } catch (const std::bad_alloc&) {
+    return fail(TG_ERR_CAPACITY, "host memory exhausted");
+  } catch (const std::exception& e) {
+    return fail(TG_ERR_STATE, e.what());
+  }
+}
+
 extern "C" {
 
 const char* tg_last_error(void) { return g_err.c_str(); }
@@ -672,8 +685,9 @@ int tg_read_fbin(const char* path, float** rows, uint32_t* N, uint32_t* D) {
 // Vamana max degree, the DPG largest degree, the CAGRA d); deg [N] and ids [N*width]
 // (malloc'ed, padded with INV); eps: the entry points the file names (NSG / Vamana: 1,
 // NSSG: 10, DPG / CAGRA: 0).
-int tg_read_index(const char* path, const char* index_type, uint32_t N, uint32_t* width,
-                  uint32_t** deg, uint32_t** ids, uint32_t* eps, uint32_t* n_eps) {
+static int read_index_impl(const char* path, const char* index_type, uint32_t N,
+                           uint32_t* width, uint32_t** deg, uint32_t** ids, uint32_t* eps,
+                           uint32_t* n_eps) {
   if (!path || !index_type || !width || !deg || !ids || !eps || !n_eps)
     return fail(TG_ERR_ARG, "null argument");
   *deg = nullptr;
@@ -792,6 +806,11 @@ int tg_read_index(const char* path, const char* index_type, uint32_t N, uint32_t
   *deg = dg;
   *ids = id;
   return TG_OK;
+}
+int tg_read_index(const char* path, const char* index_type, uint32_t N, uint32_t* width,
+                  uint32_t** deg, uint32_t** ids, uint32_t* eps, uint32_t* n_eps) {
+  return guarded(
+      [&] { return read_index_impl(path, index_type, N, width, deg, ids, eps, n_eps); });
 }
 
 // Installs a pruned graph (deg [N], ids [N*width], entry point) as the handle's current one,
@@ -1029,7 +1048,7 @@ void swap_lists(tg_state* s) {
 
 }  // namespace
 
-int tg_knn_build(tg_state* s, const tg_knn_params* p) {
+static int knn_build_impl(tg_state* s, const tg_knn_params* p) {
   if (!s || !p) return fail(TG_ERR_ARG, "null argument");
   const uint32_t K = p->K;
   if (K < 2 || K > 128) return fail(TG_ERR_ARG, "K must be in [2, 128]");
@@ -1251,6 +1270,9 @@ int tg_knn_build(tg_state* s, const tg_knn_params* p) {
   s->order_valid = use_order;
   return TG_OK;
 }
+int tg_knn_build(tg_state* s, const tg_knn_params* p) {
+  return guarded([&] { return knn_build_impl(s, p); });
+}
 
 int tg_set_knn(tg_state* s, uint32_t K, const uint32_t* ids, const float* dists) {
   if (!s || !ids || !dists) return fail(TG_ERR_ARG, "null argument");
@@ -1290,7 +1312,7 @@ int tg_get_knn(tg_state* s, uint32_t* ids, float* dists) {
 }
 
 // -------------------------------------------------------------------------- prune
-int tg_prune(tg_state* s, const tg_prune_params* p) {
+static int prune_impl(tg_state* s, const tg_prune_params* p) {
   if (!s || !p) return fail(TG_ERR_ARG, "null argument");
   if (!s->has_knn) return fail(TG_ERR_STATE, "prune needs a kNN graph (build or set one)");
   const uint32_t K = s->K, FD = p->FD;
@@ -1446,6 +1468,9 @@ int tg_prune(tg_state* s, const tg_prune_params* p) {
   s->last_prune = *p;
   return TG_OK;
 }
+int tg_prune(tg_state* s, const tg_prune_params* p) {
+  return guarded([&] { return prune_impl(s, p); });
+}
 
 int tg_get_pruned(tg_state* s, uint32_t* deg, uint32_t* ids, uint32_t* ep) {
   if (!s) return fail(TG_ERR_ARG, "null handle");
@@ -1477,7 +1502,8 @@ int tg_get_forward(tg_state* s, uint32_t* deg, uint32_t* ids, float* dists) {
 }
 
 // ------------------------------------------------------------------------ writers
-int tg_set_segments(tg_state* s, uint32_t nseg, const uint32_t* starts, const uint32_t* keys) {
+static int set_segments_impl(tg_state* s, uint32_t nseg, const uint32_t* starts,
+                             const uint32_t* keys) {
   if (!s) return fail(TG_ERR_ARG, "null handle");
   HIPX(hipSetDevice(s->device));
   if (nseg == 0) {
@@ -1515,6 +1541,9 @@ int tg_set_segments(tg_state* s, uint32_t nseg, const uint32_t* starts, const ui
   s->has_prune = false;
   s->order_valid = false;
   return TG_OK;
+}
+int tg_set_segments(tg_state* s, uint32_t nseg, const uint32_t* starts, const uint32_t* keys) {
+  return guarded([&] { return set_segments_impl(s, nseg, starts, keys); });
 }
 
 int tg_autotune_knn(tg_state* s, uint32_t K, uint64_t seed, tg_tune_result* out) {
@@ -1571,7 +1600,8 @@ int tg_get_entry_points(tg_state* s, uint32_t* eps) {
   return TG_OK;
 }
 
-int tg_store_index(tg_state* s, const char* path, const char* index_type, uint64_t nssg_seed) {
+static int store_index_impl(tg_state* s, const char* path, const char* index_type,
+                            uint64_t nssg_seed) {
   if (s && s->nseg) return fail(TG_ERR_STATE, "index files hold one graph: clear the segments");
   if (!s || !path || !index_type) return fail(TG_ERR_ARG, "null argument");
   if (!s->has_prune) return fail(TG_ERR_STATE, "no pruned graph to store");
@@ -1652,6 +1682,9 @@ int tg_store_index(tg_state* s, const char* path, const char* index_type, uint64
   ok = (fclose(f) == 0) && ok;
   if (!ok) return fail(TG_ERR_IO, std::string("short write to ") + path);
   return TG_OK;
+}
+int tg_store_index(tg_state* s, const char* path, const char* index_type, uint64_t nssg_seed) {
+  return guarded([&] { return store_index_impl(s, path, index_type, nssg_seed); });
 }
 
 // ------------------------------------------------------------------------ recall
@@ -1890,7 +1923,7 @@ int tg_search_recall(tg_state* s, uint32_t nq, const float* queries, uint32_t k,
   return rc;
 }
 
-int tg_recall(tg_state* s, uint32_t nq, uint64_t seed, uint32_t k, double* recall) {
+static int recall_impl(tg_state* s, uint32_t nq, uint64_t seed, uint32_t k, double* recall) {
   if (s && s->nseg) return fail(TG_ERR_STATE, "recall is judged on a handle without segments");
   if (!s || !recall) return fail(TG_ERR_ARG, "null argument");
   if (!s->has_knn) return fail(TG_ERR_STATE, "no kNN graph");
@@ -1936,6 +1969,9 @@ int tg_recall(tg_state* s, uint32_t nq, uint64_t seed, uint32_t k, double* recal
   dfree(kth);
   dfree(cand_d);
   return rc;
+}
+int tg_recall(tg_state* s, uint32_t nq, uint64_t seed, uint32_t k, double* recall) {
+  return guarded([&] { return recall_impl(s, nq, seed, k, recall); });
 }
 
 int tg_buffer(tg_state* s, int32_t idx, void** dev_ptr, size_t* bytes) {
