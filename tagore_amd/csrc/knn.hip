@@ -297,9 +297,9 @@ struct RowLds {
   uint8_t vf[128];
 };
 
-// loads row u into LDS: keys, flags, ids sorted ascending (padding INV)
+// loads row u into LDS: keys, flags, and (sort_ids) the ids sorted ascending (padding INV)
 __device__ void load_row_lds(const KnnArgs& a, const uint32_t* G, const float* GD, uint32_t u,
-                             RowLds& r) {
+                             RowLds& r, bool sort_ids = true) {
   const int lane = threadIdx.x, K = a.K, n = pow2_at_least(K);
   for (int j = lane; j < n; j += TG_WAVE) {
     if (j < K) {
@@ -312,7 +312,7 @@ __device__ void load_row_lds(const KnnArgs& a, const uint32_t* G, const float* G
     }
   }
   __syncthreads();
-  wave_sort<128>(r.gid, n);
+  if (sort_ids) wave_sort<128>(r.gid, n);
 }
 
 __device__ __forceinline__ bool listed(const RowLds& r, int K, uint32_t id) {
@@ -327,13 +327,14 @@ __global__ __launch_bounds__(64) void k_merge(KnnArgs a) {
   const uint32_t v = blockIdx.x;
   if (v >= a.N) return;
   const int K = a.K;
-  load_row_lds(a, a.G, a.GD, v, row);
+  // Every push was checked by the local join against this very row (the merge runs after it,
+  // before any row changes) and entered only below its worst distance and when not listed;
+  // a build's distances are canonical (its lists start from the initial sort), so a listed
+  // id would have arrived with its listed key.  No id search is needed here.
+  load_row_lds(a, a.G, a.GD, v, row, false);
   const float worst = key_d(row.gk[K - 1]);
   const int nc = pb_top(a.pc, v, K, true,
-                        [&](uint64_t k) {
-                          return key_id(k) != v && key_d(k) < worst && !listed(row, K, key_id(k));
-                        },
-                        buf);
+                        [&](uint64_t k) { return key_id(k) != v && key_d(k) < worst; }, buf);
   if (nc > 0)
     merge_row(K, row.gk, row.vf, buf, nc, a.G + size_t(v) * K, a.GD + size_t(v) * K,
               a.V + size_t(v) * K);
