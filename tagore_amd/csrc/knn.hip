@@ -659,7 +659,7 @@ __global__ __launch_bounds__(64) void k_phase2(KnnArgs a) {
   if (u >= a.N) return;
   const int lane = threadIdx.x, g = lane >> 4, t = lane & 15;
   const int K = a.K, D = a.D;
-  load_row_lds(a, a.G, a.GD, u, row);
+  load_row_lds(a, a.G, a.GD, u, row, false);
   // leaders: first LEADERS unvisited positions in list order
   int nl = 0;
   for (int base = 0; base < K && nl < LEADERS; base += TG_WAVE) {
@@ -690,11 +690,22 @@ __global__ __launch_bounds__(64) void k_phase2(KnnArgs a) {
       ids[off[l] + i] =
           i < nf ? a.G[size_t(v) * K + i] : key_id(a.R0[size_t(v) * K + (i - nf)]);
   }
-  // unique (an LDS id set in kv's space, 1024 slots for at most 512 ids: the insert that
-  // finds the slot empty keeps the id), not u, not listed; gather order (the keys are sorted
-  // afterwards, so the order of evaluation does not matter)
+  // unique, not u, not listed, in gather order (the keys are sorted afterwards, so the order
+  // of evaluation does not matter): an LDS id set in kv's space, 1024 slots for the K listed
+  // ids, inserted first, and at most 512 gathered ones; the insert that finds the slot empty
+  // keeps the id
   uint32_t* hs = reinterpret_cast<uint32_t*>(kv);
   for (int i = lane; i < 1024; i += TG_WAVE) hs[i] = TG_INV;
+  __syncthreads();
+  for (int j = lane; j < K; j += TG_WAVE) {
+    const uint32_t id = key_id(row.gk[j]);
+    uint32_t h = (id * 0x9E3779B1u) >> 22;
+    for (;;) {
+      const uint32_t old = atomicCAS(&hs[h], TG_INV, id);
+      if (old == TG_INV || old == id) break;
+      h = (h + 1) & 1023u;
+    }
+  }
   __syncthreads();
   int cnt = 0;
   for (int base = 0; base < tot; base += TG_WAVE) {
@@ -713,7 +724,6 @@ __global__ __launch_bounds__(64) void k_phase2(KnnArgs a) {
           if (old == id) break;
           h = (h + 1) & 1023u;
         }
-        ok = ok && !listed(row, K, id);
       }
     }
     // in place: this chunk was read above, and writes land at or below the read positions
