@@ -293,31 +293,18 @@ __device__ void merge_row(int K, const uint64_t* gk, const uint8_t* vf, const ui
 
 struct RowLds {
   uint64_t gk[128];
-  uint32_t gid[128];
   uint8_t vf[128];
 };
 
-// loads row u into LDS: keys, flags, and (sort_ids) the ids sorted ascending (padding INV)
+// loads row u into LDS: its keys (sorted by (d, id), as every list is) and visit flags
 __device__ void load_row_lds(const KnnArgs& a, const uint32_t* G, const float* GD, uint32_t u,
-                             RowLds& r, bool sort_ids = true) {
-  const int lane = threadIdx.x, K = a.K, n = pow2_at_least(K);
-  for (int j = lane; j < n; j += TG_WAVE) {
-    if (j < K) {
-      const uint32_t id = G[size_t(u) * K + j];
-      r.gk[j] = mk_key(GD[size_t(u) * K + j], id);
-      r.gid[j] = id;
-      r.vf[j] = a.V[size_t(u) * K + j];
-    } else {
-      r.gid[j] = TG_INV;
-    }
+                             RowLds& r) {
+  const int lane = threadIdx.x, K = a.K;
+  for (int j = lane; j < K; j += TG_WAVE) {
+    r.gk[j] = mk_key(GD[size_t(u) * K + j], G[size_t(u) * K + j]);
+    r.vf[j] = a.V[size_t(u) * K + j];
   }
   __syncthreads();
-  if (sort_ids) wave_sort<128>(r.gid, n);
-}
-
-__device__ __forceinline__ bool listed(const RowLds& r, int K, uint32_t id) {
-  const int p = lower_bound_lds(r.gid, K, id);
-  return p < K && r.gid[p] == id;
 }
 
 // merge of the local-join candidates
@@ -331,7 +318,7 @@ __global__ __launch_bounds__(64) void k_merge(KnnArgs a) {
   // before any row changes) and entered only below its worst distance and when not listed;
   // a build's distances are canonical (its lists start from the initial sort), so a listed
   // id would have arrived with its listed key.  No id search is needed here.
-  load_row_lds(a, a.G, a.GD, v, row, false);
+  load_row_lds(a, a.G, a.GD, v, row);
   const float worst = key_d(row.gk[K - 1]);
   const int nc = pb_top(a.pc, v, K, true,
                         [&](uint64_t k) { return key_id(k) != v && key_d(k) < worst; }, buf);
@@ -659,7 +646,7 @@ __global__ __launch_bounds__(64) void k_phase2(KnnArgs a) {
   if (u >= a.N) return;
   const int lane = threadIdx.x, g = lane >> 4, t = lane & 15;
   const int K = a.K, D = a.D;
-  load_row_lds(a, a.G, a.GD, u, row, false);
+  load_row_lds(a, a.G, a.GD, u, row);
   // leaders: first LEADERS unvisited positions in list order
   int nl = 0;
   for (int base = 0; base < K && nl < LEADERS; base += TG_WAVE) {
@@ -886,6 +873,8 @@ __global__ __launch_bounds__(64) void k_final(KnnArgs a) {
   const uint32_t u = blockIdx.x;
   if (u >= a.N) return;
   const int lane = threadIdx.x, K = a.K;
+  // A reverse entry (d(w, u), w) carries the canonical distance, which is symmetric, so w is
+  // listed exactly when that key is in the row: a search of the sorted keys, no id sort.
   load_row_lds(a, a.G, a.GD, u, row);
   const float worst = key_d(row.gk[K - 1]);
   const int nr = a.Rn0[u];
@@ -896,7 +885,8 @@ __global__ __launch_bounds__(64) void k_final(KnnArgs a) {
     uint64_t k = 0;
     if (i < nr) {
       k = a.R0[size_t(u) * K + i];
-      ok = key_id(k) != u && key_d(k) < worst && !listed(row, K, key_id(k));
+      const int p = lower_bound_lds(row.gk, K, k);
+      ok = key_id(k) != u && key_d(k) < worst && !(p < K && row.gk[p] == k);
     }
     const uint64_t m = ballot(ok);
     if (ok) cand[nc + mask_rank(m)] = k;
