@@ -260,12 +260,36 @@ __device__ __forceinline__ void beam_search(const PruneArgs& a, const Row16<MC>&
 }
 
 // -------------------------------------------------------------------------------------
-// Sort a pool of L keys (padded to n with TG_KEY_INF) and replace the id of every entry
-// whose id equals its predecessor's (original values) by INV.
+// The reverse pass's pool: the forward row (deg keys, ascending: selections come in key
+// order) and the reverse top (nr keys, ascending) as one sorted sequence of 64 NPL keys,
+// +inf after L = deg + nr.  [forward | +inf | reverse, last first] is bitonic, so one
+// bitonic merge gives what a sort of the union gives.
 // -------------------------------------------------------------------------------------
-__device__ void sort_dedup(uint64_t* P, int n) {
+template <int NPL>
+__device__ __forceinline__ void merge_pool(const PruneArgs& a, uint32_t u, int deg,
+                                           const uint64_t* rev, int nr, uint64_t* P) {
   const int lane = threadIdx.x;
-  wave_sort<256>(P, n);
+  constexpr int n = 64 * NPL;
+  uint64_t v[NPL];
+#pragma unroll
+  for (int b = 0; b < NPL; ++b) {
+    const int i = b * 64 + lane;
+    v[b] = TG_KEY_INF;
+    if (i < deg) v[b] = mk_key(a.fwd_d[size_t(u) * a.FD + i], a.fwd_ids[size_t(u) * a.FD + i]);
+    else if (i >= n - nr) v[b] = rev[n - 1 - i];
+  }
+  reg_bitonic_merge<NPL>(v);
+#pragma unroll
+  for (int b = 0; b < NPL; ++b) P[b * 64 + lane] = v[b];
+  __syncthreads();
+}
+
+// -------------------------------------------------------------------------------------
+// In a sorted pool of n keys, replace the id of every entry whose id equals its
+// predecessor's (original values) by INV.
+// -------------------------------------------------------------------------------------
+__device__ void mark_dups(uint64_t* P, int n) {
+  const int lane = threadIdx.x;
   bool dup[POOL_MAX / TG_WAVE];
 #pragma unroll
   for (int q = 0; q < POOL_MAX / TG_WAVE; ++q) {
@@ -614,15 +638,11 @@ __global__ __launch_bounds__(64) void k_reverse(PruneArgs a) {
   uint32_t* row = a.out_ids + size_t(u) * FD;
   uint32_t n_flt = 0;
   if (L > FD) {
-    const int n = pow2_at_least(L);
-    for (int i = lane; i < n; i += TG_WAVE) {
-      uint64_t k = TG_KEY_INF;
-      if (i < deg) k = mk_key(a.fwd_d[size_t(u) * FD + i], a.fwd_ids[size_t(u) * FD + i]);
-      else if (i < L) k = buf[i - deg];
-      P[i] = k;
-    }
-    __syncthreads();
-    sort_dedup(P, n);
+    const int n = pow2_at_least(max(L, TG_WAVE));  // L <= FD + K <= 256
+    if (n == 64) merge_pool<1>(a, u, deg, buf, nr, P);
+    else if (n == 128) merge_pool<2>(a, u, deg, buf, nr, P);
+    else merge_pool<4>(a, u, deg, buf, nr, P);
+    mark_dups(P, n);
     PhaseAcc ph;  // not reported for the reverse pass
     const int nd = filter_pool<MC, T>(a, P, L, u, a.rev_rule, sel, cl, n_flt, ph);
     __syncthreads();
