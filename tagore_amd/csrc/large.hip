@@ -42,6 +42,7 @@
 namespace {
 
 constexpr uint32_t INV = 0xFFFFFFFFu;
+constexpr uint32_t TUNE_ROWS = 1u << 20;  // rows the autotuner's probes build
 
 int lfail(int code, const std::string& msg) { return tg::set_error(code, msg); }
 
@@ -412,15 +413,22 @@ static int large_index(const tg_large_params* p, tg_large_stats* out_stats) {
     if (cur.rc != TG_OK) return bail(cur.rc, cur.msg, nullptr);
     const uint32_t b0 = batches[bi].first, b1 = batches[bi].second;
     const size_t rows = cur.X.size() / D;
-    int rc = tg_create(uint32_t(rows), D, p->devicelist[0], &h.s);
-    if (rc != TG_OK) return bail(rc, tg_last_error(), nullptr);
-    tg_state* s = h.s;
-    if ((rc = tg_set_data(s, cur.X.data())) != TG_OK) return bail(rc, tg_last_error(), s);
-    if ((rc = tg_set_segments(s, b1 - b0, cur.starts.data(), cur.keys.data())) != TG_OK)
-      return bail(rc, tg_last_error(), s);
+    int rc = TG_OK;
     if (!tuned) {
+      // Every variant builds the same graphs, so the probes time them on the batch's first
+      // clusters, about a million rows (at least one cluster), in a handle of their own that
+      // is freed before the batch's.
+      size_t nt = 1;
+      while (nt < cur.keys.size() && cur.starts[nt + 1] <= TUNE_ROWS) ++nt;
+      Handle t;
+      if ((rc = tg_create(cur.starts[nt], D, p->devicelist[0], &t.s)) != TG_OK)
+        return bail(rc, tg_last_error(), nullptr);
+      if ((rc = tg_set_data(t.s, cur.X.data())) != TG_OK ||
+          (rc = tg_set_segments(t.s, uint32_t(nt), cur.starts.data(), cur.keys.data())) != TG_OK)
+        return bail(rc, tg_last_error(), nullptr);
       tg_tune_result tr{};
-      if ((rc = tg_autotune_knn(s, K, p->seed, &tr)) != TG_OK) return bail(rc, tg_last_error(), s);
+      if ((rc = tg_autotune_knn(t.s, K, p->seed, &tr)) != TG_OK)
+        return bail(rc, tg_last_error(), nullptr);
       precision = tr.precision;
       tile = tr.tile;
       tuned = true;
@@ -429,6 +437,12 @@ static int large_index(const tg_large_params* p, tg_large_stats* out_stats) {
           printf("autotune: precision %d tile %u: %.2f ms\n", tr.cand_precision[c],
                  tr.cand_tile[c], tr.cand_ms[c]);
     }
+    if ((rc = tg_create(uint32_t(rows), D, p->devicelist[0], &h.s)) != TG_OK)
+      return bail(rc, tg_last_error(), nullptr);
+    tg_state* s = h.s;
+    if ((rc = tg_set_data(s, cur.X.data())) != TG_OK) return bail(rc, tg_last_error(), s);
+    if ((rc = tg_set_segments(s, b1 - b0, cur.starts.data(), cur.keys.data())) != TG_OK)
+      return bail(rc, tg_last_error(), s);
     tg_knn_params kp{};
     kp.K = K;
     kp.iters = p->iters ? p->iters : 16;
