@@ -534,8 +534,18 @@ __global__ __launch_bounds__(256) void k_grp_hist(const uint32_t* __restrict__ l
 __global__ void k_grp_scan(uint32_t* cnt, uint32_t nch, uint32_t C, uint32_t* tot) {
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= C) return;
-  uint32_t run = 0;
-  for (uint32_t ch = 0; ch < nch; ++ch) {
+  uint32_t run = 0, ch = 0;
+  for (; ch + 16 <= nch; ch += 16) {  // 16 counts in flight per lane
+    uint32_t t[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) t[q] = cnt[size_t(ch + q) * C + b];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      cnt[size_t(ch + q) * C + b] = run;
+      run += t[q];
+    }
+  }
+  for (; ch < nch; ++ch) {
     const uint32_t t = cnt[size_t(ch) * C + b];
     cnt[size_t(ch) * C + b] = run;
     run += t;
