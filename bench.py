@@ -23,6 +23,12 @@ by a beam search with held-out queries.  The roofline entry is the pruning forwa
 the step's largest, timed live with HIP events on the library's stream: its compulsory HBM
 bytes against the HBM peak, the measured HBM traffic, and its vector-ALU issue fraction from
 the committed counter run.  The CPU oracle runs the same pipeline on a bounded sample.
+`end_to_end` times, once and by wall clock, the same rows uploaded from host memory and
+prepared, the headline build and prune, and the NSG index file written.
+
+Other workloads (--workload): small (10k x 128 in full, the GPU against the host CPU, graphs
+compared), unit (1M x 96 unit vectors + NSSG), wide (5M x 960 + DPG/CAGRA), clusters (the
+large-index pipeline on one GPU's cluster share).
 
     python bench.py [--gpus 1] [--steps 2] [--warmup 1]
 """
@@ -377,6 +383,53 @@ def bench_wide(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_small(args):
+    """The CPU-runnable configuration in full: 10k x 128 integer GMM(64), sigma 20, GNN-Descent
+    K=32 for 10 iterations from random lists and NSG (FD=32, m=64), plain FP32, on the GPU and
+    on the host CPU (the OpenMP oracle); the two pruned graphs are compared."""
+    import numpy as np
+    import oracle
+    import tagore_amd
+
+    N, D, K, FD, M, it, seed = 10000, 128, 32, 32, 64, 10, 0x7A600001
+    st = tagore_amd.State(N, D, 0)
+    st.gen_data(0, 64, seed, 20.0)
+    X = st.get_data()
+
+    def step():
+        st.knn_build(K, it, seed, "fp32")
+        st.prune("path", "dist", "dist", FD, M, 1.0, "fp32")
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    gpu_s = (time.perf_counter() - t) / args.steps
+    deg, ids, ep = st.get_pruned()
+    st.close()
+    t = time.perf_counter()
+    G, GD, _ = oracle.gnn_descent(X, K, it, seed)
+    ref = oracle.prune(X, G, GD, FD, M, 0, 0, 0, 1.0)
+    cpu_s = time.perf_counter() - t
+    same = (np.array_equal(deg, ref["deg"]) and ep == ref["ep"] and
+            all(np.array_equal(ids[j, :deg[j]], ref["ids"][j, :deg[j]]) for j in range(N)))
+    log(f"small: GPU {gpu_s * 1e3:.1f} ms per step, CPU {cpu_s:.2f} s, identical {same}")
+    out = {
+        "metric": "kNN+NSG vectors/s, plain FP32, 1 GPU (10k x 128 in full, against the host CPU)",
+        "value": N / gpu_s, "unit": "vectors/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": gpu_s * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": f"GNN-Descent K={K} {it} iterations (random init) + NSG FD={FD} "
+                               f"m={M} over {N}x{D} integer GMM(64), sigma 20",
+                   "N": N, "dim": D, "K": K, "iters": it},
+        "cpu_baseline": {"value": N / cpu_s, "unit": "vectors/s", "cores": oracle.num_threads(),
+                         "kind": "port", "seconds": cpu_s, "sample": "the whole configuration"},
+        "identical_to_cpu": bool(same),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -405,6 +458,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--search-queries", type=int, default=10000,
                     help="held-out queries for the beam-search check of the NSG index (0: off)")
+    ap.add_argument("--end-to-end", type=int, default=1,
+                    help="also time upload + build + prune + index file once (wall clock)")
     ap.add_argument("--components", type=int, default=1,
                     help="count the weakly connected components of the kNN and NSG graphs")
     ap.add_argument("--phase-clocks", action="store_true",
@@ -414,8 +469,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=100000,
                     help="vectors in the CPU-oracle baseline sample (0 skips it)")
     ap.add_argument("--workload", default="headline",
-                    choices=["headline", "clusters", "unit", "wide"],
-                    help="clusters: the large-index pipeline on one GPU's cluster share; "
+                    choices=["headline", "small", "clusters", "unit", "wide"],
+                    help="small: 10k x 128 in full, GPU against the host CPU; "
+                         "clusters: the large-index pipeline on one GPU's cluster share; "
                          "unit: 1M x 96 unit vectors + NSSG; wide: 5M x 960 FP8 + DPG/CAGRA")
     ap.add_argument("--clusters", type=int, default=64)
     ap.add_argument("--cluster-median", type=int, default=400000,
@@ -432,6 +488,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "small":
+        bench_small(args)
+        return
     if args.workload == "clusters":
         bench_clusters(args)
         return
@@ -576,6 +635,31 @@ def main():
             st.set_profiling(True)
         log(f"prune {name}: {min(ms):.1f} ms, {other_prunes[name]}")
 
+    # The end-to-end window (the reference's load -> build -> prune -> store): the same rows
+    # uploaded from host memory and prepared (u8 copy, norms), the headline build and prune,
+    # the NSG index file written.  Wall clock around the library calls, once.
+    e2e = None
+    if args.end_to_end:
+        Xh = st.get_data()
+        path = os.path.join(os.environ.get("TMPDIR", "/tmp"), f"tagore_e2e_{os.getpid()}.nsg")
+        t0 = time.perf_counter()
+        st.set_data(Xh)
+        t1 = time.perf_counter()
+        st.knn_build(K, args.iters, args.seed, args.precision, init=args.init,
+                     sample=args.sample, epochs=args.epochs)
+        st.prune("path", "dist", "dist", args.fd, args.topm, 1.0,
+                 args.prune_rows if args.precision == "bf16" else "fp32")
+        t2 = time.perf_counter()
+        st.store_index(path, "NSG")
+        t3 = time.perf_counter()
+        e2e = {"upload_prepare_ms": (t1 - t0) * 1e3, "build_prune_ms": (t2 - t1) * 1e3,
+               "store_ms": (t3 - t2) * 1e3, "total_ms": (t3 - t0) * 1e3,
+               "vectors_per_s": N / (t3 - t0), "index_bytes": os.path.getsize(path),
+               "host_rows_bytes": int(Xh.nbytes)}
+        os.remove(path)
+        del Xh
+        log(f"end to end: {e2e}")
+
     cpu = None
     if args.cpu_sample > 0:
         import oracle
@@ -612,6 +696,7 @@ def main():
         "index_search_recall_at_10": next((r["recall_at_10"] for r in search["sweep"]
                                            if r["L"] == 64), None) if search else None,
         "headline": head,
+        "end_to_end": e2e,
         "headline_graph_iterations": {k: head_graph[k] for k in
                                       ("ms_per_step", "vectors_per_s", "knn_ms", "prune_ms",
                                        "recall_at_10")},
