@@ -1620,6 +1620,21 @@ static int store_index_impl(tg_state* s, const char* path, const char* index_typ
   if (!f) return fail(TG_ERR_IO, std::string("cannot open ") + path + " for writing");
   bool ok = true;
   auto put = [&](const void* p, size_t n) { ok = ok && fwrite(p, 1, n, f) == n; };
+  // the rows are packed into one buffer and written with one call (not two per row)
+  std::vector<uint32_t> body;
+  auto put_rows = [&](int lead, bool full_width) {
+    size_t words = 0;
+    for (uint32_t u = 0; u < N; ++u) words += size_t(lead) + (full_width ? FD : deg[u]);
+    body.resize(words);
+    uint32_t* o = body.data();
+    for (uint32_t u = 0; u < N; ++u) {
+      const uint32_t n = full_width ? FD : deg[u];
+      for (int l = 0; l < lead; ++l) *o++ = full_width ? FD : deg[u];
+      memcpy(o, &ids[size_t(u) * FD], 4ull * n);
+      o += n;
+    }
+    put(body.data(), 4 * words);
+  };
   const uint32_t width = FD, ep = s->ep;
   if (kind == 0 || kind == 2) {
     put(&width, 4);
@@ -1641,10 +1656,7 @@ static int store_index_impl(tg_state* s, const char* path, const char* index_typ
       put(&n_ep, 4);
       put(eps, 40);
     }
-    for (uint32_t u = 0; u < N; ++u) {
-      put(&deg[u], 4);
-      put(&ids[size_t(u) * FD], 4ull * deg[u]);
-    }
+    put_rows(1, false);
   } else if (kind == 1) {
     uint64_t size = 24;
     uint32_t maxd = 0;
@@ -1657,10 +1669,7 @@ static int store_index_impl(tg_state* s, const char* path, const char* index_typ
     put(&maxd, 4);
     put(&ep, 4);
     put(&frozen, 8);
-    for (uint32_t u = 0; u < N; ++u) {
-      put(&deg[u], 4);
-      put(&ids[size_t(u) * FD], 4ull * deg[u]);
-    }
+    put_rows(1, false);
   } else if (kind == 3) {
     const char magic[8] = {'K', 'N', 'N', 'G', 'R', 'A', 'P', 'H'};
     const uint32_t vmaj = 2, vmin = 0;
@@ -1668,16 +1677,9 @@ static int store_index_impl(tg_state* s, const char* path, const char* index_typ
     put(&vmaj, 4);
     put(&vmin, 4);
     put(&N, 4);
-    for (uint32_t u = 0; u < N; ++u) {
-      put(&deg[u], 4);
-      put(&deg[u], 4);
-      put(&ids[size_t(u) * FD], 4ull * deg[u]);
-    }
+    put_rows(2, false);
   } else {
-    for (uint32_t u = 0; u < N; ++u) {
-      put(&width, 4);
-      put(&ids[size_t(u) * FD], 4ull * FD);
-    }
+    put_rows(1, true);
   }
   ok = (fclose(f) == 0) && ok;
   if (!ok) return fail(TG_ERR_IO, std::string("short write to ") + path);
