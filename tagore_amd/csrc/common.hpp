@@ -389,8 +389,9 @@ __device__ __forceinline__ uint32_t lane_xor32(uint32_t v, int j) {
     case 1: return uint32_t(__builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false));
     case 2: return uint32_t(__builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false));
     case 4: {
-      // banks 0 and 2 of each 16-lane row read lane + 4, banks 1 and 3 read lane - 4
-      const int lo = __builtin_amdgcn_update_dpp(x, x, 0x104, 0xF, 0x5, false);
+      // banks 0 and 2 of each 16-lane row read lane + 4, banks 1 and 3 read lane - 4 (the
+      // second move fills the lanes the first leaves unwritten)
+      const int lo = __builtin_amdgcn_mov_dpp(x, 0x104, 0xF, 0x5, false);
       return uint32_t(__builtin_amdgcn_update_dpp(lo, x, 0x114, 0xF, 0xA, false));
     }
     case 8: return uint32_t(__builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, false));
@@ -404,15 +405,29 @@ __device__ __forceinline__ uint64_t lane_xor64(uint64_t v, int j) {
 __device__ __forceinline__ uint32_t lane_xor(uint32_t v, int j) { return lane_xor32(v, j); }
 __device__ __forceinline__ uint64_t lane_xor(uint64_t v, int j) { return lane_xor64(v, j); }
 
-// Bitonic sort of the 64*NPL keys v (layout above), ascending (descending with DESC).  The
-// direction tests depend only on lane bits, so they are formed once as lane masks; a step
-// then costs one 64-bit compare and two selects per key besides the exchange.
+// The lanes whose index has bit b set (b < 6), as a wave mask.
+__device__ __forceinline__ constexpr uint64_t lanes_with_bit(int b) {
+  return b == 0   ? 0xAAAAAAAAAAAAAAAAull
+         : b == 1 ? 0xCCCCCCCCCCCCCCCCull
+         : b == 2 ? 0xF0F0F0F0F0F0F0F0ull
+         : b == 3 ? 0xFF00FF00FF00FF00ull
+         : b == 4 ? 0xFFFF0000FFFF0000ull
+                  : 0xFFFFFFFF00000000ull;
+}
+// Whether a compare-exchange lane takes its partner's key p over its own x: p < x, negated
+// in the lanes of `flip`.  The direction of every step depends only on lane bits, so flip is
+// a constant wave mask: one 64-bit compare, one scalar XOR with the constant and the selects
+// (a mask derived from the lane index at run time would be held in scalar registers across
+// the whole sort, and spilled where they run short).
+template <class T>
+__device__ __forceinline__ bool take_partner(T p, T x, uint64_t flip) {
+  return __builtin_amdgcn_inverse_ballot_w64(__builtin_amdgcn_ballot_w64(p < x) ^ flip);
+}
+
+// Bitonic sort of the 64*NPL keys v (layout above), ascending (descending with DESC).  A
+// step costs one 64-bit compare and two selects per key besides the exchange.
 template <int NPL, class T, bool DESC = false>
 __device__ __forceinline__ void reg_bitonic(T (&v)[NPL]) {
-  const int lane = lane_id();
-  bool bit[6];
-#pragma unroll
-  for (int b = 0; b < 6; ++b) bit[b] = (lane >> b) & 1;
   constexpr int N = 64 * NPL;
 #pragma unroll
   for (int k = 2; k <= N; k <<= 1) {
@@ -431,14 +446,16 @@ __device__ __forceinline__ void reg_bitonic(T (&v)[NPL]) {
           v[r2] = sw ? x : y;
         }
       } else {
-        const bool lower = !bit[__builtin_ctz(unsigned(j))];
+        // the lower position of an ascending block keeps the smaller key: the lane takes
+        // the smaller key where (lower position) == (ascending block), else the larger
+        const uint64_t upper = lanes_with_bit(__builtin_ctz(unsigned(j)));
 #pragma unroll
         for (int r = 0; r < NPL; ++r) {
-          const bool asc = (k >= 64 ? ((r << 6) & k) == 0 : !bit[__builtin_ctz(unsigned(k))]) != DESC;
+          uint64_t desc;  // lanes in a descending block
+          if (k >= 64) desc = ((((r << 6) & k) == 0) != DESC) ? 0ull : ~0ull;
+          else desc = lanes_with_bit(__builtin_ctz(unsigned(k))) ^ (DESC ? ~0ull : 0ull);
           const T x = v[r], p = lane_xor(x, j);
-          // the lower position of an ascending block keeps the smaller key
-          const bool take = (p < x) == (lower == asc);
-          v[r] = take ? p : x;
+          v[r] = take_partner(p, x, upper ^ desc) ? p : x;
         }
       }
     }
@@ -449,16 +466,13 @@ __device__ __forceinline__ void reg_bitonic(T (&v)[NPL]) {
 // When only the first KMAX lanes hold keys below the rest, the whole wave ends up sorted.
 template <int KMAX, class T>
 __device__ __forceinline__ void reg_bitonic_upto(T& v) {
-  const int lane = lane_id();
 #pragma unroll
   for (int k = 2; k <= KMAX; k <<= 1) {
-    const bool asc = k == KMAX ? true : !((lane >> __builtin_ctz(unsigned(k))) & 1);
+    const uint64_t desc = k == KMAX ? 0ull : lanes_with_bit(__builtin_ctz(unsigned(k)));
 #pragma unroll
     for (int j = k >> 1; j > 0; j >>= 1) {
-      const bool lower = !((lane >> __builtin_ctz(unsigned(j))) & 1);
       const T p = lane_xor(v, j);
-      const bool take = (p < v) == (lower == asc);
-      v = take ? p : v;
+      v = take_partner(p, v, lanes_with_bit(__builtin_ctz(unsigned(j))) ^ desc) ? p : v;
     }
   }
 }
@@ -466,7 +480,6 @@ __device__ __forceinline__ void reg_bitonic_upto(T& v) {
 // Sorts the bitonic sequence of the 64*NPL keys v (layout above) ascending.
 template <int NPL, class T>
 __device__ __forceinline__ void reg_bitonic_merge(T (&v)[NPL]) {
-  const int lane = lane_id();
 #pragma unroll
   for (int j = 32 * NPL; j > 0; j >>= 1) {
     if (j >= 64) {
@@ -480,11 +493,11 @@ __device__ __forceinline__ void reg_bitonic_merge(T (&v)[NPL]) {
         v[r | rj] = sw ? x : y;
       }
     } else {
-      const bool lower = (lane & j) == 0;
+      const uint64_t upper = lanes_with_bit(__builtin_ctz(unsigned(j)));
 #pragma unroll
       for (int r = 0; r < NPL; ++r) {
         const T x = v[r], p = lane_xor(x, j);
-        v[r] = ((p < x) == lower) ? p : x;
+        v[r] = take_partner(p, x, upper) ? p : x;
       }
     }
   }
@@ -492,7 +505,6 @@ __device__ __forceinline__ void reg_bitonic_merge(T (&v)[NPL]) {
 // reg_bitonic_merge with a 32-bit payload moved along with each key
 template <int NPL>
 __device__ __forceinline__ void reg_bitonic_merge_kv(uint64_t (&v)[NPL], uint32_t (&f)[NPL]) {
-  const int lane = lane_id();
 #pragma unroll
   for (int j = 32 * NPL; j > 0; j >>= 1) {
     if (j >= 64) {
@@ -509,12 +521,14 @@ __device__ __forceinline__ void reg_bitonic_merge_kv(uint64_t (&v)[NPL], uint32_
         f[r | rj] = sw ? fx : fy;
       }
     } else {
-      const bool lower = (lane & j) == 0;
+      const uint64_t upper = lanes_with_bit(__builtin_ctz(unsigned(j)));
 #pragma unroll
       for (int r = 0; r < NPL; ++r) {
         const uint64_t x = v[r], p = lane_xor64(x, j);
         const uint32_t pf = lane_xor32(f[r], j);
-        const bool take = lower ? (p < x) : (x < p);  // equal keys keep their own payload
+        // lower lanes take p < x, upper ones x < p: equal keys keep their own payload
+        const bool take = __builtin_amdgcn_inverse_ballot_w64(
+            (__builtin_amdgcn_ballot_w64(p < x) & ~upper) | (__builtin_amdgcn_ballot_w64(x < p) & upper));
         v[r] = take ? p : x;
         f[r] = take ? pf : f[r];
       }
