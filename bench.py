@@ -79,8 +79,12 @@ def measure(st, args, precision, iters, init, label, graph_iters=False, graph=Fa
     ms = (t1 - t0) * 1e3 / args.steps
     last = st.stats()
     rec = st.recall(min(args.recall_queries, st.N), 10, 0x5A11)
-    log(f"{label}: {ms:.1f} ms/step, recall@10 = {rec:.4f}")
+    # the stored FP32 distances of the same judged rows against FP64 (bound 1e-5 relative)
+    derr = st.dist_check(min(args.recall_queries, st.N), 0x5A11)
+    log(f"{label}: {ms:.1f} ms/step, recall@10 = {rec:.4f}, stored distances within "
+        f"{derr:.2e} of FP64")
     return dict(ms_per_step=ms, vectors_per_s=st.N / (ms / 1e3), recall_at_10=rec,
+                dist_max_rel_err=derr, dist_within_1e5=bool(derr <= 1e-5),
                 precision=precision, iters=iters, init=init,
                 learn_ms=last["learn_ms"], learn_loss=last["learn_loss"],
                 knn_ms=sum(knn_ms) / len(knn_ms), prune_ms=sum(prune_ms) / len(prune_ms),

@@ -221,6 +221,11 @@ int tg_large_index(const tg_large_params* p, tg_large_stats* stats);
  * counts as a hit).  Queries: the first nq of a seeded permutation prefix. */
 int tg_recall(tg_state* s, uint32_t nq, uint64_t seed, uint32_t k, double* recall);
 
+/* Largest relative difference |d32 - d64| / d64 between the stored FP32 distances of the
+ * kNN rows of the same nq sample nodes as tg_recall (same seed) and their FP64 distances.
+ * Measurement only: no counterpart in the reference, whose distances this judges. */
+int tg_knn_dist_check(tg_state* s, uint32_t nq, uint64_t seed, double* max_rel_err);
+
 /* Device buffers in the reference's capsule order (pybind.cu:132-141):
  * 0 graph u32[N*K], 1 reverse graph, 2 data f32[N*D], 3 distances f32[N*K],
  * 4 reverse distances, 5 reverse counts, 6 centroid f32[D], 7 sample lists u32[N*128]. */

@@ -152,6 +152,7 @@ def lib():
     L.tg_pruned_width.argtypes = [vp, P(c.c_uint32)]
     L.tg_store_index.argtypes = [vp, c.c_char_p, c.c_char_p, c.c_uint64]
     L.tg_recall.argtypes = [vp, c.c_uint32, c.c_uint64, c.c_uint32, P(c.c_double)]
+    L.tg_knn_dist_check.argtypes = [vp, c.c_uint32, c.c_uint64, P(c.c_double)]
     L.tg_buffer.argtypes = [vp, c.c_int32, P(vp), P(c.c_size_t)]
     L.tg_get_stats.argtypes = [vp, P(Stats)]
     L.tg_set_profiling.argtypes = [vp, c.c_int]
@@ -346,6 +347,13 @@ class State:
     def recall(self, nq: int, k: int = 10, seed: int = 0x5A11):
         r = ctypes.c_double()
         _check(lib().tg_recall(self._h, nq, seed, k, ctypes.byref(r)))
+        return r.value
+
+    def dist_check(self, nq: int, seed: int = 0x5A11):
+        """Largest relative difference of the stored FP32 kNN distances of nq judged nodes
+        (those of recall() with the same seed) from their FP64 distances."""
+        r = ctypes.c_double()
+        _check(lib().tg_knn_dist_check(self._h, nq, seed, ctypes.byref(r)))
         return r.value
 
     def stats(self):

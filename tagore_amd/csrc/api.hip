@@ -1925,6 +1925,19 @@ int tg_search_recall(tg_state* s, uint32_t nq, const float* queries, uint32_t k,
   return rc;
 }
 
+// judged nodes: the first nq entries of a SplitMix64 Fisher-Yates shuffle of 0..N-1
+static std::vector<uint32_t> judged_nodes(uint32_t N, uint32_t nq, uint64_t seed) {
+  std::vector<uint32_t> perm(N);
+  for (uint32_t i = 0; i < N; ++i) perm[i] = i;
+  uint64_t x = seed;
+  for (uint32_t i = 0; i < nq; ++i) {
+    const uint64_t j = i + splitmix64(x) % uint64_t(N - i);
+    std::swap(perm[i], perm[j]);
+  }
+  perm.resize(nq);
+  return perm;
+}
+
 static int recall_impl(tg_state* s, uint32_t nq, uint64_t seed, uint32_t k, double* recall) {
   if (s && s->nseg) return fail(TG_ERR_STATE, "recall is judged on a handle without segments");
   if (!s || !recall) return fail(TG_ERR_ARG, "null argument");
@@ -1932,14 +1945,7 @@ static int recall_impl(tg_state* s, uint32_t nq, uint64_t seed, uint32_t k, doub
   if (k < 1 || k > 16 || k > s->K) return fail(TG_ERR_ARG, "k must be in [1, min(16, K)]");
   if (nq < 1 || nq > s->N) return fail(TG_ERR_ARG, "nq must be in [1, N]");
   HIPX(hipSetDevice(s->device));
-  // queries: first nq entries of a SplitMix64 Fisher-Yates shuffle of 0..N-1
-  std::vector<uint32_t> perm(s->N);
-  for (uint32_t i = 0; i < s->N; ++i) perm[i] = i;
-  uint64_t x = seed;
-  for (uint32_t i = 0; i < nq; ++i) {
-    const uint64_t j = i + splitmix64(x) % uint64_t(s->N - i);
-    std::swap(perm[i], perm[j]);
-  }
+  const std::vector<uint32_t> perm = judged_nodes(s->N, nq, seed);
   const uint32_t cap = 256;
   uint32_t *q = nullptr, *cand_n = nullptr, *hits = nullptr;
   double *kth = nullptr, *cand_d = nullptr;
@@ -1974,6 +1980,38 @@ static int recall_impl(tg_state* s, uint32_t nq, uint64_t seed, uint32_t k, doub
 }
 int tg_recall(tg_state* s, uint32_t nq, uint64_t seed, uint32_t k, double* recall) {
   return guarded([&] { return recall_impl(s, nq, seed, k, recall); });
+}
+
+static int dist_check_impl(tg_state* s, uint32_t nq, uint64_t seed, double* max_rel_err) {
+  if (s && s->nseg) return fail(TG_ERR_STATE, "distances are judged on a handle without segments");
+  if (!s || !max_rel_err) return fail(TG_ERR_ARG, "null argument");
+  if (!s->has_knn) return fail(TG_ERR_STATE, "no kNN graph");
+  if (nq < 1 || nq > s->N) return fail(TG_ERR_ARG, "nq must be in [1, N]");
+  HIPX(hipSetDevice(s->device));
+  const std::vector<uint32_t> nodes = judged_nodes(s->N, nq, seed);
+  uint32_t* q = nullptr;
+  unsigned long long* mx = nullptr;
+  int rc = TG_OK;
+  if (dalloc(&q, nq) || dalloc(&mx, 1)) {
+    rc = TG_ERR_HIP;
+  } else {
+    hipError_t e = hipMemcpy(q, nodes.data(), size_t(nq) * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      launch_dist_check(s->X, s->N, s->D, s->G, s->GD, s->K, q, nq, mx, s->stream);
+      e = hipStreamSynchronize(s->stream);
+    }
+    if (e == hipSuccess) e = hipGetLastError();
+    unsigned long long bits = 0;
+    if (e == hipSuccess) e = hipMemcpy(&bits, mx, 8, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = fail(TG_ERR_HIP, std::string("distance check: ") + hipGetErrorString(e));
+    else memcpy(max_rel_err, &bits, 8);
+  }
+  dfree(q);
+  dfree(mx);
+  return rc;
+}
+int tg_knn_dist_check(tg_state* s, uint32_t nq, uint64_t seed, double* max_rel_err) {
+  return guarded([&] { return dist_check_impl(s, nq, seed, max_rel_err); });
 }
 
 int tg_buffer(tg_state* s, int32_t idx, void** dev_ptr, size_t* bytes) {

@@ -193,5 +193,9 @@ void launch_recall(const float* X, uint32_t N, uint32_t D, const uint32_t* G, ui
                    const uint32_t* q, uint32_t nq, uint32_t k, double* kth, uint32_t* cand_n,
                    double* cand_d, uint32_t cand_cap, uint32_t* hits, hipStream_t s,
                    const float* Q = nullptr);
+// largest relative difference of the stored FP32 kNN distances of rows q[0..nq) from FP64
+void launch_dist_check(const float* X, uint32_t N, uint32_t D, const uint32_t* G,
+                       const float* GD, uint32_t K, const uint32_t* q, uint32_t nq,
+                       unsigned long long* max_bits, hipStream_t s);
 
 }  // namespace tg

@@ -406,6 +406,34 @@ void launch_recall(const float* X, uint32_t N, uint32_t D, const uint32_t* G, ui
   k_recall_full<<<nq, 256, 0, s>>>(X, N, D, q, nq, k, gd64, kth, hits, Q);
 }
 
+// Stored FP32 kNN distances against FP64: for each sampled node u = q[i] (one block), every
+// valid entry v of its row gives |GD[u][j] - d64(u, v)| / d64(u, v) (0 when both are 0, +inf
+// when only the FP64 distance is 0); the largest goes to *max_bits as the bits of a
+// non-negative double (their order is the values' order).
+__global__ __launch_bounds__(128) void k_dist_check(const float* X, uint32_t N, uint32_t D,
+                                                    const uint32_t* G, const float* GD,
+                                                    uint32_t K, const uint32_t* q,
+                                                    unsigned long long* max_bits) {
+  const uint32_t u = q[blockIdx.x];
+  for (uint32_t j = threadIdx.x; j < K; j += blockDim.x) {
+    const uint32_t v = G[size_t(u) * K + j];
+    if (v >= N) continue;
+    const double d = d64_rows(X + size_t(u) * D, X + size_t(v) * D, D);
+    const double st = double(GD[size_t(u) * K + j]);
+    double rel = 0.0;
+    if (d > 0.0) rel = fabs(st - d) / d;
+    else if (st != 0.0) rel = __longlong_as_double(0x7FF0000000000000ll);
+    atomicMax(max_bits, (unsigned long long)__double_as_longlong(rel));
+  }
+}
+
+void launch_dist_check(const float* X, uint32_t N, uint32_t D, const uint32_t* G,
+                       const float* GD, uint32_t K, const uint32_t* q, uint32_t nq,
+                       unsigned long long* max_bits, hipStream_t s) {
+  fill_u32(reinterpret_cast<uint32_t*>(max_bits), 0u, 2, s);
+  k_dist_check<<<nq, 128, 0, s>>>(X, N, D, G, GD, K, q, max_bits);
+}
+
 // ------------------------------------------------------- register sort self-test
 // One wave per group of 64*NPL keys.  mode 0: reg_bitonic; mode 1 (NPL 1): the 64 keys form
 // a bitonic sequence, reg_bitonic_merge64 sorts it; mode 2 (NPL 2): slots 0 and 1 hold two

@@ -172,6 +172,8 @@ def test_unit_k64_nssg():
         G0, GD0 = s.get_knn()
         s.knn_build(K, 4, seed, "bf16", init="learned")
         GL, GDL = s.get_knn()
+        # stored FP32 distances of 2000 judged rows within 1e-5 relative of FP64
+        assert s.dist_check(2000, 0x5A11) <= 1e-5
     GLr, GDLr, _ = oracle.gnn_descent(X, K, 4, seed, init=(G0, GD0))
     _knn_equal(GL, GDL, GLr, GDLr, "learned 4")
     for rows in ("fp32", "bf16"):
@@ -191,7 +193,10 @@ def test_d960_k64(prec):
         s.knn_build(K, it, seed, prec)
         G, GD = s.get_knn()
         st = s.stats()
-    _say(f"  re-scored {st['knn_rescored']} of {st['knn_evals']}")
+        derr = s.dist_check(N, 0x5A11)
+    _say(f"  re-scored {st['knn_rescored']} of {st['knn_evals']}, stored distances within "
+         f"{derr:.1e} of FP64")
+    assert derr <= 1e-5
     _knn_equal(G, GD, Gr, GDr, prec)
     assert st["knn_evals"] == ev
     _compare(X, Gr, GDr, 3, 0, 2, 32, 64, 0.0, prec)  # DPG
@@ -259,8 +264,10 @@ def test_headline_full_size_properties():
         s.gen_data(0, 1024, seed, 20.0)
         s.knn_build(K, 2, seed, "bf16", init="learned")
         rec = s.recall(10000, 10, 0x5A11)
-        _say(f"1M headline: recall@10 {rec:.4f}")
+        derr = s.dist_check(10000, 0x5A11)
+        _say(f"1M headline: recall@10 {rec:.4f}, stored distances within {derr:.1e} of FP64")
         assert rec >= 0.95
+        assert derr == 0.0  # integer rows: the FP32 distances are exact
         X = s.get_data()
         G, GD = s.get_knn()
         s.prune("path", "dist", "dist", FD, TOPM, 1.0)
