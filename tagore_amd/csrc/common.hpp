@@ -871,9 +871,10 @@ __device__ __forceinline__ void eval_batch2_u8(const uint8_t* X, int D, const Ro
 }
 
 // Evaluates the targets id_of(0..n) against the register row `tgt`, emit(q, d) once per item.
-//  * u8 rows (quad layout): one candidate per quad, 32 per step (items q0 + lane/4 and
-//    q0 + 16 + lane/4) while more than 16 remain, then 16; the results are moved to lanes
-//    0..31 (0..15), lane j holding item q0 + j, and emit runs there;
+//  * u8 rows (quad layout): one candidate per quad, 64 per step (items q0 + 16h + lane/4,
+//    h < 4: four rows in flight per quad) while more than 48 remain, then 32 while more than
+//    16 remain, then 16; the results are moved to lanes 0..63 (0..31, 0..15), lane j holding
+//    item q0 + j, and emit runs there;
 //  * FP32 / BF16 rows: four candidates per 16-lane group at a time (item q0 + g + 4b in group
 //    g); the 16 results are moved to lanes 0..15 and emit runs there, once per item.
 // Every emit writes only what its own item owns, so the lanes may run it together.
@@ -884,6 +885,21 @@ __device__ __forceinline__ void eval_batch(const T* X, int D, const Row16<MC>& t
   if constexpr (std::is_same<T, uint8_t>::value) {
     const int c = lane >> 2, src = (lane & 15) * 4;
     int q0 = 0;
+    for (; q0 + 48 < n; q0 += 64) {
+      // 64 per step: four rows in flight per quad
+      const int qb = q0 + 16 + c, qc = q0 + 32 + c, qd = q0 + 48 + c;
+      uint4 ya[MC], yb[MC], yc[MC], yd[MC];
+      load_quad_u8<MC>(ya, X + size_t(id_of(q0 + c)) * D, D);  // qc < q0 + 48 < n
+      load_quad_u8<MC>(yb, X + size_t(id_of(qb)) * D, D);
+      load_quad_u8<MC>(yc, X + size_t(id_of(qc)) * D, D);
+      load_quad_u8<MC>(yd, qd < n ? X + size_t(id_of(qd)) * D : X, D);
+      const float da = quad_dist_u8<MC>(tgt, ya, D), db = quad_dist_u8<MC>(tgt, yb, D);
+      const float dc = quad_dist_u8<MC>(tgt, yc, D), dd = quad_dist_u8<MC>(tgt, yd, D);
+      const float xa = __shfl(da, src, TG_WAVE), xb = __shfl(db, src, TG_WAVE);
+      const float xc = __shfl(dc, src, TG_WAVE), xd = __shfl(dd, src, TG_WAVE);
+      const float v = lane < 16 ? xa : lane < 32 ? xb : lane < 48 ? xc : xd;
+      if (q0 + lane < n) emit(q0 + lane, v);
+    }
     for (; q0 + 16 < n; q0 += 32) {
       const int qb = q0 + 16 + c;
       uint4 ya[MC], yb[MC];

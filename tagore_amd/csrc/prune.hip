@@ -510,7 +510,8 @@ __host__ __device__ inline FwdLayout fwd_layout(int K, int TOPM, int FD) {
 }
 
 template <int MC, class T, int KIND, int W>
-__global__ __launch_bounds__(64) void k_forward(PruneArgs a) {
+// Six waves per SIMD: 80 VGPRs at most (the 64-candidate distance steps need 86 otherwise).
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_forward(PruneArgs a) {
   extern __shared__ __attribute__((aligned(16))) unsigned char dsm[];
   const uint32_t u = row_of_block(a.perm, a.N);
   if (u >= a.N) return;
