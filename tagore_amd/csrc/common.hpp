@@ -871,21 +871,21 @@ __device__ __forceinline__ void eval_batch2_u8(const uint8_t* X, int D, const Ro
 }
 
 // Evaluates the targets id_of(0..n) against the register row `tgt`, emit(q, d) once per item.
-//  * u8 rows (quad layout): one candidate per quad, 64 per step (items q0 + 16h + lane/4,
-//    h < 4: four rows in flight per quad) while more than 48 remain, then 32 while more than
-//    16 remain, then 16; the results are moved to lanes 0..63 (0..31, 0..15), lane j holding
-//    item q0 + j, and emit runs there;
+//  * u8 rows (quad layout): one candidate per quad; with WIDE, 64 per step (items
+//    q0 + 16h + lane/4, h < 4: four rows in flight per quad, 16 more VGPRs) while more than
+//    48 remain; then 32 per step while more than 16 remain, then 16; the results are moved
+//    to lanes 0..63 (0..31, 0..15), lane j holding item q0 + j, and emit runs there;
 //  * FP32 / BF16 rows: four candidates per 16-lane group at a time (item q0 + g + 4b in group
 //    g); the 16 results are moved to lanes 0..15 and emit runs there, once per item.
 // Every emit writes only what its own item owns, so the lanes may run it together.
-template <int MC, class T, class IdOf, class Emit>
+template <int MC, class T, bool WIDE = false, class IdOf, class Emit>
 __device__ __forceinline__ void eval_batch(const T* X, int D, const Row16<MC>& tgt, int n,
                                            IdOf id_of, Emit emit) {
   const int lane = threadIdx.x;
   if constexpr (std::is_same<T, uint8_t>::value) {
     const int c = lane >> 2, src = (lane & 15) * 4;
     int q0 = 0;
-    for (; q0 + 48 < n; q0 += 64) {
+    for (; WIDE && q0 + 48 < n; q0 += 64) {
       // 64 per step: four rows in flight per quad
       const int qb = q0 + 16 + c, qc = q0 + 32 + c, qd = q0 + 48 + c;
       uint4 ya[MC], yb[MC], yc[MC], yd[MC];

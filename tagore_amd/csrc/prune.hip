@@ -93,7 +93,7 @@ __device__ __forceinline__ void beam_search(const PruneArgs& a, const Row16<MC>&
   for (int j = lane; j < NS; j += TG_WAVE)
     S[j] = j < K ? (INFKEY | a.G[size_t(start) * K + j]) : TG_KEY_INF;
   __syncthreads();
-  eval_batch<MC>(X, D, tgt, K, [&](int q) { return key_id(S[q]); },
+  eval_batch<MC, T, true>(X, D, tgt, K, [&](int q) { return key_id(S[q]); },
                  [&](int q, float d) {
                    const uint64_t k = mk_key(d, key_id(S[q]));
                    S[q] = k;
@@ -162,7 +162,7 @@ __device__ __forceinline__ void beam_search(const PruneArgs& a, const Row16<MC>&
     const uint32_t c0 = ring_c;
     const long long e0 = pclock(a);
     ph.sub[0] += e0 - s0;
-    eval_batch<MC>(X, D, tgt, cnt, [&](int q) { return key_id(S[cl[q]]); },
+    eval_batch<MC, T, true>(X, D, tgt, cnt, [&](int q) { return key_id(S[cl[q]]); },
                    [&](int q, float d) {
                      const int j = cl[q];
                      const uint64_t k = mk_key(d, key_id(S[j]));
@@ -482,9 +482,10 @@ __device__ __forceinline__ void count_add(unsigned long long* c, int idx, uint32
 // Forward pass (Collect + Filter + Store reverse pushes).  kind 0: path, 1: two-hop.
 //
 // LDS (dynamic, sized from K, TOPM and FD so that more workgroups fit a CU):
-//   A : the 480-slot ring (padded to 512 for its sort), then the pool (K + 480 keys);
-//   B : during the beam S | tab | expl | beam compaction indices; afterwards the selected
-//       list and the filter's compaction indices (K + 480).
+//   during the beam: the 480-slot ring (padded to 512 for its sort), then S | tab | expl |
+//       beam compaction indices;
+//   afterwards: the pool (K + 480 keys, over the ring and the dead start of the beam's
+//       scratch), then the selected list and the filter's compaction indices (K + 480).
 // The pool is the merge of the sorted own list and the sorted ring, formed in registers as
 // one bitonic merge of [own list ascending | +inf | ring descending] (1024 slots); it is
 // the sequence a sort of their union gives.
@@ -492,7 +493,7 @@ __device__ __forceinline__ void count_add(unsigned long long* c, int idx, uint32
 constexpr int RING_P = 512;
 
 struct FwdLayout {
-  int W, NS, L, HB, offB, bytes;
+  int W, NS, L, HB, offB, offF, bytes;
 };
 __host__ __device__ inline FwdLayout fwd_layout(int K, int TOPM, int FD) {
   FwdLayout l;
@@ -502,10 +503,11 @@ __host__ __device__ inline FwdLayout fwd_layout(int K, int TOPM, int FD) {
   // the beam's id set holds at most TOPM + K ids: 256 slots up to 192 of them (three
   // quarters full at most), else 512
   l.HB = TOPM + K <= 192 ? 8 : 9;
-  l.offB = ((l.L > RING_P ? l.L : RING_P) * 8 + 15) / 16 * 16;
-  const int beam = l.NS * 8 + (4 << l.HB) + EXPL_MAX * 4 + l.NS * 2;
-  const int filt = (FD + 1) * 8 + l.L * 2;
-  l.bytes = l.offB + ((beam > filt ? beam : filt) + 15) / 16 * 16;
+  l.offB = RING_P * 8;
+  l.offF = ((l.L > RING_P ? l.L : RING_P) * 8 + 15) / 16 * 16;
+  const int beam = l.offB + l.NS * 8 + (4 << l.HB) + EXPL_MAX * 4 + l.NS * 2;
+  const int filt = l.offF + (FD + 1) * 8 + l.L * 2;
+  l.bytes = ((beam > filt ? beam : filt) + 15) / 16 * 16;
   return l;
 }
 
@@ -524,7 +526,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
   uint32_t* tab = reinterpret_cast<uint32_t*>(S + ly.NS);
   uint32_t* expl = tab + (1 << ly.HB);
   uint16_t* clb = reinterpret_cast<uint16_t*>(expl + EXPL_MAX);
-  uint64_t* sel = S;  // after the beam
+  uint64_t* sel = reinterpret_cast<uint64_t*>(dsm + ly.offF);  // after the beam
   uint16_t* cl = reinterpret_cast<uint16_t*>(sel + a.FD + 1);
   const int L = ly.L;
   Row16<MC> xu;
