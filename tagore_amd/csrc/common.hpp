@@ -153,21 +153,37 @@ __device__ __forceinline__ int quad_sum(int q) {
   return q + __builtin_amdgcn_mov_dpp(q, 0x4E, 0xF, 0xF, false);  // lane ^ 2
 }
 
+// u8 rows, quad layout: v[m] holds the lane's 16 bytes of chunk m as bits.  The loads are
+// issued by row16_u8_load and |x|^2 (n2, in every lane) formed by row16_u8_norm, so that
+// other work can run while the loads are in flight.
+template <int MC>
+__device__ __forceinline__ void row16_u8_load(Row16<MC>& r, const uint8_t* __restrict__ x, int D) {
+  const int j = threadIdx.x & 3;
+#pragma unroll
+  for (int m = 0; m < MC; ++m) {
+    const uint4 w = m * 64 < D ? load_u8x16(x, m, D, j) : make_uint4(0u, 0u, 0u, 0u);
+    r.v[m] = make_float4(__uint_as_float(w.x), __uint_as_float(w.y), __uint_as_float(w.z),
+                         __uint_as_float(w.w));
+  }
+}
+template <int MC>
+__device__ __forceinline__ void row16_u8_norm(Row16<MC>& r) {
+  uint32_t sq = 0;
+#pragma unroll
+  for (int m = 0; m < MC; ++m) {
+    const uint4 w = make_uint4(__float_as_uint(r.v[m].x), __float_as_uint(r.v[m].y),
+                               __float_as_uint(r.v[m].z), __float_as_uint(r.v[m].w));
+    sq = dot16_u8(w, w, sq);
+  }
+  r.n2 = quad_sum(int(sq));
+}
+
 template <int MC, class T>
 __device__ __forceinline__ void load_row16(Row16<MC>& r, const T* __restrict__ x, int D, int t) {
   if constexpr (std::is_same<T, uint8_t>::value) {
-    // u8 rows, quad layout: v[m] holds the lane's 16 bytes of chunk m as bits; n2 = |x|^2
-    // in every lane (t is not used: the layout follows lane & 3)
-    const int j = threadIdx.x & 3;
-    uint32_t sq = 0;
-#pragma unroll
-    for (int m = 0; m < MC; ++m) {
-      const uint4 w = m * 64 < D ? load_u8x16(x, m, D, j) : make_uint4(0u, 0u, 0u, 0u);
-      r.v[m] = make_float4(__uint_as_float(w.x), __uint_as_float(w.y), __uint_as_float(w.z),
-                           __uint_as_float(w.w));
-      sq = dot16_u8(w, w, sq);
-    }
-    r.n2 = quad_sum(int(sq));
+    // u8 rows, quad layout (t is not used: the layout follows lane & 3)
+    row16_u8_load<MC>(r, x, D);
+    row16_u8_norm<MC>(r);
   } else {
 #pragma unroll
     for (int m = 0; m < MC; ++m) {

@@ -373,8 +373,20 @@ __device__ __forceinline__ int filter_pool(const PruneArgs& a, uint64_t* P, int 
   int from = 0;  // cl[from..from+nl) is the live list
   __syncthreads();
   while (nsel < FD) {
-    Row16<MC> prow;
-    load_row16<MC>(prow, reinterpret_cast<const T*>(a.rows) + size_t(pid) * D, D, t);
+    Row16<MC> prow, qrow;
+    int gi = -1;  // u8 rows: pool index whose row qrow holds (loads in flight)
+    if constexpr (std::is_same<T, uint8_t>::value) {
+      // p's row, and the row of the live list's first entry -- the first candidate c1 below
+      // unless it repeats p's id -- are loaded while the list is compacted
+      const uint8_t* X8 = reinterpret_cast<const uint8_t*>(a.rows);
+      row16_u8_load<MC>(prow, X8 + size_t(pid) * D, D);
+      if (nl > 1 && nsel + 1 < FD) {
+        gi = cl[from];
+        row16_u8_load<MC>(qrow, X8 + size_t(key_id(P[gi])) * D, D);
+      }
+    } else {
+      load_row16<MC>(prow, reinterpret_cast<const T*>(a.rows) + size_t(pid) * D, D, t);
+    }
     // candidates: the live entries; those repeating p's id are invalidated
     const int cnt = compact_cl(cl, from, nl, [&](int i) {
       const uint32_t id = key_id(P[i]);
@@ -383,6 +395,7 @@ __device__ __forceinline__ int filter_pool(const PruneArgs& a, uint64_t* P, int 
     });
     const long long e0 = pclock(a);
     if constexpr (std::is_same<T, uint8_t>::value) {
+      row16_u8_norm<MC>(prow);
       // Two rounds in one pass: the first live candidate c1 is the next selection exactly
       // when this round keeps it, which d(p, c1) -- two rows in registers -- decides.  Then
       // each candidate row is read once for d(p, c) and d(c1, c), and every candidate
@@ -394,8 +407,9 @@ __device__ __forceinline__ int filter_pool(const PruneArgs& a, uint64_t* P, int 
         const uint64_t k1 = P[i1];
         const uint32_t id1 = key_id(k1);
         const float cur1 = key_d(k1);
-        Row16<MC> qrow;
-        load_row16<MC>(qrow, reinterpret_cast<const T*>(a.rows) + size_t(id1) * D, D, t);
+        if (i1 != gi)
+          row16_u8_load<MC>(qrow, reinterpret_cast<const uint8_t*>(a.rows) + size_t(id1) * D, D);
+        row16_u8_norm<MC>(qrow);
         const float d1 = dist_rows_u8<MC>(prow, qrow, D);
         if (!(rule_eval(rule, d1, cur1, cur, a.th) < 0.0f) && i1 + 1 < L) {
           int n2 = 0;
