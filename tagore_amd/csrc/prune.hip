@@ -31,7 +31,6 @@ struct PhaseAcc {
   long long sub[4] = {0, 0, 0, 0};
 };
 constexpr int POOL_MAX = 1024;  // K + 480 <= 608, padded to a power of two
-constexpr int EXPL_MAX = 32;
 
 // -------------------------------------------------------------------------------------
 // Id set in LDS: open addressing over 2^bits slots, INV marks an empty slot.  Callers size
@@ -74,14 +73,15 @@ __device__ __forceinline__ bool hset_has(const uint32_t* tab, uint32_t id, int b
 // one bitonic merge: min(beam[p], cand[64W-1-p]) is bitonic and holds the 64W smallest of
 // both, so sorting it gives the first 64W keys of merge(beam, candidates).
 // tab: id set of the previous round's beam and candidates (the dedup set).  An entry is
-// explored when its id is in `expl`: the beam never holds two finite entries with one id,
-// and an expanded node that falls out of the beam can never re-enter it (the beam's TOPM-th
-// key only decreases).  `out` (optional) receives the final beam (TOPM keys).
+// explored when its id is among the expanded ids (held in a register, one per lane): the
+// beam never holds two finite entries with one id, and an expanded node that falls out of
+// the beam can never re-enter it (the beam's TOPM-th key only decreases).  `out` (optional)
+// receives the final beam (TOPM keys).
 // -------------------------------------------------------------------------------------
 template <int MC, class T, int W>
 __device__ __forceinline__ void beam_search(const PruneArgs& a, const Row16<MC>& tgt,
                                             uint32_t start,
-                            int max_exp, uint64_t* S, uint32_t* tab, int hbits, uint32_t* expl,
+                            int max_exp, uint64_t* S, uint32_t* tab, int hbits,
                             uint16_t* cl, uint64_t* ring, uint64_t* out, uint32_t& ring_c,
                             uint32_t& n_eval, uint32_t& n_adj, PhaseAcc& ph) {
   const int lane = threadIdx.x;
@@ -114,31 +114,31 @@ __device__ __forceinline__ void beam_search(const PruneArgs& a, const Row16<MC>&
 #pragma unroll
   for (int b = 0; b < W; ++b) bk[b] = b * 64 + lane < TOPM ? v[b] : TG_KEY_INF;
   __syncthreads();
+  // the expanded ids in registers: lane e < nexpl holds the e-th (max_exp <= 64)
+  uint32_t ex = TG_INV;
   int nexpl = 0;
   for (int r = 0; r < max_exp; ++r) {
     const long long s0 = pclock(a);
     // first explorable beam entry (position order)
     int first = -1;
-    uint64_t fk = 0;
+    uint32_t x = 0;
 #pragma unroll
     for (int b = 0; b < W; ++b) {
       if (first >= 0) break;
+      const uint32_t id = key_id(bk[b]);
       bool ok = key_finite(bk[b]);
-      if (ok) {
-        const uint32_t id = key_id(bk[b]);
-        for (int e = 0; e < nexpl; ++e) ok = ok && (expl[e] != id);
-      }
+      for (int e = 0; e < nexpl; ++e)
+        ok = ok & (id != uint32_t(__builtin_amdgcn_readlane(int(ex), e)));
       const uint64_t m = ballot(ok);
       if (m) {
         const int fl = __ffsll((unsigned long long)m) - 1;
         first = b * 64 + fl;
-        fk = uint64_t(__shfl(int(key_id(bk[b])), fl, TG_WAVE));
+        x = uint32_t(__builtin_amdgcn_readlane(int(id), fl));
       }
     }
     if (first < 0) break;
-    const uint32_t x = uint32_t(fk);
     __syncthreads();
-    if (lane == 0) expl[nexpl] = x;
+    if (lane == nexpl) ex = x;
     ++nexpl;
     n_adj += 1;
     // new candidates: duplicates of the dedup set get +inf; the rest compacted in j order
@@ -496,8 +496,8 @@ __device__ __forceinline__ void count_add(unsigned long long* c, int idx, uint32
 // Forward pass (Collect + Filter + Store reverse pushes).  kind 0: path, 1: two-hop.
 //
 // LDS (dynamic, sized from K, TOPM and FD so that more workgroups fit a CU):
-//   during the beam: the 480-slot ring (padded to 512 for its sort), then S | tab | expl |
-//       beam compaction indices;
+//   during the beam: the 480-slot ring (padded to 512 for its sort), then S | tab | beam
+//       compaction indices;
 //   afterwards: the pool (K + 480 keys, over the ring and the dead start of the beam's
 //       scratch), then the selected list and the filter's compaction indices (K + 480).
 // The pool is the merge of the sorted own list and the sorted ring, formed in registers as
@@ -519,7 +519,7 @@ __host__ __device__ inline FwdLayout fwd_layout(int K, int TOPM, int FD) {
   l.HB = TOPM + K <= 192 ? 8 : 9;
   l.offB = RING_P * 8;
   l.offF = ((l.L > RING_P ? l.L : RING_P) * 8 + 15) / 16 * 16;
-  const int beam = l.offB + l.NS * 8 + (4 << l.HB) + EXPL_MAX * 4 + l.NS * 2;
+  const int beam = l.offB + l.NS * 8 + (4 << l.HB) + l.NS * 2;
   const int filt = l.offF + (FD + 1) * 8 + l.L * 2;
   l.bytes = ((beam > filt ? beam : filt) + 15) / 16 * 16;
   return l;
@@ -538,8 +538,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
   uint64_t* P = R;  // after the pool is formed
   uint64_t* S = reinterpret_cast<uint64_t*>(dsm + ly.offB);
   uint32_t* tab = reinterpret_cast<uint32_t*>(S + ly.NS);
-  uint32_t* expl = tab + (1 << ly.HB);
-  uint16_t* clb = reinterpret_cast<uint16_t*>(expl + EXPL_MAX);
+  uint16_t* clb = reinterpret_cast<uint16_t*>(tab + (1 << ly.HB));
   uint64_t* sel = reinterpret_cast<uint64_t*>(dsm + ly.offF);  // after the beam
   uint16_t* cl = reinterpret_cast<uint16_t*>(sel + a.FD + 1);
   const int L = ly.L;
@@ -554,7 +553,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
   if (KIND == 0) {
     uint32_t ring_c = 0;
     const uint32_t ep = a.ep[a.seg_of ? a.seg_of[u] : 0];
-    beam_search<MC, T, W>(a, xu, ep, 10, S, tab, ly.HB, expl, clb, R, nullptr, ring_c, n_col,
+    beam_search<MC, T, W>(a, xu, ep, 10, S, tab, ly.HB, clb, R, nullptr, ring_c, n_col,
                           n_adj, ph);
   } else {
     // two-hop ids y_i = G[G[u][i/K]][i%K], i < 480
@@ -825,7 +824,6 @@ template <int MC, class T>
 __global__ __launch_bounds__(64) void k_cal_ep(PruneArgs a, const float* centroid, uint32_t* ep) {
   __shared__ uint64_t M0[128], S[128];
   __shared__ uint32_t tab[HSET];
-  __shared__ uint32_t expl[EXPL_MAX];
   __shared__ uint16_t cl[128];
   const uint32_t seg = blockIdx.x;
   const uint32_t start = a.seg_lo ? a.seg_lo[seg] : 0u;
@@ -834,9 +832,9 @@ __global__ __launch_bounds__(64) void k_cal_ep(PruneArgs a, const float* centroi
   uint32_t rc = 0, ne = 0, na = 0;
   PhaseAcc ph;  // not reported for the entry point
   if (a.K <= 64 && a.TOPM <= 64)
-    beam_search<MC, T, 1>(a, c, start, 30, S, tab, 9, expl, cl, nullptr, M0, rc, ne, na, ph);
+    beam_search<MC, T, 1>(a, c, start, 30, S, tab, 9, cl, nullptr, M0, rc, ne, na, ph);
   else
-    beam_search<MC, T, 2>(a, c, start, 30, S, tab, 9, expl, cl, nullptr, M0, rc, ne, na, ph);
+    beam_search<MC, T, 2>(a, c, start, 30, S, tab, 9, cl, nullptr, M0, rc, ne, na, ph);
   if (threadIdx.x == 0) ep[seg] = key_id(M0[0]);
   count_add(a.counters, 4, ne);
 }
