@@ -281,6 +281,38 @@ def test_recall_judge_matches_oracle():
     assert 0.5 < r_gpu <= 1.0
 
 
+def test_distance_check_matches_numpy():
+    """tg_knn_dist_check: the largest |d32 - d64| / d64 over the kNN rows of the judged nodes
+    (tg_recall's selection), against the same figure from numpy FP64; and its rejections."""
+    X = gmm_unit(3000, 96, 16, 9)
+    with State(*X.shape) as s:
+        s.set_data(X)
+        with pytest.raises(RuntimeError, match="no kNN graph"):
+            s.dist_check(100)
+        s.knn_build(32, 4, 77, "bf16")
+        G, GD = s.get_knn()
+        e_gpu = s.dist_check(500, 0x5A11)
+        with pytest.raises(RuntimeError, match="nq must be"):
+            s.dist_check(0)
+    perm = np.arange(X.shape[0])
+    M = (1 << 64) - 1
+    x = 0x5A11
+    for i in range(500):
+        x = (x + 0x9E3779B97F4A7C15) & M
+        z = x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        z ^= z >> 31
+        j = i + z % (X.shape[0] - i)
+        perm[i], perm[j] = perm[j], perm[i]
+    q = perm[:500]
+    X64 = X.astype(np.float64)
+    d64 = ((X64[q][:, None, :] - X64[G[q].astype(np.int64)]) ** 2).sum(-1)
+    e_ref = float((np.abs(GD[q].astype(np.float64) - d64) / d64).max())
+    assert abs(e_gpu - e_ref) <= 1e-12
+    assert 0.0 < e_gpu <= 1e-5
+
+
 def test_generated_data_properties():
     with State(20000, 128) as s:
         s.gen_data(0, 64, 42, 20.0)
