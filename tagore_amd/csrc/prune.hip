@@ -322,21 +322,40 @@ __device__ int next_valid(const uint64_t* P, int from, int L, uint32_t skip) {
 }
 
 // -------------------------------------------------------------------------------------
-// Keeps, in order, the entries cl[from..from+n) for which keep(index) holds, writing them to
-// cl[0..); returns how many.  Each 64-entry chunk is read before any of its kept entries is
-// written, and writes land at or below the read position, so the compaction is in place.
+// The next selection and the next round's candidate list, in one pass over the entries
+// cl[from..from+n) (pool indices, in pool order): the first entry not invalidated is the
+// selection (its pool index goes to pi, -1 when there is none); the entries after it that
+// are not invalidated are the candidates, compacted to cl[0..), except those repeating the
+// selection's id, which are invalidated uncounted (a round's start rule).  Returns the
+// candidates' number.  Each 64-entry chunk is read before any of its entries is written,
+// and writes land below the next chunk, so the compaction is in place.
 // -------------------------------------------------------------------------------------
-template <class Keep>
-__device__ __forceinline__ int compact_cl(uint16_t* cl, int from, int n, Keep keep) {
+__device__ __forceinline__ int advance_cl(uint64_t* P, uint16_t* cl, int from, int n, int& pi) {
   const int lane = threadIdx.x;
   int out = 0;
+  uint32_t sid = TG_INV;
+  pi = -1;
   for (int base = 0; base < n; base += TG_WAVE) {
     const int j = base + lane;
-    const uint16_t c = j < n ? cl[from + j] : uint16_t(0);
-    const bool k = j < n && keep(int(c));
+    const int c = j < n ? int(cl[from + j]) : 0;
+    const uint32_t id = j < n ? key_id(P[c]) : TG_INV;
+    bool k = id != TG_INV;
+    if (pi < 0) {
+      const uint64_t m = ballot(k);
+      if (m) {
+        const int f = __ffsll((unsigned long long)m) - 1;
+        pi = __builtin_amdgcn_readlane(c, f);
+        sid = uint32_t(__builtin_amdgcn_readlane(int(id), f));
+        k = k && lane > f;
+      }
+    }
+    if (k && id == sid) {
+      P[c] = key_with_id(P[c], TG_INV);
+      k = false;
+    }
     const uint64_t m = ballot(k);
     __syncthreads();
-    if (k) cl[out + mask_rank(m)] = c;
+    if (k) cl[out + mask_rank(m)] = uint16_t(c);
     out += __popcll(m);
     __syncthreads();
   }
@@ -345,9 +364,9 @@ __device__ __forceinline__ int compact_cl(uint16_t* cl, int from, int n, Keep ke
 
 // -------------------------------------------------------------------------------------
 // Incremental Filter over the sorted pool P[0..L) (spec A.4 in the oracle).  Writes the
-// kept entries to sel[] and returns their number.  cl holds the live list: the pool
-// indices after the current selection that are not invalidated, in pool order, so a
-// round touches only the survivors of the previous round.
+// kept entries to sel[] and returns their number.  cl holds the round's candidates: the pool
+// indices after the current selection p that are not invalidated and do not repeat p's id,
+// in pool order, so a round touches only the survivors of the previous round.
 // -------------------------------------------------------------------------------------
 template <int MC, class T>
 __device__ __forceinline__ int filter_pool(const PruneArgs& a, uint64_t* P, int L, uint32_t u,
@@ -362,59 +381,48 @@ __device__ __forceinline__ int filter_pool(const PruneArgs& a, uint64_t* P, int 
   float cur = key_d(P[tix]);
   uint32_t pid = key_id(P[tix]);
   ++tix;
+  // the first round's candidates; entries repeating p's id are invalidated uncounted
   int nl = 0;
   for (int base = tix; base < L; base += TG_WAVE) {
     const int i = base + lane;
-    const bool live = i < L && key_id(P[i]) != TG_INV;
+    const uint32_t id = i < L ? key_id(P[i]) : TG_INV;
+    bool live = id != TG_INV;
+    if (live && id == pid) {
+      P[i] = key_with_id(P[i], TG_INV);
+      live = false;
+    }
     const uint64_t m = ballot(live);
     if (live) cl[nl + mask_rank(m)] = uint16_t(i);
     nl += __popcll(m);
   }
-  int from = 0;  // cl[from..from+nl) is the live list
   __syncthreads();
   while (nsel < FD) {
-    Row16<MC> prow, qrow;
-    int gi = -1;  // u8 rows: pool index whose row qrow holds (loads in flight)
-    if constexpr (std::is_same<T, uint8_t>::value) {
-      // p's row, and the row of the live list's first entry -- the first candidate c1 below
-      // unless it repeats p's id -- are loaded while the list is compacted
-      const uint8_t* X8 = reinterpret_cast<const uint8_t*>(a.rows);
-      row16_u8_load<MC>(prow, X8 + size_t(pid) * D, D);
-      if (nl > 1 && nsel + 1 < FD) {
-        gi = cl[from];
-        row16_u8_load<MC>(qrow, X8 + size_t(key_id(P[gi])) * D, D);
-      }
-    } else {
-      load_row16<MC>(prow, reinterpret_cast<const T*>(a.rows) + size_t(pid) * D, D, t);
-    }
-    // candidates: the live entries; those repeating p's id are invalidated
-    const int cnt = compact_cl(cl, from, nl, [&](int i) {
-      const uint32_t id = key_id(P[i]);
-      if (id == pid) P[i] = key_with_id(P[i], TG_INV);
-      return id != TG_INV && id != pid;
-    });
+    const int cnt = nl;  // this round's candidates cl[0..cnt)
+    Row16<MC> prow;
     const long long e0 = pclock(a);
     if constexpr (std::is_same<T, uint8_t>::value) {
+      // Two rounds in one pass: the first candidate c1 is the next selection exactly when
+      // this round keeps it, which d(p, c1) -- two rows in registers -- decides.  Then each
+      // candidate row is read once for d(p, c) and d(c1, c), and every candidate meets the
+      // two rounds' rules in their order: one invalidated by p is not an evaluation of the
+      // second round, an entry repeating c1's id is dropped uncounted (as at a round's
+      // start), the others count for the second round.
+      const uint8_t* X8 = reinterpret_cast<const uint8_t*>(a.rows);
+      Row16<MC> qrow;
+      const bool two = cnt > 1 && nsel + 1 < FD;
+      const int i1 = two ? int(cl[0]) : 0;
+      row16_u8_load<MC>(prow, X8 + size_t(pid) * D, D);
+      if (two) row16_u8_load<MC>(qrow, X8 + size_t(key_id(P[i1])) * D, D);
       row16_u8_norm<MC>(prow);
-      // Two rounds in one pass: the first live candidate c1 is the next selection exactly
-      // when this round keeps it, which d(p, c1) -- two rows in registers -- decides.  Then
-      // each candidate row is read once for d(p, c) and d(c1, c), and every candidate
-      // meets the two rounds' rules in their order: one invalidated by p is not an
-      // evaluation of the second round, an entry repeating c1's id is dropped uncounted
-      // (as at a round's start), the others count for the second round.
-      if (cnt > 1 && nsel + 1 < FD) {
-        const int i1 = cl[0];
+      if (two) {
+        row16_u8_norm<MC>(qrow);
         const uint64_t k1 = P[i1];
         const uint32_t id1 = key_id(k1);
         const float cur1 = key_d(k1);
-        if (i1 != gi)
-          row16_u8_load<MC>(qrow, reinterpret_cast<const uint8_t*>(a.rows) + size_t(id1) * D, D);
-        row16_u8_norm<MC>(qrow);
         const float d1 = dist_rows_u8<MC>(prow, qrow, D);
         if (!(rule_eval(rule, d1, cur1, cur, a.th) < 0.0f) && i1 + 1 < L) {
           int n2 = 0;
-          eval_batch2_u8<MC>(reinterpret_cast<const uint8_t*>(a.rows), D, prow, qrow, cnt,
-                             [&](int q) { return key_id(P[cl[q]]); },
+          eval_batch2_u8<MC>(X8, D, prow, qrow, cnt, [&](int q) { return key_id(P[cl[q]]); },
                              [&](int q, float da, float db) {
                                if (q == 0) return;  // c1: kept by this round
                                const int i = cl[q];
@@ -437,15 +445,13 @@ __device__ __forceinline__ int filter_pool(const PruneArgs& a, uint64_t* P, int 
           // the first round's selection is c1; the second's is its first survivor
           if (lane == 0) sel[nsel] = k1;
           ++nsel;
-          nl = compact_cl(cl, 1, cnt - 1, [&](int i) { return key_id(P[i]) != TG_INV; });
-          if (nl > 0) {
-            const int pi = cl[0];
+          int pi;
+          nl = advance_cl(P, cl, 1, cnt - 1, pi);
+          if (pi >= 0) {
             if (lane == 0) sel[nsel] = P[pi];
             cur = key_d(P[pi]);
             pid = key_id(P[pi]);
             tix = pi;
-            from = 1;
-            --nl;
           } else {
             tix = L;
           }
@@ -456,6 +462,8 @@ __device__ __forceinline__ int filter_pool(const PruneArgs& a, uint64_t* P, int 
           continue;
         }
       }
+    } else {
+      load_row16<MC>(prow, reinterpret_cast<const T*>(a.rows) + size_t(pid) * D, D, t);
     }
     eval_batch<MC>(reinterpret_cast<const T*>(a.rows), D, prow, cnt, [&](int q) { return key_id(P[cl[q]]); },
                    [&](int q, float res) {
@@ -467,16 +475,15 @@ __device__ __forceinline__ int filter_pool(const PruneArgs& a, uint64_t* P, int 
     ph.ev += pclock(a) - e0;
     n_eval += cnt;
     __syncthreads();
-    // survivors in pool order; the first is the next selection
-    nl = compact_cl(cl, 0, cnt, [&](int i) { return key_id(P[i]) != TG_INV; });
-    if (nl > 0) {
-      const int pi = cl[0];
+    // the next selection is the first survivor; the survivors after it are the next round's
+    // candidates
+    int pi;
+    nl = advance_cl(P, cl, 0, cnt, pi);
+    if (pi >= 0) {
       if (lane == 0) sel[nsel] = P[pi];
       cur = key_d(P[pi]);
       pid = key_id(P[pi]);
       tix = pi;
-      from = 1;
-      --nl;
     } else {
       tix = L;
     }
