@@ -335,7 +335,9 @@ __global__ __launch_bounds__(64) void k_merge(KnnArgs a) {
 //   2. an exclusive scan of those counts, block-major, gives every (block, tile) pair its
 //      output range;
 //   3. k_rev_part: each tile scatters its edges -- key (d, u) and destination v -- into its
-//      ranges, the rank inside a range from an LDS cursor;
+//      ranges, REV_CH edges per pass: a pass is grouped by block in LDS first, so each
+//      block's edges leave as one contiguous run of stores (whole cache lines, not scattered
+//      8- and 4-byte writes);
 //   4. k_rev_runs: per block, LDS counts and a scan give every destination its run
 //      [off[v], off[v] + cnt[v]); LDS cursors place the keys.
 // The runs land in the reverse PushBuf as its grouped overflow with cap 0 (cnt, ov_off,
@@ -380,23 +382,85 @@ __global__ __launch_bounds__(REV_THREADS) void k_rev_hist(KnnArgs a, RevPlan r) 
   for (uint32_t b = threadIdx.x; b < r.P; b += REV_THREADS) r.H[size_t(b) * r.tiles + t] = h[b];
 }
 
-__global__ __launch_bounds__(REV_THREADS) void k_rev_part(KnnArgs a, RevPlan r) {
-  __shared__ uint32_t cur[REV_MAXP];
-  const uint32_t t = blockIdx.x;
-  for (uint32_t b = threadIdx.x; b < r.P; b += REV_THREADS) cur[b] = r.H[size_t(b) * r.tiles + t];
+constexpr int REV_CH = 4096;  // edges per pass of k_rev_part
+
+// Exclusive scan in place of x[0..n) (n <= REV_MAXP) by a REV_THREADS-thread workgroup, each
+// thread owning a run of consecutive entries; returns the total (in every thread).
+__device__ uint32_t rev_block_scan(uint32_t* x, uint32_t n, uint32_t* wsum) {
+  const uint32_t per = (n + REV_THREADS - 1) / REV_THREADS, i0 = threadIdx.x * per;
+  uint32_t own = 0;
+  for (uint32_t i = 0; i < per; ++i)
+    if (i0 + i < n) own += x[i0 + i];
+  const uint32_t incl = uint32_t(wave_incl_scan(int(own)));
+  const int w = threadIdx.x / TG_WAVE;
+  if (lane_id() == TG_WAVE - 1) wsum[w] = incl;
   __syncthreads();
+  uint32_t run = incl - own, total = 0;
+  for (int q = 0; q < REV_THREADS / TG_WAVE; ++q) {
+    if (q < w) run += wsum[q];
+    total += wsum[q];
+  }
+  for (uint32_t i = 0; i < per; ++i)
+    if (i0 + i < n) {
+      const uint32_t c = x[i0 + i];
+      x[i0 + i] = run;
+      run += c;
+    }
+  __syncthreads();
+  return total;
+}
+
+__global__ __launch_bounds__(REV_THREADS) void k_rev_part(KnnArgs a, RevPlan r) {
+  __shared__ uint32_t cur[REV_MAXP];  // this tile's next output position of every block
+  __shared__ uint32_t off[REV_MAXP];  // a pass's counts per block, then their offsets
+  __shared__ uint64_t skey[REV_CH];   // a pass's keys and destinations, grouped by block
+  __shared__ uint32_t sv[REV_CH];
+  __shared__ uint32_t wsum[REV_THREADS / TG_WAVE];
+  constexpr int EPT = REV_CH / REV_THREADS;
+  const uint32_t t = blockIdx.x, tid = threadIdx.x;
+  for (uint32_t b = tid; b < r.P; b += REV_THREADS) cur[b] = r.H[size_t(b) * r.tiles + t];
   uint64_t* pkey = a.pr.slots;
   uint32_t* pv = a.pr.ov_dest;
   const uint64_t e0 = uint64_t(t) * r.T, e1 = min(r.n, e0 + r.T);
-  for (uint64_t e = e0 + threadIdx.x; e < e1; e += REV_THREADS) {
-    const uint32_t v = a.G[e];
-    if (v < a.N) {
-      const uint32_t p = atomicAdd(&cur[v >> r.lgR], 1u);
-      if (p < r.n) {  // always, for the counts of k_rev_hist
-        pkey[p] = mk_key(a.GD[e], uint32_t(e / a.K));
-        pv[p] = v;
+  for (uint64_t c0 = e0; c0 < e1; c0 += REV_CH) {
+    for (uint32_t b = tid; b < r.P; b += REV_THREADS) off[b] = 0;
+    __syncthreads();
+    uint32_t v[EPT], rk[EPT];
+    uint64_t key[EPT];
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const uint64_t e = c0 + uint64_t(k) * REV_THREADS + tid;
+      v[k] = e < e1 ? a.G[e] : TG_INV;
+      key[k] = 0;
+      rk[k] = 0;
+      if (v[k] < a.N) {
+        key[k] = mk_key(a.GD[e], uint32_t(e / a.K));
+        rk[k] = atomicAdd(&off[v[k] >> r.lgR], 1u);
       }
     }
+    __syncthreads();
+    const uint32_t m = rev_block_scan(off, r.P, wsum);
+#pragma unroll
+    for (int k = 0; k < EPT; ++k)
+      if (v[k] < a.N) {
+        const uint32_t p = off[v[k] >> r.lgR] + rk[k];
+        skey[p] = key[k];
+        sv[p] = v[k];
+      }
+    __syncthreads();
+    // each block's run of the pass to consecutive positions from its cursor
+    for (uint32_t i = tid; i < m; i += REV_THREADS) {
+      const uint32_t vv = sv[i], b = vv >> r.lgR;
+      const uint32_t p = cur[b] + (i - off[b]);
+      if (p < r.n) {  // always, for the counts of k_rev_hist
+        pkey[p] = skey[i];
+        pv[p] = vv;
+      }
+    }
+    __syncthreads();
+    for (uint32_t b = tid; b < r.P; b += REV_THREADS)
+      cur[b] += (b + 1 < r.P ? off[b + 1] : m) - off[b];
+    __syncthreads();
   }
 }
 
