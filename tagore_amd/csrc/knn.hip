@@ -475,11 +475,20 @@ __global__ __launch_bounds__(REV_THREADS) void k_rev_runs(KnnArgs a, RevPlan r) 
   lo = min(lo, hi);
   const uint32_t* pv = a.pr.ov_dest;
   const uint64_t* pkey = a.pr.slots;
+  // RUN_U entries per thread and step: their loads are all issued before any is used
+  constexpr uint32_t RUN_U = 8;
   for (uint32_t i = threadIdx.x; i < R; i += REV_THREADS) c[i] = 0;
   __syncthreads();
-  for (uint32_t p = lo + threadIdx.x; p < hi; p += REV_THREADS) {
-    const uint32_t l = pv[p] - v0;
-    if (l < R) atomicAdd(&c[l], 1u);
+  for (uint64_t p0 = lo; p0 < hi; p0 += REV_THREADS * RUN_U) {
+    uint32_t l[RUN_U];
+#pragma unroll
+    for (uint32_t k = 0; k < RUN_U; ++k) {
+      const uint64_t p = p0 + k * REV_THREADS + threadIdx.x;
+      l[k] = p < hi ? pv[p] - v0 : R;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < RUN_U; ++k)
+      if (l[k] < R) atomicAdd(&c[l[k]], 1u);
   }
   __syncthreads();
   // exclusive scan of c[0..R): thread i owns c[i*per .. (i+1)*per)
@@ -503,12 +512,21 @@ __global__ __launch_bounds__(REV_THREADS) void k_rev_runs(KnnArgs a, RevPlan r) 
   }
   if (b + 1 == r.P && threadIdx.x == 0) a.pr.ov_off[a.N] = hi;
   __syncthreads();
-  for (uint32_t p = lo + threadIdx.x; p < hi; p += REV_THREADS) {
-    const uint32_t l = pv[p] - v0;
-    if (l < R) {
-      const uint32_t q = atomicAdd(&c[l], 1u);
-      if (lo + q < hi) a.pr.ov_sorted[lo + q] = pkey[p];
+  for (uint64_t p0 = lo; p0 < hi; p0 += REV_THREADS * RUN_U) {
+    uint32_t l[RUN_U];
+    uint64_t key[RUN_U];
+#pragma unroll
+    for (uint32_t k = 0; k < RUN_U; ++k) {
+      const uint64_t p = p0 + k * REV_THREADS + threadIdx.x;
+      l[k] = p < hi ? pv[p] - v0 : R;
+      key[k] = p < hi ? pkey[p] : 0ull;
     }
+#pragma unroll
+    for (uint32_t k = 0; k < RUN_U; ++k)
+      if (l[k] < R) {
+        const uint32_t q = atomicAdd(&c[l[k]], 1u);
+        if (lo + q < hi) a.pr.ov_sorted[lo + q] = key[k];
+      }
   }
 }
 
