@@ -374,9 +374,17 @@ __global__ __launch_bounds__(REV_THREADS) void k_rev_hist(KnnArgs a, RevPlan r) 
   for (uint32_t b = threadIdx.x; b < r.P; b += REV_THREADS) h[b] = 0;
   __syncthreads();
   const uint64_t e0 = uint64_t(t) * r.T, e1 = min(r.n, e0 + r.T);
-  for (uint64_t e = e0 + threadIdx.x; e < e1; e += REV_THREADS) {
-    const uint32_t v = a.G[e];
-    if (v < a.N) atomicAdd(&h[v >> r.lgR], 1u);
+  constexpr int U = 8;  // loads per thread issued before any is used
+  for (uint64_t c0 = e0; c0 < e1; c0 += uint64_t(REV_THREADS) * U) {
+    uint32_t v[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const uint64_t e = c0 + uint64_t(k) * REV_THREADS + threadIdx.x;
+      v[k] = e < e1 ? a.G[e] : TG_INV;
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k)
+      if (v[k] < a.N) atomicAdd(&h[v[k] >> r.lgR], 1u);
   }
   __syncthreads();
   for (uint32_t b = threadIdx.x; b < r.P; b += REV_THREADS) r.H[size_t(b) * r.tiles + t] = h[b];
