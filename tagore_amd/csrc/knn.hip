@@ -391,6 +391,7 @@ __global__ __launch_bounds__(REV_THREADS) void k_rev_hist(KnnArgs a, RevPlan r) 
 }
 
 constexpr int REV_CH = 4096;  // edges per pass of k_rev_part
+static_assert(REV_CH % REV_THREADS == 0, "a pass is a whole number of edges per thread");
 
 // Exclusive scan in place of x[0..n) (n <= REV_MAXP) by a REV_THREADS-thread workgroup, each
 // thread owning a run of consecutive entries; returns the total (in every thread).
